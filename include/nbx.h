@@ -1,0 +1,155 @@
+/*
+ * nbx — C ABI of the MI355X (gfx950) N-body force engine.
+ *
+ * Plain pointers, sizes and status codes; no C++ or torch types. Every entry
+ * point returns NBX_OK (0) or a non-zero status; nbx_last_error() returns the
+ * calling thread's message for the last failure. Device pointers are HIP
+ * device memory; a stream argument is a hipStream_t passed as void* (NULL is
+ * the null stream). Calls are asynchronous on their stream unless the comment
+ * says "synchronous".
+ *
+ * Each declaration names the reference interface it replaces
+ * (build-workbench/nbody-gpu-sim, paths relative to its root).
+ */
+#ifndef NBX_H
+#define NBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  NBX_OK = 0,
+  NBX_ERR_INVALID = 1,  /* bad argument (maps to ValidationException) */
+  NBX_ERR_HIP = 2,      /* HIP runtime failure (maps to the GPU exception) */
+  NBX_ERR_NOMEM = 3,    /* device or host allocation failed (ResourceException) */
+  NBX_ERR_CAPACITY = 4, /* input larger than the object was created for */
+  NBX_ERR_STATE = 5     /* call order violated (e.g. forces before build) */
+};
+
+typedef void* nbx_stream; /* hipStream_t */
+
+/* Layout-identical to nbody::ParticleData (include/nbody/types.hpp:96-131):
+ * 13 FP32 SoA arrays + count. Pointers are device pointers unless a function
+ * says host. */
+typedef struct nbx_particles {
+  float* pos_x;
+  float* pos_y;
+  float* pos_z;
+  float* vel_x;
+  float* vel_y;
+  float* vel_z;
+  float* acc_x;
+  float* acc_y;
+  float* acc_z;
+  float* acc_old_x;
+  float* acc_old_y;
+  float* acc_old_z;
+  float* mass;
+  size_t count;
+} nbx_particles;
+
+/* ------------------------------------------------------------ runtime */
+const char* nbx_status_string(int status);
+const char* nbx_last_error(void);
+/* Device 0 properties (replaces the cudaGetDeviceProperties probe of
+ * src/utils/error_handling.cpp:26-45). Synchronous. */
+int nbx_device_info(char* name, size_t name_len, int* cu_count, int* max_clock_khz,
+                    size_t* total_mem);
+int nbx_malloc(void** ptr, size_t bytes);
+int nbx_free(void* ptr);
+int nbx_memcpy_htod(void* dst, const void* src, size_t bytes);
+int nbx_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int nbx_memcpy_dtod(void* dst, const void* src, size_t bytes);
+int nbx_synchronize(void);
+int nbx_stream_create(nbx_stream* out);
+int nbx_stream_destroy(nbx_stream s);
+int nbx_stream_synchronize(nbx_stream s);
+/* HIP events for timing kernels on the stream they run on. */
+int nbx_event_create(void** ev);
+int nbx_event_destroy(void* ev);
+int nbx_event_record(void* ev, nbx_stream s);
+int nbx_event_elapsed_ms(void* start, void* stop, float* ms); /* synchronous */
+
+/* ParticleDataManager (include/nbody/particle_data.hpp:9-28, src/cuda/particle_init.cu:11-160).
+ * alloc: 13 device arrays of count floats, acc and acc_old zeroed (particle_init.cu:11-45).
+ * free: releases non-null arrays and zeroes the struct (:47-75).
+ * upload/download: pos, vel, acc, mass — not acc_old (:134-160). Synchronous. */
+int nbx_particles_alloc(nbx_particles* p, size_t count);
+int nbx_particles_free(nbx_particles* p);
+int nbx_particles_upload(nbx_particles* dev, const nbx_particles* host);
+int nbx_particles_download(nbx_particles* host, const nbx_particles* dev);
+
+/* ------------------------------------------------------------ direct N^2 */
+/* launchDirectForceKernel(ParticleData*, float G, float eps2, int block_size)
+ * (src/cuda/force_direct.cu:88-98): acc_i = G sum_{j!=i} m_j r_ij (|r_ij|^2+eps2)^{-3/2}.
+ * block_size is validated (1..1024) as the reference launch would; the tile
+ * shape is chosen by the autotuner (nbx_direct_set_variant overrides it). */
+int nbx_direct_forces(nbx_particles* d, float G, float eps2, int block_size, nbx_stream s);
+/* Forces for a contiguous block of targets [i0, i0+ni) against all j (sample
+ * checks and the ensemble's standalone comparison). acc written at i0..i0+ni-1. */
+int nbx_direct_forces_range(nbx_particles* d, size_t i0, size_t ni, float G, float eps2,
+                            nbx_stream s);
+/* One whole Velocity Verlet step with direct forces (Integrator::integrate,
+ * src/cuda/integrator.cu:175-189): store+drift fused, then the force kernel
+ * with the kick fused into its epilogue. Same arithmetic as the four separate
+ * reference kernels. */
+int nbx_direct_step(nbx_particles* d, float G, float eps2, float dt, nbx_stream s);
+/* Second half of nbx_direct_step: the force kernel with the kick
+ * v += (a_old + a_new)*(0.5f*dt) in its epilogue (after nbx_verlet_drift). */
+int nbx_direct_forces_kick(nbx_particles* d, float G, float eps2, float dt, nbx_stream s);
+
+/* Tile variants of the direct kernel (i per lane, waves splitting j). */
+int nbx_direct_variant_count(void);
+int nbx_direct_variant_name(int v, char* buf, size_t len);
+/* -1 restores the autotuned choice. */
+int nbx_direct_set_variant(int v);
+/* Time every variant on a synthetic input of n bodies; remember the fastest for
+ * n's size class (used by later calls). Synchronous. best_ms may be NULL. */
+int nbx_direct_autotune(size_t n, int reps, int* best_variant, double* best_ms);
+
+/* ------------------------------------------------------------ integrator */
+/* storeAccelerationsKernel (integrator.cu:40-49). */
+int nbx_store_old_accelerations(nbx_particles* d, nbx_stream s);
+/* updatePositionsKernel (integrator.cu:12-22): x += v*dt + a*(0.5f*dt*dt). */
+int nbx_update_positions(nbx_particles* d, float dt, nbx_stream s);
+/* updateVelocitiesKernel (integrator.cu:25-37): v += (a_old + a_new)*(0.5f*dt). */
+int nbx_update_velocities(nbx_particles* d, float dt, nbx_stream s);
+/* storeAccelerations + updatePositions fused in one launch. */
+int nbx_verlet_drift(nbx_particles* d, float dt, nbx_stream s);
+
+/* Energies and momentum with FP64 reductions (replaces Integrator::computeKineticEnergy /
+ * computePotentialEnergy, integrator.cu:203-240, whose sums are FP32). PE uses the
+ * softened pair potential of integrator.cu:82-104 with eps2 = eps*eps. Synchronous.
+ * Any output pointer may be NULL to skip that quantity (PE is O(N^2)). */
+int nbx_energy(const nbx_particles* d, float G, float eps2, double* ke, double* pe,
+               double momentum[3], nbx_stream s);
+
+/* ------------------------------------------------------------ input generators */
+/* Deterministic synthetic inputs written to HOST arrays (float), generated in
+ * double from SplitMix64 streams and rounded once. Portable across compilers,
+ * unlike std::uniform_real_distribution. Masses are M/n. */
+/* Plummer sphere (G = 1): inverse-CDF radii truncated at enclosed-mass fraction
+ * mass_cut (<= 0 means 0.999), von Neumann speeds on q^2 (1-q^2)^{7/2}
+ * (Aarseth, Henon & Wielen 1974), isotropic directions, centre of mass and
+ * mean velocity removed. */
+int nbx_gen_plummer(size_t n, uint64_t seed, double a, double M, double mass_cut, float* x,
+                    float* y, float* z, float* vx, float* vy, float* vz, float* m);
+/* Uniform-density sphere of radius R, masses M/n, Gaussian velocities (sigma per axis). */
+int nbx_gen_uniform_sphere(size_t n, uint64_t seed, double R, double M, double sigma, float* x,
+                           float* y, float* z, float* vx, float* vy, float* vz, float* m);
+/* Two equal Plummer spheres of n/2 each at x = -+sep/2 approaching with relative
+ * speed vrel along x; each sphere has mass M_each. n must be even. */
+int nbx_gen_two_plummer(size_t n, uint64_t seed, double a, double M_each, double sep,
+                        double vrel, float* x, float* y, float* z, float* vx, float* vy,
+                        float* vz, float* m);
+/* nsys sizes floor(exp(U(ln nmin, ln nmax))), clamped to [nmin, nmax]. */
+int nbx_gen_ensemble_sizes(int nsys, uint64_t seed, int64_t nmin, int64_t nmax, int64_t* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBX_H */

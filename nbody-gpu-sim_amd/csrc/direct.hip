@@ -1,0 +1,229 @@
+// Direct N^2 force kernels, launch dispatch over tile variants, and the
+// in-project autotuner. Replaces launchDirectForceKernel / DirectForceCalculator
+// (src/cuda/force_direct.cu:88-106).
+#include <cmath>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "direct_core.hpp"
+#include "verlet.hpp"
+
+namespace nbx {
+
+template <int IPL, int WS, bool CHECKED, bool KICK>
+__global__ __launch_bounds__(64 * WS) void direct_forces_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    const float* __restrict__ m, int n, int i0, int iend, double G, float eps2, float* ax,
+    float* ay, float* az, KickArgs kick) {
+  __shared__ double red[WS * 3 * 64 * IPL];
+  const int ibase = i0 + (int)blockIdx.x * 64 * IPL;
+  direct_tile<IPL, WS, CHECKED, KICK>(x, y, z, m, n, ibase, iend, G, eps2, ax, ay, az, kick, red);
+}
+
+struct DirectVariant {
+  const char* name;
+  int ipl;
+  int ws;
+  // launches over targets [i0, iend) of the system d
+  void (*launch)(const nbx_particles* d, int i0, int iend, double G, float eps2, bool checked,
+                 const KickArgs* kick, hipStream_t s);
+};
+
+template <int IPL, int WS>
+static void launch_variant(const nbx_particles* d, int i0, int iend, double G, float eps2,
+                           bool checked, const KickArgs* kick, hipStream_t s) {
+  const int n = (int)d->count;
+  const int tiles = (int)div_up((size_t)(iend - i0), 64 * IPL);
+  dim3 grid(tiles), block(64 * WS);
+  KickArgs k{};
+  if (kick) k = *kick;
+  if (checked) {
+    if (kick)
+      hipLaunchKernelGGL((direct_forces_kernel<IPL, WS, true, true>), grid, block, 0, s, d->pos_x,
+                         d->pos_y, d->pos_z, d->mass, n, i0, iend, G, eps2, d->acc_x, d->acc_y,
+                         d->acc_z, k);
+    else
+      hipLaunchKernelGGL((direct_forces_kernel<IPL, WS, true, false>), grid, block, 0, s,
+                         d->pos_x, d->pos_y, d->pos_z, d->mass, n, i0, iend, G, eps2, d->acc_x,
+                         d->acc_y, d->acc_z, k);
+  } else {
+    if (kick)
+      hipLaunchKernelGGL((direct_forces_kernel<IPL, WS, false, true>), grid, block, 0, s,
+                         d->pos_x, d->pos_y, d->pos_z, d->mass, n, i0, iend, G, eps2, d->acc_x,
+                         d->acc_y, d->acc_z, k);
+    else
+      hipLaunchKernelGGL((direct_forces_kernel<IPL, WS, false, false>), grid, block, 0, s,
+                         d->pos_x, d->pos_y, d->pos_z, d->mass, n, i0, iend, G, eps2, d->acc_x,
+                         d->acc_y, d->acc_z, k);
+  }
+}
+
+static const DirectVariant kVariants[] = {
+    {"ipl1_ws4", 1, 4, launch_variant<1, 4>},  {"ipl1_ws8", 1, 8, launch_variant<1, 8>},
+    {"ipl1_ws16", 1, 16, launch_variant<1, 16>}, {"ipl2_ws2", 2, 2, launch_variant<2, 2>},
+    {"ipl2_ws4", 2, 4, launch_variant<2, 4>},  {"ipl2_ws8", 2, 8, launch_variant<2, 8>},
+    {"ipl4_ws2", 4, 2, launch_variant<4, 2>},  {"ipl4_ws4", 4, 4, launch_variant<4, 4>},
+};
+constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+
+// Autotuned choice per size class (floor(log2 n)); -1 = not tuned.
+static std::mutex g_tune_mu;
+static int g_forced = -1;
+static int g_tuned[64];
+static bool g_tuned_init = false;
+
+static int size_class(size_t n) {
+  int c = 0;
+  while ((size_t(1) << (c + 1)) <= n && c < 62) ++c;
+  return c;
+}
+
+// Default when nothing was tuned: enough waves to fill 1024 SIMDs at ~8 waves each.
+static int default_variant(size_t n) {
+  if (n >= (size_t(1) << 20)) return 4;  // ipl2_ws4
+  if (n >= (size_t(1) << 17)) return 4;  // ipl2_ws4
+  if (n >= (size_t(1) << 15)) return 1;  // ipl1_ws8
+  return 2;                              // ipl1_ws16
+}
+
+int direct_pick_variant(size_t n) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tuned_init) {
+    for (int& t : g_tuned) t = -1;
+    g_tuned_init = true;
+  }
+  if (g_forced >= 0) return g_forced;
+  const int c = size_class(n);
+  if (g_tuned[c] >= 0) return g_tuned[c];
+  return default_variant(n);
+}
+
+int direct_launch(const nbx_particles* d, size_t i0, size_t ni, float G, float eps2,
+                  const KickArgs* kick, hipStream_t s, int variant) {
+  if (ni == 0) return NBX_OK;
+  if (!(eps2 >= 0.0f) || !std::isfinite(eps2))
+    return fail(NBX_ERR_INVALID, "softening eps2 must be finite and >= 0 (got %g)", (double)eps2);
+  const int v = variant >= 0 ? variant : direct_pick_variant(d->count);
+  const bool checked = eps2 < kUncheckedMinEps2;
+  kVariants[v].launch(d, (int)i0, (int)(i0 + ni), (double)G, eps2, checked, kick, s);
+  NBX_LAUNCHED();
+  return NBX_OK;
+}
+
+}  // namespace nbx
+
+using namespace nbx;
+
+extern "C" {
+
+int nbx_direct_forces(nbx_particles* d, float G, float eps2, int block_size, nbx_stream s) {
+  NBX_TRY(check_particles(d, "nbx_direct_forces"));
+  if (block_size < 1 || block_size > 1024)
+    return fail(NBX_ERR_INVALID, "block size must be between 1 and 1024 (got %d)", block_size);
+  return direct_launch(d, 0, d->count, G, eps2, nullptr, as_stream(s), -1);
+}
+
+int nbx_direct_forces_range(nbx_particles* d, size_t i0, size_t ni, float G, float eps2,
+                            nbx_stream s) {
+  NBX_TRY(check_particles(d, "nbx_direct_forces_range"));
+  if (i0 > d->count || ni > d->count - i0)
+    return fail(NBX_ERR_INVALID, "target range [%zu, %zu) outside [0, %zu)", i0, i0 + ni,
+                d->count);
+  return direct_launch(d, i0, ni, G, eps2, nullptr, as_stream(s), -1);
+}
+
+int nbx_direct_step(nbx_particles* d, float G, float eps2, float dt, nbx_stream s) {
+  NBX_TRY(check_particles(d, "nbx_direct_step"));
+  hipStream_t st = as_stream(s);
+  NBX_TRY(verlet_drift_launch(d, dt, true, st));
+  KickArgs k{d->vel_x, d->vel_y, d->vel_z, d->acc_old_x, d->acc_old_y, d->acc_old_z, 0.5f * dt};
+  return direct_launch(d, 0, d->count, G, eps2, &k, st, -1);
+}
+
+int nbx_direct_forces_kick(nbx_particles* d, float G, float eps2, float dt, nbx_stream s) {
+  NBX_TRY(check_particles(d, "nbx_direct_forces_kick"));
+  KickArgs k{d->vel_x, d->vel_y, d->vel_z, d->acc_old_x, d->acc_old_y, d->acc_old_z, 0.5f * dt};
+  return direct_launch(d, 0, d->count, G, eps2, &k, as_stream(s), -1);
+}
+
+int nbx_direct_variant_count(void) { return kNumVariants; }
+
+int nbx_direct_variant_name(int v, char* buf, size_t len) {
+  if (v < 0 || v >= kNumVariants || !buf || !len)
+    return fail(NBX_ERR_INVALID, "variant %d out of range", v);
+  snprintf(buf, len, "%s", kVariants[v].name);
+  return NBX_OK;
+}
+
+int nbx_direct_set_variant(int v) {
+  if (v < -1 || v >= kNumVariants) return fail(NBX_ERR_INVALID, "variant %d out of range", v);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_forced = v;
+  return NBX_OK;
+}
+
+int nbx_direct_autotune(size_t n, int reps, int* best_variant, double* best_ms) {
+  if (n == 0 || n > (size_t)INT32_MAX) return fail(NBX_ERR_INVALID, "autotune size %zu", n);
+  if (reps < 1) reps = 3;
+  // Synthetic bodies on a lattice-jittered cube; only timing matters here.
+  nbx_particles d;
+  NBX_TRY(nbx_particles_alloc(&d, n));
+  std::vector<float> h(n);
+  for (size_t i = 0; i < n; ++i) h[i] = (float)((i * 2654435761u) % 1000003u) * 1e-6f;
+  int rc = NBX_OK;
+  if ((rc = nbx_memcpy_htod(d.pos_x, h.data(), n * 4)) ||
+      (rc = nbx_memcpy_htod(d.pos_y, h.data(), n * 4)) ||
+      (rc = nbx_memcpy_htod(d.pos_z, h.data(), n * 4))) {
+    nbx_particles_free(&d);
+    return rc;
+  }
+  for (auto& v : h) v = 1.0f / (float)n;
+  if ((rc = nbx_memcpy_htod(d.mass, h.data(), n * 4))) {
+    nbx_particles_free(&d);
+    return rc;
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    nbx_particles_free(&d);
+    return fail(NBX_ERR_HIP, "hipEventCreate failed");
+  }
+  int best = -1;
+  float best_t = 1e30f;
+  for (int v = 0; v < kNumVariants && rc == NBX_OK; ++v) {
+    rc = direct_launch(&d, 0, n, 1.0f, 1e-4f, nullptr, nullptr, v);  // warm-up
+    if (rc) break;
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps && rc == NBX_OK; ++r)
+      rc = direct_launch(&d, 0, n, 1.0f, 1e-4f, nullptr, nullptr, v);
+    (void)hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(NBX_ERR_HIP, "autotune synchronisation failed");
+      break;
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)reps;
+    if (ms < best_t) {
+      best_t = ms;
+      best = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  nbx_particles_free(&d);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tuned_init) {
+      for (int& t : g_tuned) t = -1;
+      g_tuned_init = true;
+    }
+    g_tuned[size_class(n)] = best;
+  }
+  if (best_variant) *best_variant = best;
+  if (best_ms) *best_ms = best_t;
+  return NBX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,299 @@
+"""ctypes binding of libnbx.so (the C ABI declared in include/nbx.h).
+
+Host-side mirror used by the Python tests, bench.py and smoke(): it calls the
+HIP kernels through the same extern "C" entry points a C++ or other FFI caller
+would bind. There is no fallback: if libnbx.so is missing or fails to load,
+importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnbx.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libnbx.so not built at {LIB_PATH}; run __graft_entry__.build()")
+lib = C.CDLL(LIB_PATH)
+
+F32P = C.POINTER(C.c_float)
+I32P = C.POINTER(C.c_int32)
+I64P = C.POINTER(C.c_int64)
+U32P = C.POINTER(C.c_uint32)
+F64P = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+FIELDS = ["pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "acc_x", "acc_y", "acc_z",
+          "acc_old_x", "acc_old_y", "acc_old_z", "mass"]
+
+
+class Particles(C.Structure):
+    """nbx_particles == nbody::ParticleData (13 float pointers + size_t count)."""
+    _fields_ = [(f, F32P) for f in FIELDS] + [("count", C.c_size_t)]
+
+
+class DSParticles(C.Structure):
+    _fields_ = [("p", Particles), ("pos_lo_x", F32P), ("pos_lo_y", F32P), ("pos_lo_z", F32P)]
+
+
+class NbxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"nbx status {status}: {msg}")
+        self.status = status
+
+
+def _sig(name, restype, *argtypes):
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = list(argtypes)
+    return fn
+
+
+PP = C.POINTER(Particles)
+DSP = C.POINTER(DSParticles)
+_sig("nbx_status_string", C.c_char_p, C.c_int)
+_sig("nbx_last_error", C.c_char_p)
+_sig("nbx_device_info", C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int),
+     C.POINTER(C.c_size_t))
+_sig("nbx_malloc", C.c_int, C.POINTER(VP), C.c_size_t)
+_sig("nbx_free", C.c_int, VP)
+_sig("nbx_memcpy_htod", C.c_int, VP, VP, C.c_size_t)
+_sig("nbx_memcpy_dtoh", C.c_int, VP, VP, C.c_size_t)
+_sig("nbx_memcpy_dtod", C.c_int, VP, VP, C.c_size_t)
+_sig("nbx_synchronize", C.c_int)
+_sig("nbx_stream_create", C.c_int, C.POINTER(VP))
+_sig("nbx_stream_destroy", C.c_int, VP)
+_sig("nbx_stream_synchronize", C.c_int, VP)
+_sig("nbx_event_create", C.c_int, C.POINTER(VP))
+_sig("nbx_event_destroy", C.c_int, VP)
+_sig("nbx_event_record", C.c_int, VP, VP)
+_sig("nbx_event_elapsed_ms", C.c_int, VP, VP, C.POINTER(C.c_float))
+_sig("nbx_particles_alloc", C.c_int, PP, C.c_size_t)
+_sig("nbx_particles_free", C.c_int, PP)
+_sig("nbx_particles_upload", C.c_int, PP, PP)
+_sig("nbx_particles_download", C.c_int, PP, PP)
+_sig("nbx_direct_forces", C.c_int, PP, C.c_float, C.c_float, C.c_int, VP)
+_sig("nbx_direct_forces_range", C.c_int, PP, C.c_size_t, C.c_size_t, C.c_float, C.c_float, VP)
+_sig("nbx_direct_step", C.c_int, PP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_direct_forces_kick", C.c_int, PP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_direct_variant_count", C.c_int)
+_sig("nbx_direct_variant_name", C.c_int, C.c_int, C.c_char_p, C.c_size_t)
+_sig("nbx_direct_set_variant", C.c_int, C.c_int)
+_sig("nbx_direct_autotune", C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_int), F64P)
+_sig("nbx_store_old_accelerations", C.c_int, PP, VP)
+_sig("nbx_update_positions", C.c_int, PP, C.c_float, VP)
+_sig("nbx_update_velocities", C.c_int, PP, C.c_float, VP)
+_sig("nbx_verlet_drift", C.c_int, PP, C.c_float, VP)
+_sig("nbx_energy", C.c_int, PP, C.c_float, C.c_float, F64P, F64P, F64P, VP)
+_sig("nbx_gen_plummer", C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double, C.c_double,
+     *([F32P] * 7))
+_sig("nbx_gen_uniform_sphere", C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double,
+     C.c_double, *([F32P] * 7))
+_sig("nbx_gen_two_plummer", C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double, C.c_double,
+     C.c_double, *([F32P] * 7))
+_sig("nbx_gen_ensemble_sizes", C.c_int, C.c_int, C.c_uint64, C.c_int64, C.c_int64, I64P)
+
+EXPORTED = [
+    "nbx_status_string", "nbx_last_error", "nbx_device_info", "nbx_malloc", "nbx_free",
+    "nbx_memcpy_htod", "nbx_memcpy_dtoh", "nbx_memcpy_dtod", "nbx_synchronize",
+    "nbx_stream_create", "nbx_stream_destroy", "nbx_stream_synchronize", "nbx_event_create",
+    "nbx_event_destroy", "nbx_event_record", "nbx_event_elapsed_ms", "nbx_particles_alloc",
+    "nbx_particles_free", "nbx_particles_upload", "nbx_particles_download", "nbx_direct_forces",
+    "nbx_direct_forces_range", "nbx_direct_step", "nbx_direct_forces_kick",
+    "nbx_direct_variant_count",
+    "nbx_direct_variant_name", "nbx_direct_set_variant", "nbx_direct_autotune",
+    "nbx_store_old_accelerations", "nbx_update_positions", "nbx_update_velocities",
+    "nbx_verlet_drift", "nbx_energy", "nbx_gen_plummer", "nbx_gen_uniform_sphere",
+    "nbx_gen_two_plummer", "nbx_gen_ensemble_sizes",
+]
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise NbxError(status, lib.nbx_last_error().decode(errors="replace"))
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(F32P)
+
+
+# --------------------------------------------------------------- host data
+
+def empty_host(n: int) -> dict:
+    return {f: np.zeros(n, dtype=np.float32) for f in FIELDS}
+
+
+def host_struct(h: dict) -> Particles:
+    n = len(h["mass"])
+    p = Particles()
+    for f in FIELDS:
+        arr = h.setdefault(f, np.zeros(n, dtype=np.float32))
+        setattr(p, f, fptr(arr))
+    p.count = n
+    return p
+
+
+def _gen(fn, n, *args) -> dict:
+    h = empty_host(n)
+    outs = [fptr(h[f]) for f in ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "mass")]
+    st = fn(n, *args, *outs)
+    if st != 0:
+        raise NbxError(st, "generator rejected its arguments")
+    return h
+
+
+def gen_plummer(n: int, seed: int = 42, a: float = 1.0, M: float = 1.0,
+                mass_cut: float = 0.999) -> dict:
+    return _gen(lib.nbx_gen_plummer, n, seed, a, M, mass_cut)
+
+
+def gen_uniform_sphere(n: int, seed: int = 42, R: float = 1.0, M: float = 1.0,
+                       sigma: float = 0.05) -> dict:
+    return _gen(lib.nbx_gen_uniform_sphere, n, seed, R, M, sigma)
+
+
+def gen_two_plummer(n: int, seed: int = 42, a: float = 1.0, M_each: float = 1.0,
+                    sep: float = 10.0, vrel: float = 0.5) -> dict:
+    return _gen(lib.nbx_gen_two_plummer, n, seed, a, M_each, sep, vrel)
+
+
+def gen_ensemble_sizes(nsys: int, seed: int = 42, nmin: int = 256, nmax: int = 16384) -> np.ndarray:
+    out = np.zeros(nsys, dtype=np.int64)
+    st = lib.nbx_gen_ensemble_sizes(nsys, seed, nmin, nmax, out.ctypes.data_as(I64P))
+    if st != 0:
+        raise NbxError(st, "bad ensemble size arguments")
+    return out
+
+
+# --------------------------------------------------------------- device data
+
+class DeviceParticles:
+    """Owns an nbx_particles set in device memory (ParticleDataManager semantics)."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.s = Particles()
+        check(lib.nbx_particles_alloc(C.byref(self.s), n))
+
+    @classmethod
+    def from_host(cls, h: dict) -> "DeviceParticles":
+        d = cls(len(h["mass"]))
+        d.upload(h)
+        return d
+
+    def upload(self, h: dict) -> None:
+        hs = host_struct(h)
+        check(lib.nbx_particles_upload(C.byref(self.s), C.byref(hs)))
+
+    def download(self) -> dict:
+        h = empty_host(self.n)
+        hs = host_struct(h)
+        check(lib.nbx_particles_download(C.byref(hs), C.byref(self.s)))
+        return h
+
+    def read(self, field: str) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float32)
+        check(lib.nbx_memcpy_dtoh(out.ctypes.data, C.cast(getattr(self.s, field), VP), self.n * 4))
+        return out
+
+    def write(self, field: str, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        assert a.shape == (self.n,)
+        check(lib.nbx_memcpy_htod(C.cast(getattr(self.s, field), VP), a.ctypes.data, self.n * 4))
+
+    def acc(self) -> np.ndarray:
+        return np.stack([self.read("acc_x"), self.read("acc_y"), self.read("acc_z")], axis=1)
+
+    @property
+    def ptr(self):
+        return C.byref(self.s)
+
+    def free(self) -> None:
+        if self.s.count or self.s.pos_x:
+            check(lib.nbx_particles_free(C.byref(self.s)))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self.h = VP()
+        check(lib.nbx_stream_create(C.byref(self.h)))
+
+    def synchronize(self) -> None:
+        check(lib.nbx_stream_synchronize(self.h))
+
+    def __del__(self):
+        try:
+            lib.nbx_stream_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.h = VP()
+        check(lib.nbx_event_create(C.byref(self.h)))
+
+    def record(self, stream=None) -> None:
+        check(lib.nbx_event_record(self.h, stream.h if stream is not None else None))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = C.c_float()
+        check(lib.nbx_event_elapsed_ms(self.h, later.h, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib.nbx_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    check(lib.nbx_synchronize())
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(256)
+    cu, clk, mem = C.c_int(), C.c_int(), C.c_size_t()
+    check(lib.nbx_device_info(name, 256, C.byref(cu), C.byref(clk), C.byref(mem)))
+    return {"name": name.value.decode(), "cu": cu.value, "clock_khz": clk.value,
+            "mem": mem.value}
+
+
+def direct_forces(d: DeviceParticles, G: float, eps: float, block_size: int = 256,
+                  stream=None) -> None:
+    eps2 = np.float32(np.float32(eps) * np.float32(eps))
+    check(lib.nbx_direct_forces(d.ptr, G, float(eps2), block_size, stream))
+
+
+def direct_step(d: DeviceParticles, G: float, eps: float, dt: float, stream=None) -> None:
+    eps2 = np.float32(np.float32(eps) * np.float32(eps))
+    check(lib.nbx_direct_step(d.ptr, G, float(eps2), dt, stream))
+
+
+def energy(d: DeviceParticles, G: float, eps: float, potential: bool = True) -> dict:
+    eps2 = np.float32(np.float32(eps) * np.float32(eps))
+    ke, pe = C.c_double(), C.c_double()
+    mom = (C.c_double * 3)()
+    check(lib.nbx_energy(d.ptr, G, float(eps2), C.byref(ke), C.byref(pe) if potential else None,
+                         mom, None))
+    return {"ke": ke.value, "pe": pe.value if potential else None, "mom": list(mom)}
+
+
+def variant_names() -> list:
+    out = []
+    buf = C.create_string_buffer(64)
+    for v in range(lib.nbx_direct_variant_count()):
+        check(lib.nbx_direct_variant_name(v, buf, 64))
+        out.append(buf.value.decode())
+    return out
