@@ -128,6 +128,40 @@ int nbx_verlet_drift(nbx_particles* d, float dt, nbx_stream s);
 int nbx_energy(const nbx_particles* d, float G, float eps2, double* ke, double* pe,
                double momentum[3], nbx_stream s);
 
+/* ------------------------------------------------------------ Barnes-Hut */
+/* BarnesHutTree (include/nbody/barnes_hut_tree.hpp:33-86, src/cuda/force_barnes_hut.cu).
+ * Node layout exported by nbx_bh_copy_nodes is the 76-byte OctreeNode
+ * (barnes_hut_tree.hpp:9-30) in the reference's node numbering. */
+typedef struct nbx_bh nbx_bh;
+#define NBX_OCTREE_NODE_BYTES 76
+
+/* BarnesHutTree(size_t max_particles) (:252-258): node budget 2*max_particles. */
+int nbx_bh_create(size_t max_particles, nbx_bh** out);
+int nbx_bh_destroy(nbx_bh* t);
+/* BarnesHutTree::build (:330-336): bbox, 30-bit Morton codes, stable radix
+ * sort, octree build and centre of mass, all on the GPU. */
+int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream s);
+/* BarnesHutTree::computeForces(ParticleData*, theta, G, eps) (:545-555). */
+int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, nbx_stream s);
+/* getNodeCount / getMaxDepth (barnes_hut_tree.hpp:51-52). Synchronous. */
+int nbx_bh_node_count(nbx_bh* t, int32_t* node_count, int32_t* max_depth);
+/* copyNodesToHost (:557-560): node_count 76-byte nodes into host memory. Synchronous. */
+int nbx_bh_copy_nodes(nbx_bh* t, void* host_nodes, size_t capacity_nodes);
+/* Parity export (no reference counterpart): bbox {minx,miny,minz,maxx,maxy,maxz},
+ * sorted Morton codes and the sorted particle indices of the last build.
+ * Synchronous; any pointer may be NULL. */
+int nbx_bh_copy_sort(nbx_bh* t, float bbox[6], uint32_t* codes, int32_t* indices, size_t n);
+/* Build diagnostics: which builder ran (0 parallel, 1 sequential exact), the
+ * number of multi-particle leaves at depth 32, and per-phase times of the last
+ * build/forces in ms when timing is enabled (NULL to skip). Synchronous. */
+int nbx_bh_build_info(nbx_bh* t, int32_t* builder, int32_t* depth32_leaves);
+int nbx_bh_set_timing(nbx_bh* t, int enable);
+/* Builder selection for cross-checks: 0 = automatic (parallel unless the node
+ * budget could fire), 1 = always the exact sequential builder. */
+int nbx_bh_set_builder(nbx_bh* t, int mode);
+/* phases: bbox, morton, sort, paths, tree, com, export, forces (8 values, ms). */
+int nbx_bh_phase_ms(nbx_bh* t, double ms[8]);
+
 /* ------------------------------------------------------------ input generators */
 /* Deterministic synthetic inputs written to HOST arrays (float), generated in
  * double from SplitMix64 streams and rounded once. Portable across compilers,

@@ -1,0 +1,1224 @@
+// Barnes-Hut pipeline on the GPU, bit-exact with the reference's host build
+// (src/cuda/force_barnes_hut.cu). Every phase runs on the device and the
+// element counts of the level-synchronous build live in device memory, so a
+// build issues no host round trip.
+//
+//  1 bbox       exact per-axis min/max (:78-122, :266-306), then the Morton
+//               reciprocals (:313-320) and the root cell (:418-422) on device
+//  2 morton     30-bit codes of the per-axis normalised positions (:125-139)
+//  3 sort       stable LSD radix sort of (code, index)        (:324-328)
+//  4 paths      each particle's octant digit at depths 0..31, walking the
+//               float centre chain c +- h/2 exactly as allocateLeaf does (:375-379)
+//  5 tree       level-synchronous PR-octree build over the paths. The
+//               reference inserts particles one by one in sorted order; its
+//               node numbering follows from two ranks per cell: t1 = first
+//               inserted particle, t2 = second. A non-root cell X with parent
+//               P is created at insertion k(X) = max(t2(P), t1(X)); within one
+//               insertion the displaced particle's chain comes first by depth,
+//               then the new particle's leaf (t1 == k). Sorting the cells by
+//               (k, t1 == k, depth) reproduces the reference indices exactly.
+//  6 com        bottom-up, children in octant order, IEEE float ops (:512-538);
+//               depth-32 cells holding several particles merge them in
+//               insertion order (:355-365, :490-507)
+//  7 traverse   theta-criterion traversal with the reference's acceptance test,
+//               self-exclusion rule and DFS order (:142-249), one shared stack
+//               per wave (lanes carry a 64-bit "still needs this subtree" mask),
+//               so node data is read with wave-uniform scalar loads.
+//
+// When the reference's node budget could stop a split (node_count + 33 >=
+// 2*max_particles, :449-453) the tree depends on insertion order in ways the
+// parallel build does not model; then a single-thread kernel replays the
+// reference's insertion loop exactly. Both builders write the same 76-byte
+// node array.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "radix.hpp"
+
+namespace nbx {
+
+namespace {
+
+constexpr int kMaxDepth = 32;  // force_barnes_hut.cu:24
+constexpr int kStack = 256;    // force_barnes_hut.cu:28
+constexpr int kThreads = 256;
+
+// 76-byte node, byte layout of nbody::OctreeNode (barnes_hut_tree.hpp:9-30).
+struct RefNode {
+  float cx, cy, cz, half;
+  float comx, comy, comz, mass;
+  int32_t child[8];
+  int32_t pidx;
+  uint8_t leaf;
+  uint8_t pad[3];
+  int32_t count;
+};
+static_assert(sizeof(RefNode) == NBX_OCTREE_NODE_BYTES, "OctreeNode layout");
+
+// Device-resident bookkeeping of one build.
+struct BhState {
+  float bmin[3], bmax[3];
+  float inv[3];
+  float root_c[3];
+  float root_h;
+  int32_t n;
+  int32_t node_count;  // parallel builder: cells allocated
+  int32_t seg_count[2];
+  int32_t elem_count[2];
+  int32_t level_start[kMaxDepth + 2];
+  int32_t deepest;
+  int32_t multi_count;
+  int32_t member_count;
+  int32_t use_seq;
+  int32_t final_count;  // node_count_ of the reference
+  int32_t max_depth;    // max_depth_ of the reference
+  int32_t budget_merges;
+  int32_t depth_merges;
+  int32_t overflow;  // parallel builder ran past the node budget
+  int32_t sort_n;    // cells to number (0 when the sequential builder runs)
+};
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {  // :34-40
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t quantise(float t) {  // :45-47
+  const float c = fminf(fmaxf(t, 0.0f), 1.0f);
+  return (uint32_t)mul_rn(c, 1023.0f);
+}
+
+__device__ __forceinline__ int octant_of(float px, float py, float pz, float cx, float cy,
+                                         float cz) {  // :339-348
+  return (px >= cx ? 1 : 0) | (py >= cy ? 2 : 0) | (pz >= cz ? 4 : 0);
+}
+
+__device__ __forceinline__ int digit_at(const uint4* path, int r, int d) {
+  const uint4 p = path[r];
+  const int w = d / 10;
+  const uint32_t word = w == 0 ? p.x : (w == 1 ? p.y : (w == 2 ? p.z : p.w));
+  return (int)((word >> (3 * (d % 10))) & 7u);
+}
+
+// Exclusive block scan of one int per thread (kThreads threads); returns the
+// exclusive prefix and writes the block total.
+__device__ int block_exclusive_scan(int v, int* total) {
+  __shared__ int wsum[kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int q = 0; q < kThreads / 64; ++q) {
+    if (q < w) before += wsum[q];
+    tot += wsum[q];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + incl - v;
+}
+
+// ---------------------------------------------------------------- 1: bbox
+
+__global__ __launch_bounds__(kThreads) void bbox_partial(const float* __restrict__ x,
+                                                         const float* __restrict__ y,
+                                                         const float* __restrict__ z, int n,
+                                                         float* partial) {
+  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};  // :18-19
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+    const float v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], v[a]);
+      hi[a] = fmaxf(hi[a], v[a]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+    }
+  }
+  __shared__ float sh[6][kThreads / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) {
+      sh[a][w] = lo[a];
+      sh[3 + a][w] = hi[a];
+    }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    float v = sh[threadIdx.x][0];
+    for (int q = 1; q < kThreads / 64; ++q)
+      v = threadIdx.x < 3 ? fminf(v, sh[threadIdx.x][q]) : fmaxf(v, sh[threadIdx.x][q]);
+    partial[blockIdx.x * 6 + threadIdx.x] = v;
+  }
+}
+
+__global__ void bbox_final(const float* partial, int nparts, int n, BhState* st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+  for (int b = 0; b < nparts; ++b)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], partial[b * 6 + a]);
+      hi[a] = fmaxf(hi[a], partial[b * 6 + 3 + a]);
+    }
+  float ext[3];
+  for (int a = 0; a < 3; ++a) {
+    st->bmin[a] = lo[a];
+    st->bmax[a] = hi[a];
+    const float size = add_rn(sub_rn(hi[a], lo[a]), 0.001f);  // :313-315
+    st->inv[a] = div_rn(1.0f, size);                          // :319-320
+    st->root_c[a] = mul_rn(add_rn(lo[a], hi[a]), 0.5f);       // :418-419
+    ext[a] = sub_rn(hi[a], lo[a]);
+  }
+  float big = ext[0];  // std::max({...}) :420-421
+  if (ext[1] > big) big = ext[1];
+  if (ext[2] > big) big = ext[2];
+  st->root_h = add_rn(mul_rn(big, 0.5f), 0.001f);  // :422
+  st->n = n;
+}
+
+// ---------------------------------------------------------------- 2: morton
+
+__global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restrict__ x,
+                                                          const float* __restrict__ y,
+                                                          const float* __restrict__ z, int n,
+                                                          const BhState* __restrict__ st,
+                                                          uint32_t* codes, int32_t* idx) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const float nx = mul_rn(sub_rn(x[i], st->bmin[0]), st->inv[0]);
+  const float ny = mul_rn(sub_rn(y[i], st->bmin[1]), st->inv[1]);
+  const float nz = mul_rn(sub_rn(z[i], st->bmin[2]), st->inv[2]);
+  codes[i] = (expand_bits(quantise(nx)) << 2) | (expand_bits(quantise(ny)) << 1) |
+             expand_bits(quantise(nz));
+  idx[i] = i;
+}
+
+// ---------------------------------------------------------------- 4: paths
+
+__global__ __launch_bounds__(kThreads) void path_kernel(const float* __restrict__ x,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ z,
+                                                        const int32_t* __restrict__ sidx, int n,
+                                                        const BhState* __restrict__ st,
+                                                        uint4* path) {
+  const int r = blockIdx.x * kThreads + threadIdx.x;
+  if (r >= n) return;
+  const int p = sidx[r];
+  const float px = x[p], py = y[p], pz = z[p];
+  float cx = st->root_c[0], cy = st->root_c[1], cz = st->root_c[2], h = st->root_h;
+  uint32_t w[4] = {0, 0, 0, 0};
+  for (int d = 0; d < kMaxDepth; ++d) {
+    const int o = octant_of(px, py, pz, cx, cy, cz);
+    w[d / 10] |= (uint32_t)o << (3 * (d % 10));
+    const float hs = mul_rn(h, 0.5f);  // :375
+    cx = add_rn(cx, (o & 1) ? hs : -hs);
+    cy = add_rn(cy, (o & 2) ? hs : -hs);
+    cz = add_rn(cz, (o & 4) ? hs : -hs);
+    h = hs;
+  }
+  path[r] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// ---------------------------------------------------------------- 5: tree
+
+// Cells ("tmp nodes") of the parallel builder, indexed in allocation order.
+struct TmpNodes {
+  int32_t* t1;      // first inserted rank
+  int32_t* k;       // insertion step that creates the cell
+  int32_t* parent;
+  int32_t* count;   // particles in the cell
+  int32_t* pidx;    // leaf: particle index
+  uint32_t* info;   // depth | octant << 8 | leaf << 11
+  int32_t* child;   // [8] per cell
+  float4* box;      // centre, half size
+  float4* com;      // centre of mass, total mass
+  int32_t* member_base;  // depth-32 multi leaves: slot range start (else -1)
+  int32_t* member_fill;
+};
+
+struct Levels {
+  int32_t* seg_node[2];
+  int32_t* seg_t1[2];
+  int32_t* seg_t2[2];
+  int32_t* seg_cnt[2];   // [8] per segment
+  int32_t* seg_min[2];   // [8] per segment
+  int32_t* seg_next[2];  // [8] per segment: next-level segment, -1, or -(2+tmp leaf)
+  int32_t* elem_rank[2];
+  int32_t* elem_seg[2];
+  int32_t* multi_list;   // tmp ids of depth-32 multi leaves
+  int32_t* members;      // their member ranks
+};
+
+__global__ void tree_init(BhState* st, TmpNodes T, Levels L, int max_nodes) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int n = st->n;
+  T.t1[0] = 0;
+  T.k[0] = -1;
+  T.parent[0] = -1;
+  T.count[0] = n;
+  T.pidx[0] = -1;
+  T.info[0] = 0;
+  for (int o = 0; o < 8; ++o) T.child[o] = -1;
+  T.box[0] = make_float4(st->root_c[0], st->root_c[1], st->root_c[2], st->root_h);
+  T.com[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+  T.member_base[0] = -1;
+  st->node_count = 1;
+  st->seg_count[0] = 1;
+  st->seg_count[1] = 0;
+  st->elem_count[0] = n;
+  st->elem_count[1] = 0;
+  st->level_start[0] = 0;
+  st->deepest = 0;
+  st->multi_count = 0;
+  st->member_count = 0;
+  st->use_seq = 0;
+  st->overflow = 0;
+  st->sort_n = 0;
+  L.seg_node[0][0] = 0;
+  L.seg_t1[0][0] = 0;
+  L.seg_t2[0][0] = INT_MAX;
+  for (int o = 0; o < 8; ++o) {
+    L.seg_cnt[0][o] = 0;
+    L.seg_min[0][o] = INT_MAX;
+  }
+  (void)max_nodes;
+}
+
+// Level d, pass 1: count each segment's particles per octant, first rank per
+// octant, and the segment's second-smallest rank.
+__global__ __launch_bounds__(kThreads) void level_count(BhState* st, Levels L,
+                                                        const uint4* __restrict__ path, int d) {
+  const int cur = d & 1, nxt = cur ^ 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->level_start[d + 1] = st->node_count;
+    st->seg_count[nxt] = 0;
+    st->elem_count[nxt] = 0;
+  }
+  const int ne = st->elem_count[cur];
+  const int ns = st->overflow ? 0 : st->seg_count[cur];
+  if (ns == 0) return;
+  __shared__ int l_cnt[512], l_min[512], l_t2[64];
+  const bool local = ns <= 64;  // few segments: aggregate in LDS first
+  if (local) {
+    for (int k = threadIdx.x; k < 512; k += kThreads) {
+      l_cnt[k] = 0;
+      l_min[k] = INT_MAX;
+    }
+    for (int k = threadIdx.x; k < 64; k += kThreads) l_t2[k] = INT_MAX;
+    __syncthreads();
+  }
+  for (int e = blockIdx.x * kThreads + threadIdx.x; e < ne; e += gridDim.x * kThreads) {
+    const int r = d == 0 ? e : L.elem_rank[cur][e];
+    const int s = d == 0 ? 0 : L.elem_seg[cur][e];
+    const int o = digit_at(path, r, d);
+    const bool not_first = r != L.seg_t1[cur][s];
+    if (local) {
+      atomicAdd(&l_cnt[s * 8 + o], 1);
+      atomicMin(&l_min[s * 8 + o], r);
+      if (not_first) atomicMin(&l_t2[s], r);
+    } else {
+      atomicAdd(&L.seg_cnt[cur][s * 8 + o], 1);
+      atomicMin(&L.seg_min[cur][s * 8 + o], r);
+      if (not_first) atomicMin(&L.seg_t2[cur][s], r);
+    }
+  }
+  if (local) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < ns * 8; k += kThreads) {
+      if (l_cnt[k]) {
+        atomicAdd(&L.seg_cnt[cur][k], l_cnt[k]);
+        atomicMin(&L.seg_min[cur][k], l_min[k]);
+      }
+    }
+    for (int k = threadIdx.x; k < ns; k += kThreads)
+      if (l_t2[k] != INT_MAX) atomicMin(&L.seg_t2[cur][k], l_t2[k]);
+  }
+}
+
+// Level d, pass 2: create the children of every segment (cells at depth d+1).
+__global__ __launch_bounds__(kThreads) void level_children(BhState* st, Levels L, TmpNodes T,
+                                                           const int32_t* __restrict__ sidx,
+                                                           const float* __restrict__ x,
+                                                           const float* __restrict__ y,
+                                                           const float* __restrict__ z,
+                                                           const float* __restrict__ m, int d,
+                                                           int max_nodes) {
+  const int cur = d & 1, nxt = cur ^ 1;
+  const int ns = st->overflow ? 0 : st->seg_count[cur];
+  const int cd = d + 1;  // child depth
+  __shared__ int sh_node_base, sh_seg_base, sh_deep;
+  for (int base = blockIdx.x * kThreads; base < ns; base += gridDim.x * kThreads) {
+    const int s = base + threadIdx.x;
+    const bool valid = s < ns;
+    int cnt[8];
+    int nchild = 0, nseg = 0;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      cnt[o] = valid ? L.seg_cnt[cur][s * 8 + o] : 0;
+      nchild += cnt[o] > 0;
+      nseg += (cnt[o] >= 2 && cd < kMaxDepth);
+    }
+    int tot_child, tot_seg;
+    const int off_child = block_exclusive_scan(nchild, &tot_child);
+    const int off_seg = block_exclusive_scan(nseg, &tot_seg);
+    if (threadIdx.x == 0) {
+      sh_node_base = atomicAdd(&st->node_count, tot_child);
+      sh_seg_base = atomicAdd(&st->seg_count[nxt], tot_seg);
+      sh_deep = 0;
+    }
+    __syncthreads();
+    if (valid && nchild) {
+      const int X = L.seg_node[cur][s];
+      const int t2 = L.seg_t2[cur][s];
+      const float4 pb = T.box[X];
+      const float hs = mul_rn(pb.w, 0.5f);
+      int id = sh_node_base + off_child;
+      int sid = sh_seg_base + off_seg;
+      for (int o = 0; o < 8; ++o) {
+        const int c = cnt[o];
+        if (c == 0) {
+          L.seg_next[cur][s * 8 + o] = -1;
+          continue;
+        }
+        if (id >= max_nodes) {  // past the budget: the sequential builder takes over
+          L.seg_next[cur][s * 8 + o] = -1;
+          st->overflow = 1;
+          continue;
+        }
+        const int t1 = L.seg_min[cur][s * 8 + o];
+        const bool leaf = (c == 1) || (cd == kMaxDepth);
+        T.t1[id] = t1;
+        T.k[id] = d == 0 ? t1 : max(t2, t1);
+        T.parent[id] = X;
+        T.count[id] = c;
+        T.info[id] = (uint32_t)cd | ((uint32_t)o << 8) | (leaf ? (1u << 11) : 0u);
+        T.box[id] = make_float4(add_rn(pb.x, (o & 1) ? hs : -hs), add_rn(pb.y, (o & 2) ? hs : -hs),
+                                add_rn(pb.z, (o & 4) ? hs : -hs), hs);
+        for (int q = 0; q < 8; ++q) T.child[id * 8 + q] = -1;
+        T.child[X * 8 + o] = id;
+        T.member_base[id] = -1;
+        if (c == 1) {
+          const int p = sidx[t1];
+          T.pidx[id] = p;
+          T.com[id] = make_float4(x[p], y[p], z[p], m[p]);
+          L.seg_next[cur][s * 8 + o] = -1;
+        } else if (leaf) {  // depth-32 cell with several particles
+          T.pidx[id] = sidx[t1];
+          const int slot = atomicAdd(&st->multi_count, 1);
+          L.multi_list[slot] = id;
+          T.member_base[id] = atomicAdd(&st->member_count, c);
+          T.member_fill[id] = 0;
+          L.seg_next[cur][s * 8 + o] = -(2 + id);
+        } else {
+          T.pidx[id] = -1;
+          L.seg_node[nxt][sid] = id;
+          L.seg_t1[nxt][sid] = t1;
+          L.seg_t2[nxt][sid] = INT_MAX;
+          for (int q = 0; q < 8; ++q) {
+            L.seg_cnt[nxt][sid * 8 + q] = 0;
+            L.seg_min[nxt][sid * 8 + q] = INT_MAX;
+          }
+          L.seg_next[cur][s * 8 + o] = sid;
+          ++sid;
+        }
+        ++id;
+      }
+      sh_deep = cd;  // benign race: every writer stores the same value
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sh_deep) atomicMax(&st->deepest, sh_deep);
+    __syncthreads();
+  }
+}
+
+// Level d, pass 3: move each active particle to its child segment (or to its
+// depth-32 leaf's member list).
+__global__ __launch_bounds__(kThreads) void level_route(BhState* st, Levels L, TmpNodes T,
+                                                        const uint4* __restrict__ path, int d) {
+  const int cur = d & 1, nxt = cur ^ 1;
+  const int ne = st->overflow ? 0 : st->elem_count[cur];
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int base = blockIdx.x * kThreads; base < ne; base += gridDim.x * kThreads) {
+    const int e = base + threadIdx.x;
+    int r = 0, t = -1;
+    if (e < ne) {
+      r = d == 0 ? e : L.elem_rank[cur][e];
+      const int s = d == 0 ? 0 : L.elem_seg[cur][e];
+      t = L.seg_next[cur][s * 8 + digit_at(path, r, d)];
+    }
+    const uint64_t go = __ballot(t >= 0);
+    int wbase = 0;
+    if (go) {
+      if ((go & lt) == 0 && t >= 0) wbase = atomicAdd(&st->elem_count[nxt], __popcll(go));
+      wbase = __shfl(wbase, __ffsll((unsigned long long)go) - 1, 64);
+    }
+    if (t >= 0) {
+      const int pos = wbase + __popcll(go & lt);
+      L.elem_rank[nxt][pos] = r;
+      L.elem_seg[nxt][pos] = t;
+    } else if (t <= -2) {
+      const int leaf = -(t + 2);
+      const int slot = T.member_base[leaf] + atomicAdd(&T.member_fill[leaf], 1);
+      L.members[slot] = r;
+    }
+  }
+}
+
+__global__ void tree_levels_done(BhState* st, int max_nodes) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->level_start[kMaxDepth + 1] = st->node_count;
+  st->use_seq = (st->overflow || st->node_count + kMaxDepth + 1 >= max_nodes) ? 1 : 0;  // :449
+  st->sort_n = st->use_seq ? 0 : st->node_count;
+  st->final_count = st->node_count;
+  st->max_depth = st->multi_count > 0 ? kMaxDepth : st->deepest - 1;
+  st->budget_merges = 0;
+  st->depth_merges = 0;
+}
+
+// Depth-32 cells with several particles: merge them in insertion (rank) order.
+__global__ __launch_bounds__(64) void multi_leaf_merge(BhState* st, Levels L, TmpNodes T,
+                                                       const int32_t* __restrict__ sidx,
+                                                       const float* __restrict__ x,
+                                                       const float* __restrict__ y,
+                                                       const float* __restrict__ z,
+                                                       const float* __restrict__ m) {
+  if (st->use_seq) return;
+  const int nm = st->multi_count;
+  for (int q = blockIdx.x * 64 + threadIdx.x; q < nm; q += gridDim.x * 64) {
+    const int id = L.multi_list[q];
+    int32_t* mem = L.members + T.member_base[id];
+    const int c = T.count[id];
+    // heap sort of the member ranks (ascending)
+    for (int start = c / 2 - 1; start >= 0; --start) {
+      int root = start;
+      for (;;) {
+        int child = 2 * root + 1;
+        if (child >= c) break;
+        if (child + 1 < c && mem[child + 1] > mem[child]) ++child;
+        if (mem[root] >= mem[child]) break;
+        const int tmp = mem[root];
+        mem[root] = mem[child];
+        mem[child] = tmp;
+        root = child;
+      }
+    }
+    for (int end = c - 1; end > 0; --end) {
+      const int tmp = mem[0];
+      mem[0] = mem[end];
+      mem[end] = tmp;
+      int root = 0;
+      for (;;) {
+        int child = 2 * root + 1;
+        if (child >= end) break;
+        if (child + 1 < end && mem[child + 1] > mem[child]) ++child;
+        if (mem[root] >= mem[child]) break;
+        const int t2 = mem[root];
+        mem[root] = mem[child];
+        mem[child] = t2;
+        root = child;
+      }
+    }
+    int p = sidx[mem[0]];
+    float cx = x[p], cy = y[p], cz = z[p], tm = m[p];
+    for (int k = 1; k < c; ++k) {  // :355-365
+      p = sidx[mem[k]];
+      const float pm = m[p];
+      const float tot = add_rn(tm, pm);
+      cx = div_rn(add_rn(mul_rn(cx, tm), mul_rn(x[p], pm)), tot);
+      cy = div_rn(add_rn(mul_rn(cy, tm), mul_rn(y[p], pm)), tot);
+      cz = div_rn(add_rn(mul_rn(cz, tm), mul_rn(z[p], pm)), tot);
+      tm = tot;
+    }
+    T.com[id] = make_float4(cx, cy, cz, tm);
+  }
+}
+
+// Reference node-numbering key of every cell (root first).
+__global__ __launch_bounds__(kThreads) void node_keys(const BhState* __restrict__ st, TmpNodes T,
+                                                      uint32_t* keys, int32_t* vals) {
+  const int nn = st->sort_n;
+  for (int t = blockIdx.x * kThreads + threadIdx.x; t < nn; t += gridDim.x * kThreads) {
+    uint32_t key = 0;
+    if (t > 0) {
+      const uint32_t depth = T.info[t] & 0xFFu;
+      const uint32_t flag = T.t1[t] == T.k[t] ? 1u : 0u;
+      key = ((uint32_t)(T.k[t] + 1) << 7) | (flag << 6) | depth;
+    }
+    keys[t] = key;
+    vals[t] = t;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void invert_perm(const BhState* __restrict__ st,
+                                                        const int32_t* __restrict__ order,
+                                                        int32_t* final_of) {
+  const int nn = st->sort_n;
+  for (int f = blockIdx.x * kThreads + threadIdx.x; f < nn; f += gridDim.x * kThreads)
+    final_of[order[f]] = f;
+}
+
+// COM of internal cells at depth d from their children (:514-538).
+__global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict__ st, TmpNodes T,
+                                                      int d) {
+  if (st->use_seq) return;
+  const int b = st->level_start[d], e = st->level_start[d + 1];
+  for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
+    if (T.info[t] & (1u << 11)) continue;
+    float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int o = 0; o < 8; ++o) {
+      const int c = T.child[t * 8 + o];
+      if (c < 0) continue;
+      const float4 q = T.com[c];
+      tm = add_rn(tm, q.w);
+      sx = add_rn(sx, mul_rn(q.x, q.w));
+      sy = add_rn(sy, mul_rn(q.y, q.w));
+      sz = add_rn(sz, mul_rn(q.z, q.w));
+    }
+    float4 r = make_float4(0.f, 0.f, 0.f, tm);
+    if (tm > 0.0f) {
+      r.x = div_rn(sx, tm);
+      r.y = div_rn(sy, tm);
+      r.z = div_rn(sz, tm);
+    }
+    T.com[t] = r;
+  }
+}
+
+// Writes the reference-layout node array from the parallel builder's cells.
+__global__ __launch_bounds__(kThreads) void emit_nodes(const BhState* __restrict__ st, TmpNodes T,
+                                                       const int32_t* __restrict__ final_of,
+                                                       RefNode* out) {
+  const int nn = st->sort_n;
+  for (int t = blockIdx.x * kThreads + threadIdx.x; t < nn; t += gridDim.x * kThreads) {
+    RefNode nd;
+    const float4 b = T.box[t], c = T.com[t];
+    nd.cx = b.x;
+    nd.cy = b.y;
+    nd.cz = b.z;
+    nd.half = b.w;
+    nd.comx = c.x;
+    nd.comy = c.y;
+    nd.comz = c.z;
+    nd.mass = c.w;
+    for (int o = 0; o < 8; ++o) {
+      const int ch = T.child[t * 8 + o];
+      nd.child[o] = ch >= 0 ? final_of[ch] : -1;
+    }
+    const bool leaf = (T.info[t] >> 11) & 1u;
+    nd.pidx = leaf ? T.pidx[t] : -1;
+    nd.leaf = leaf ? 1 : 0;
+    nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
+    nd.count = T.count[t];
+    out[final_of[t]] = nd;
+  }
+}
+
+// ------------------------------------------------- exact sequential builder
+
+__device__ void seq_default(RefNode* nd) {
+  nd->cx = nd->cy = nd->cz = nd->half = 0.f;
+  nd->comx = nd->comy = nd->comz = nd->mass = 0.f;
+  for (int o = 0; o < 8; ++o) nd->child[o] = -1;
+  nd->pidx = -1;
+  nd->leaf = 1;
+  nd->pad[0] = nd->pad[1] = nd->pad[2] = 0;
+  nd->count = 0;
+}
+
+__device__ void seq_merge(RefNode* nd, int pid, float px, float py, float pz, float pm) {
+  const float tot = add_rn(nd->mass, pm);  // :355-365
+  const float cx = div_rn(add_rn(mul_rn(nd->comx, nd->mass), mul_rn(px, pm)), tot);
+  const float cy = div_rn(add_rn(mul_rn(nd->comy, nd->mass), mul_rn(py, pm)), tot);
+  const float cz = div_rn(add_rn(mul_rn(nd->comz, nd->mass), mul_rn(pz, pm)), tot);
+  nd->comx = cx;
+  nd->comy = cy;
+  nd->comz = cz;
+  nd->mass = tot;
+  nd->count += 1;
+  if (nd->pidx < 0) nd->pidx = pid;
+}
+
+__device__ int seq_leaf(RefNode* nodes, int* count, int max_nodes, int parent, int o, int pid,
+                        float px, float py, float pz, float pm) {  // :368-389
+  if (*count >= max_nodes) return -1;
+  const int id = (*count)++;
+  RefNode* p = &nodes[parent];
+  p->child[o] = id;
+  p->leaf = 0;
+  const float hs = mul_rn(p->half, 0.5f);
+  RefNode* c = &nodes[id];
+  c->cx = add_rn(p->cx, (o & 1) ? hs : -hs);
+  c->cy = add_rn(p->cy, (o & 2) ? hs : -hs);
+  c->cz = add_rn(p->cz, (o & 4) ? hs : -hs);
+  c->half = hs;
+  c->leaf = 1;
+  c->pad[0] = c->pad[1] = c->pad[2] = 0;
+  c->pidx = pid;
+  c->count = 1;
+  c->mass = pm;
+  c->comx = px;
+  c->comy = py;
+  c->comz = pz;
+  for (int q = 0; q < 8; ++q) c->child[q] = -1;
+  return id;
+}
+
+// One thread replays force_barnes_hut.cu:391-538 (only when the budget may fire).
+__global__ void seq_build(BhState* st, const int32_t* __restrict__ sidx, const float* __restrict__ x,
+                          const float* __restrict__ y, const float* __restrict__ z,
+                          const float* __restrict__ m, RefNode* nodes, int max_nodes) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || !st->use_seq) return;
+  const int n = st->n;
+  RefNode* root = &nodes[0];
+  seq_default(root);
+  root->cx = st->root_c[0];
+  root->cy = st->root_c[1];
+  root->cz = st->root_c[2];
+  root->half = st->root_h;
+  root->leaf = 0;
+  root->count = n;
+  int count = 1, deepest = 0, nb = 0, nd = 0;
+  for (int r = 0; r < n; ++r) {
+    const int pid = sidx[r];
+    const float px = x[pid], py = y[pid], pz = z[pid], pm = m[pid];
+    int cur = 0, depth = 0;
+    bool merged = false, overflow = false;
+    while (depth < kMaxDepth) {
+      RefNode* q = &nodes[cur];
+      if (q->leaf) {
+        if (count + kMaxDepth + 1 >= max_nodes) {
+          seq_merge(q, pid, px, py, pz, pm);
+          merged = true;
+          ++nb;
+          break;
+        }
+        const int old = q->pidx;
+        const float ox = q->comx, oy = q->comy, oz = q->comz, om = q->mass;
+        q->leaf = 0;
+        q->pidx = -1;
+        q->count = 0;
+        q->mass = 0.f;
+        q->comx = q->comy = q->comz = 0.f;
+        const int oo = octant_of(ox, oy, oz, q->cx, q->cy, q->cz);
+        if (q->child[oo] < 0 && seq_leaf(nodes, &count, max_nodes, cur, oo, old, ox, oy, oz, om) < 0) {
+          overflow = true;
+          break;
+        }
+      }
+      const int o = octant_of(px, py, pz, nodes[cur].cx, nodes[cur].cy, nodes[cur].cz);
+      const int ch = nodes[cur].child[o];
+      if (ch < 0) {
+        if (seq_leaf(nodes, &count, max_nodes, cur, o, pid, px, py, pz, pm) < 0) overflow = true;
+        break;
+      }
+      cur = ch;
+      ++depth;
+    }
+    if (overflow) break;  // the reference would write out of bounds here
+    if (!merged && depth >= kMaxDepth) {
+      int leaf = cur;
+      for (int guard = 0; guard <= kMaxDepth && !nodes[leaf].leaf; ++guard) {
+        int next = leaf;
+        for (int c = 0; c < 8; ++c)
+          if (nodes[leaf].child[c] >= 0) {
+            next = nodes[leaf].child[c];
+            break;
+          }
+        if (next == leaf) break;
+        leaf = next;
+      }
+      seq_merge(&nodes[leaf], pid, px, py, pz, pm);
+      ++nd;
+    }
+    if (depth > deepest) deepest = depth;
+  }
+  for (int i = count - 1; i >= 0; --i) {
+    RefNode* p = &nodes[i];
+    if (p->leaf) continue;
+    float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    int cnt = 0;
+    for (int c = 0; c < 8; ++c) {
+      const int ci = p->child[c];
+      if (ci < 0) continue;
+      const RefNode* q = &nodes[ci];
+      tm = add_rn(tm, q->mass);
+      sx = add_rn(sx, mul_rn(q->comx, q->mass));
+      sy = add_rn(sy, mul_rn(q->comy, q->mass));
+      sz = add_rn(sz, mul_rn(q->comz, q->mass));
+      cnt += q->count;
+    }
+    p->mass = tm;
+    p->count = cnt;
+    if (tm > 0.f) {
+      p->comx = div_rn(sx, tm);
+      p->comy = div_rn(sy, tm);
+      p->comz = div_rn(sz, tm);
+    }
+  }
+  st->final_count = count;
+  st->max_depth = deepest;
+  st->budget_merges = nb;
+  st->depth_merges = nd;
+}
+
+// Compact traversal layout from the reference-layout nodes.
+__global__ __launch_bounds__(kThreads) void pack_traversal(const BhState* __restrict__ st,
+                                                           const RefNode* __restrict__ nodes,
+                                                           float4* com4, float4* box4,
+                                                           int4* child4) {
+  const int nn = st->final_count;
+  for (int f = blockIdx.x * kThreads + threadIdx.x; f < nn; f += gridDim.x * kThreads) {
+    const RefNode& q = nodes[f];
+    com4[f] = make_float4(q.comx, q.comy, q.comz, q.mass);
+    // half size carries the leaf flag in its sign bit (half >= 0 always)
+    box4[f] = make_float4(q.cx, q.cy, q.cz, q.leaf ? -q.half - 0.0f : q.half);
+    child4[2 * f] = make_int4(q.child[0], q.child[1], q.child[2], q.child[3]);
+    child4[2 * f + 1] = make_int4(q.child[4], q.child[5], q.child[6], q.child[7]);
+  }
+}
+
+// ---------------------------------------------------------------- 7: traversal
+
+__global__ __launch_bounds__(kThreads) void bh_traverse(
+    const float4* __restrict__ com4, const float4* __restrict__ box4,
+    const int4* __restrict__ child4, const BhState* __restrict__ st,
+    const int32_t* __restrict__ order, const float* __restrict__ x, const float* __restrict__ y,
+    const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
+    float eps2, float* ax, float* ay, float* az) {
+  __shared__ int32_t st_node[kThreads / 64][kStack];
+  __shared__ uint64_t st_mask[kThreads / 64][kStack];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int r = blockIdx.x * kThreads + threadIdx.x;
+  const bool valid = r < n;
+  const int i = valid ? order[r] : 0;
+  const float xi = x[i], yi = y[i], zi = z[i], mi = m[i];
+  const int nn = st->final_count;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  const uint64_t me = 1ull << lane;
+  const uint64_t live = __ballot(valid);
+  if (lane == 0) {
+    st_node[w][0] = 0;
+    st_mask[w][0] = live;
+  }
+  int sp = 1;
+  int budget = nn + 1;  // every node is pushed at most once per wave
+  while (sp > 0 && budget-- > 0) {
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    const int node = __builtin_amdgcn_readfirstlane(st_node[w][sp]);
+    const uint64_t mk = st_mask[w][sp];
+    const uint32_t mlo = __builtin_amdgcn_readfirstlane((uint32_t)mk);
+    const uint32_t mhi = __builtin_amdgcn_readfirstlane((uint32_t)(mk >> 32));
+    const uint64_t mask = ((uint64_t)mhi << 32) | mlo;
+    if (node < 0 || node >= nn) continue;
+    const float4 c4 = com4[node];
+    if (c4.w == 0.0f) continue;  // :172-173
+    const float4 b4 = box4[node];
+    const bool leaf = __builtin_signbit(b4.w);
+    const float h = fabsf(b4.w);
+    bool open = false;
+    if (mask & me) {
+      const bool contains =
+          fabsf(xi - b4.x) <= h && fabsf(yi - b4.y) <= h && fabsf(zi - b4.z) <= h;  // :180-182
+      float meff = c4.w, cx = c4.x, cy = c4.y, cz = c4.z;
+      bool interact = true;
+      if (leaf) {
+        if (contains) {  // :190-200
+          meff = sub_rn(c4.w, mi);
+          if (meff <= 0.0f) {
+            interact = false;
+          } else {
+            cx = div_rn(sub_rn(mul_rn(c4.x, c4.w), mul_rn(xi, mi)), meff);
+            cy = div_rn(sub_rn(mul_rn(c4.y, c4.w), mul_rn(yi, mi)), meff);
+            cz = div_rn(sub_rn(mul_rn(c4.z, c4.w), mul_rn(zi, mi)), meff);
+          }
+        }
+      }
+      const float dx = sub_rn(cx, xi), dy = sub_rn(cy, yi), dz = sub_rn(cz, zi);
+      const float d2 =
+          add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
+      if (!leaf) {  // :218-224
+        const float size2 = mul_rn(mul_rn(4.0f, h), h);
+        if (contains || !(div_rn(size2, d2) < theta2)) {
+          interact = false;
+          open = true;
+        }
+      }
+      if (interact) {
+        const float inv = __builtin_amdgcn_rsqf(d2);
+        const float f = mul_rn(G, meff) * (inv * inv * inv);
+        sx = __builtin_fmaf(f, dx, sx);
+        sy = __builtin_fmaf(f, dy, sy);
+        sz = __builtin_fmaf(f, dz, sz);
+      }
+    }
+    const uint64_t om = __ballot(open);
+    if (om != 0 && sp + 8 <= kStack) {  // :234-240, children 0..7 pushed in order
+      const int4 c0 = child4[2 * node], c1 = child4[2 * node + 1];
+      const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (ch[c] >= 0) {
+          if (lane == 0) {
+            st_node[w][sp] = ch[c];
+            st_mask[w][sp] = om;
+          }
+          ++sp;
+        }
+      }
+    }
+  }
+  if (valid) {
+    ax[i] = sx;
+    ay[i] = sy;
+    az[i] = sz;
+  }
+}
+
+}  // namespace
+
+}  // namespace nbx
+
+// ---------------------------------------------------------------- object
+
+struct nbx_bh {
+  size_t max_particles = 0;
+  int max_nodes = 0;
+  nbx::BhState* st = nullptr;
+  float* bbox_part = nullptr;
+  int bbox_blocks = 0;
+  uint32_t* codes = nullptr;
+  int32_t* idx = nullptr;
+  uint4* path = nullptr;
+  nbx::TmpNodes T{};
+  nbx::Levels L{};
+  uint32_t* nkeys = nullptr;
+  int32_t* nvals = nullptr;
+  int32_t* final_of = nullptr;
+  nbx::RefNode* nodes = nullptr;
+  float4* com4 = nullptr;
+  float4* box4 = nullptr;
+  int4* child4 = nullptr;
+  nbx::RadixSort sort_particles;
+  nbx::RadixSort sort_nodes;
+  bool built = false;
+  size_t built_n = 0;
+  int force_seq = 0;
+  bool timing = false;
+  hipEvent_t ev[10] = {};
+  double phase_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<void*> allocs;
+};
+
+namespace nbx {
+namespace {
+
+template <typename T>
+int dalloc(nbx_bh* t, T** p, size_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(NBX_ERR_NOMEM, "Barnes-Hut buffers: allocation of %zu bytes failed",
+                sizeof(T) * count);
+  }
+  t->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return NBX_OK;
+}
+
+void bh_release(nbx_bh* t) {
+  for (void* p : t->allocs) (void)hipFree(p);
+  t->allocs.clear();
+  t->sort_particles.release();
+  t->sort_nodes.release();
+  for (auto& e : t->ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+int grid_for(size_t n, int cap = 2048) {
+  return (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, cap));
+}
+
+int key_bits_for(size_t n) {
+  int b = 0;
+  while ((size_t(1) << b) <= n) ++b;  // k + 1 <= n
+  return b + 7;
+}
+
+void mark(nbx_bh* t, int k, hipStream_t s) {
+  if (t->timing) (void)hipEventRecord(t->ev[k], s);
+}
+
+}  // namespace
+}  // namespace nbx
+
+using namespace nbx;
+
+extern "C" {
+
+int nbx_bh_create(size_t max_particles, nbx_bh** out) {
+  if (!out) return fail(NBX_ERR_INVALID, "nbx_bh_create: null output");
+  *out = nullptr;
+  if (max_particles == 0) return fail(NBX_ERR_INVALID, "Barnes-Hut tree needs max_particles > 0");
+  if (key_bits_for(max_particles) > 32)
+    return fail(NBX_ERR_INVALID, "Barnes-Hut tree supports at most 2^25-1 particles (got %zu)",
+                max_particles);
+  nbx_bh* t = new (std::nothrow) nbx_bh();
+  if (!t) return fail(NBX_ERR_NOMEM, "nbx_bh_create: host allocation");
+  const size_t P = max_particles;
+  const size_t NN = 2 * P;  // max_nodes_ = 2 * max_particles (:253)
+  t->max_particles = P;
+  t->max_nodes = (int)NN;
+  t->bbox_blocks = grid_for(P, 1024);
+  int rc = NBX_OK;
+  auto A = [&](auto** p, size_t c) {
+    if (rc == NBX_OK) rc = dalloc(t, p, c);
+  };
+  A(&t->st, 1);
+  A(&t->bbox_part, (size_t)t->bbox_blocks * 6);
+  A(&t->codes, P);
+  A(&t->idx, P);
+  A(&t->path, P);
+  A(&t->T.t1, NN);
+  A(&t->T.k, NN);
+  A(&t->T.parent, NN);
+  A(&t->T.count, NN);
+  A(&t->T.pidx, NN);
+  A(&t->T.info, NN);
+  A(&t->T.child, NN * 8);
+  A(&t->T.box, NN);
+  A(&t->T.com, NN);
+  A(&t->T.member_base, NN);
+  A(&t->T.member_fill, NN);
+  const size_t segs = P / 2 + 1;
+  for (int b = 0; b < 2; ++b) {
+    A(&t->L.seg_node[b], segs);
+    A(&t->L.seg_t1[b], segs);
+    A(&t->L.seg_t2[b], segs);
+    A(&t->L.seg_cnt[b], segs * 8);
+    A(&t->L.seg_min[b], segs * 8);
+    A(&t->L.seg_next[b], segs * 8);
+    A(&t->L.elem_rank[b], P);
+    A(&t->L.elem_seg[b], P);
+  }
+  A(&t->L.multi_list, P);
+  A(&t->L.members, P);
+  A(&t->nkeys, NN);
+  A(&t->nvals, NN);
+  A(&t->final_of, NN);
+  A(&t->nodes, NN);
+  A(&t->com4, NN);
+  A(&t->box4, NN);
+  A(&t->child4, NN * 2);
+  if (rc == NBX_OK) rc = t->sort_particles.init((int)P);
+  if (rc == NBX_OK) rc = t->sort_nodes.init((int)NN);
+  for (auto& e : t->ev)
+    if (rc == NBX_OK && hipEventCreate(&e) != hipSuccess) rc = fail(NBX_ERR_HIP, "hipEventCreate");
+  if (rc != NBX_OK) {
+    bh_release(t);
+    delete t;
+    return rc;
+  }
+  *out = t;
+  return NBX_OK;
+}
+
+int nbx_bh_destroy(nbx_bh* t) {
+  if (!t) return NBX_OK;
+  bh_release(t);
+  delete t;
+  return NBX_OK;
+}
+
+int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
+  if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_build: null tree");
+  NBX_TRY(check_particles(d, "nbx_bh_build"));
+  if (d->count > t->max_particles)
+    return fail(NBX_ERR_CAPACITY, "tree built for %zu particles, got %zu", t->max_particles,
+                d->count);
+  hipStream_t s = as_stream(stream);
+  const int n = (int)d->count;
+  const int g = grid_for(n);
+  mark(t, 0, s);
+  const int bb = std::min(t->bbox_blocks, grid_for(n, 1024));
+  hipLaunchKernelGGL(bbox_partial, dim3(bb), dim3(kThreads), 0, s, d->pos_x, d->pos_y, d->pos_z,
+                     n, t->bbox_part);
+  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, s, t->bbox_part, bb, n, t->st);
+  NBX_LAUNCHED();
+  mark(t, 1, s);
+  hipLaunchKernelGGL(morton_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, d->pos_x,
+                     d->pos_y, d->pos_z, n, t->st, t->codes, t->idx);
+  NBX_LAUNCHED();
+  mark(t, 2, s);
+  NBX_TRY(t->sort_particles.sort(t->codes, t->idx, &t->st->n, 30, s));
+  mark(t, 3, s);
+  hipLaunchKernelGGL(path_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, d->pos_x,
+                     d->pos_y, d->pos_z, t->idx, n, t->st, t->path);
+  NBX_LAUNCHED();
+  mark(t, 4, s);
+  hipLaunchKernelGGL(tree_init, dim3(1), dim3(64), 0, s, t->st, t->T, t->L, t->max_nodes);
+  for (int lv = 0; lv < kMaxDepth; ++lv) {
+    hipLaunchKernelGGL(level_count, dim3(g), dim3(kThreads), 0, s, t->st, t->L, t->path, lv);
+    hipLaunchKernelGGL(level_children, dim3(grid_for((size_t)n / 2 + 1)), dim3(kThreads), 0, s,
+                       t->st, t->L, t->T, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, lv,
+                       t->max_nodes);
+    hipLaunchKernelGGL(level_route, dim3(g), dim3(kThreads), 0, s, t->st, t->L, t->T, t->path,
+                       lv);
+  }
+  hipLaunchKernelGGL(tree_levels_done, dim3(1), dim3(64), 0, s, t->st, t->max_nodes);
+  NBX_LAUNCHED();
+  if (t->force_seq) {
+    // diagnostic: route through the exact sequential builder
+    const int one = 1, zero = 0;
+    NBX_HIP(hipMemcpyAsync(&t->st->use_seq, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    NBX_HIP(hipMemcpyAsync(&t->st->sort_n, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    NBX_HIP(hipStreamSynchronize(s));
+  }
+  mark(t, 5, s);
+  const int gn = grid_for((size_t)t->max_nodes);
+  hipLaunchKernelGGL(multi_leaf_merge, dim3(grid_for(n / 2 + 1, 256)), dim3(64), 0, s, t->st,
+                     t->L, t->T, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass);
+  hipLaunchKernelGGL(node_keys, dim3(gn), dim3(kThreads), 0, s, t->st, t->T, t->nkeys, t->nvals);
+  NBX_LAUNCHED();
+  NBX_TRY(t->sort_nodes.sort(t->nkeys, t->nvals, &t->st->sort_n, key_bits_for(t->max_particles),
+                             s));
+  hipLaunchKernelGGL(invert_perm, dim3(gn), dim3(kThreads), 0, s, t->st, t->nvals, t->final_of);
+  for (int lv = kMaxDepth; lv >= 0; --lv)
+    hipLaunchKernelGGL(com_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv);
+  NBX_LAUNCHED();
+  mark(t, 6, s);
+  hipLaunchKernelGGL(emit_nodes, dim3(gn), dim3(kThreads), 0, s, t->st, t->T, t->final_of,
+                     t->nodes);
+  hipLaunchKernelGGL(seq_build, dim3(1), dim3(64), 0, s, t->st, t->idx, d->pos_x, d->pos_y,
+                     d->pos_z, d->mass, t->nodes, t->max_nodes);
+  hipLaunchKernelGGL(pack_traversal, dim3(gn), dim3(kThreads), 0, s, t->st, t->nodes, t->com4,
+                     t->box4, t->child4);
+  NBX_LAUNCHED();
+  mark(t, 7, s);
+  t->built = true;
+  t->built_n = d->count;
+  return NBX_OK;
+}
+
+int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, nbx_stream stream) {
+  if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_forces: null tree");
+  NBX_TRY(check_particles(d, "nbx_bh_forces"));
+  if (!t->built || t->built_n != d->count)
+    return fail(NBX_ERR_STATE, "nbx_bh_forces: build the tree for these particles first");
+  hipStream_t s = as_stream(stream);
+  const int n = (int)d->count;
+  mark(t, 8, s);
+  hipLaunchKernelGGL(bh_traverse, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->com4,
+                     t->box4, t->child4, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, n,
+                     theta * theta, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
+  NBX_LAUNCHED();
+  mark(t, 9, s);
+  return NBX_OK;
+}
+
+int nbx_bh_node_count(nbx_bh* t, int32_t* node_count, int32_t* max_depth) {
+  if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_node_count: null tree");
+  if (!t->built) {
+    if (node_count) *node_count = 0;
+    if (max_depth) *max_depth = 0;
+    return NBX_OK;
+  }
+  BhState h;
+  NBX_HIP(hipDeviceSynchronize());
+  NBX_HIP(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
+  if (node_count) *node_count = h.final_count;
+  if (max_depth) *max_depth = h.max_depth;
+  return NBX_OK;
+}
+
+int nbx_bh_copy_nodes(nbx_bh* t, void* host_nodes, size_t capacity_nodes) {
+  if (!t || !host_nodes) return fail(NBX_ERR_INVALID, "nbx_bh_copy_nodes: null argument");
+  int32_t cnt = 0;
+  NBX_TRY(nbx_bh_node_count(t, &cnt, nullptr));
+  if ((size_t)cnt > capacity_nodes)
+    return fail(NBX_ERR_CAPACITY, "%d nodes do not fit in %zu", cnt, capacity_nodes);
+  if (cnt) NBX_HIP(hipMemcpy(host_nodes, t->nodes, sizeof(RefNode) * cnt, hipMemcpyDeviceToHost));
+  return NBX_OK;
+}
+
+int nbx_bh_copy_sort(nbx_bh* t, float bbox[6], uint32_t* codes, int32_t* indices, size_t n) {
+  if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_copy_sort: null tree");
+  if (!t->built) return fail(NBX_ERR_STATE, "nbx_bh_copy_sort: no build yet");
+  if ((codes || indices) && n != t->built_n)
+    return fail(NBX_ERR_INVALID, "nbx_bh_copy_sort: n=%zu but the build had %zu", n, t->built_n);
+  NBX_HIP(hipDeviceSynchronize());
+  if (bbox) {
+    BhState h;
+    NBX_HIP(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
+    for (int a = 0; a < 3; ++a) {
+      bbox[a] = h.bmin[a];
+      bbox[3 + a] = h.bmax[a];
+    }
+  }
+  if (codes) NBX_HIP(hipMemcpy(codes, t->codes, 4 * n, hipMemcpyDeviceToHost));
+  if (indices) NBX_HIP(hipMemcpy(indices, t->idx, 4 * n, hipMemcpyDeviceToHost));
+  return NBX_OK;
+}
+
+int nbx_bh_build_info(nbx_bh* t, int32_t* builder, int32_t* depth32_leaves) {
+  if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_build_info: null tree");
+  BhState h;
+  memset(&h, 0, sizeof(h));
+  if (t->built) {
+    NBX_HIP(hipDeviceSynchronize());
+    NBX_HIP(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
+  }
+  if (builder) *builder = h.use_seq;
+  if (depth32_leaves) *depth32_leaves = h.use_seq ? h.depth_merges : h.multi_count;
+  return NBX_OK;
+}
+
+int nbx_bh_set_timing(nbx_bh* t, int enable) {
+  if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_set_timing: null tree");
+  t->timing = enable != 0;
+  return NBX_OK;
+}
+
+int nbx_bh_set_builder(nbx_bh* t, int mode) {
+  if (!t || mode < 0 || mode > 1) return fail(NBX_ERR_INVALID, "nbx_bh_set_builder: bad argument");
+  t->force_seq = mode;
+  return NBX_OK;
+}
+
+int nbx_bh_phase_ms(nbx_bh* t, double ms[8]) {
+  if (!t || !ms) return fail(NBX_ERR_INVALID, "nbx_bh_phase_ms: null argument");
+  if (!t->timing) return fail(NBX_ERR_STATE, "nbx_bh_phase_ms: timing is off");
+  NBX_HIP(hipDeviceSynchronize());
+  const int from[8] = {0, 1, 2, 3, 4, 5, 6, 8};
+  const int to[8] = {1, 2, 3, 4, 5, 6, 7, 9};
+  for (int k = 0; k < 8; ++k) {
+    float v = 0.f;
+    if (hipEventElapsedTime(&v, t->ev[from[k]], t->ev[to[k]]) != hipSuccess) {
+      (void)hipGetLastError();
+      v = -1.f;
+    }
+    ms[k] = v;
+  }
+  return NBX_OK;
+}
+
+}  // extern "C"

@@ -79,11 +79,11 @@ static int size_class(size_t n) {
   return c;
 }
 
-// Default when nothing was tuned: enough waves to fill 1024 SIMDs at ~8 waves each.
+// Default when nothing was tuned, from a sweep on MI355X (ipl = targets per
+// lane, ws = waves splitting the j range of one tile).
 static int default_variant(size_t n) {
-  if (n >= (size_t(1) << 20)) return 4;  // ipl2_ws4
-  if (n >= (size_t(1) << 17)) return 4;  // ipl2_ws4
-  if (n >= (size_t(1) << 15)) return 1;  // ipl1_ws8
+  if (n >= (size_t(1) << 17)) return 7;  // ipl4_ws4
+  if (n >= (size_t(1) << 15)) return 5;  // ipl2_ws8
   return 2;                              // ipl1_ws16
 }
 

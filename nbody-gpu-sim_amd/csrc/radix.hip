@@ -1,0 +1,180 @@
+// Stable LSD radix sort (see radix.hpp).
+#include "radix.hpp"
+
+namespace nbx {
+
+namespace {
+
+constexpr int kBins = 256;
+constexpr int kThreads = RadixSort::kWaves * 64;
+
+__global__ __launch_bounds__(kThreads) void radix_count(const uint32_t* __restrict__ keys,
+                                                        const int32_t* __restrict__ n_dev,
+                                                        int shift, uint32_t* table, int tiles) {
+  __shared__ uint32_t hist[kBins];
+  const int n = *n_dev;
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RadixSort::kTile;
+  for (int k = threadIdx.x; k < RadixSort::kTile; k += kThreads) {
+    const int i = base + k;
+    if (i < n) atomicAdd(&hist[(keys[i] >> shift) & 0xFFu], 1u);
+  }
+  __syncthreads();
+  table[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Exclusive scan of table[0 .. 256*tiles) in place, one workgroup of 1024.
+__global__ __launch_bounds__(1024) void radix_scan(uint32_t* table, int total) {
+  __shared__ uint32_t part[1024];
+  const int per = (total + 1023) / 1024;
+  const int b = threadIdx.x * per;
+  const int e = min(total, b + per);
+  uint32_t s = 0;
+  for (int i = b; i < e; ++i) s += table[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - s;  // exclusive prefix of this chunk
+  for (int i = b; i < e; ++i) {
+    const uint32_t v = table[i];
+    table[i] = run;
+    run += v;
+  }
+}
+
+__device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid, int nbits) {
+  uint64_t m = __ballot(valid);
+  for (int bit = 0; bit < nbits; ++bit) {
+    const bool b = (digit >> bit) & 1u;
+    const uint64_t v = __ballot(b);
+    m &= b ? v : ~v;
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(kThreads) void radix_scatter(
+    const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    int32_t* __restrict__ vout, const uint32_t* __restrict__ table, const int32_t* __restrict__ n_dev,
+    int shift, int nbits, int tiles) {
+  constexpr int R = RadixSort::kRounds;
+  constexpr int W = RadixSort::kWaves;
+  __shared__ uint32_t wave_hist[W][kBins];
+  const int n = *n_dev;
+  const int base = blockIdx.x * RadixSort::kTile;
+  if (base >= n) return;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  for (int k = threadIdx.x; k < W * kBins; k += kThreads) (&wave_hist[0][0])[k] = 0;
+
+  uint32_t key[R];
+  int32_t val[R];
+  const int wbase = base + w * R * 64;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = wbase + r * 64 + lane;
+    key[r] = i < n ? kin[i] : 0u;
+    val[r] = i < n ? vin[i] : 0;
+  }
+  const uint32_t dmask = (1u << nbits) - 1u;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  __syncthreads();
+  // pass 1: digit counts of this wave's sub-tile
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool valid = wbase + r * 64 + lane < n;
+    const uint32_t dg = (key[r] >> shift) & dmask;
+    const uint64_t m = match_digit(dg, valid, nbits);
+    if (valid && (m & lt) == 0) wave_hist[w][dg] += (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  // per-wave starting offsets: tile offset + earlier waves of the tile
+  {
+    const int d = threadIdx.x;  // kThreads == kBins
+    uint32_t run = table[d * tiles + blockIdx.x];
+    for (int q = 0; q < W; ++q) {
+      const uint32_t c = wave_hist[q][d];
+      wave_hist[q][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  // pass 2: stable scatter
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool valid = wbase + r * 64 + lane < n;
+    const uint32_t dg = (key[r] >> shift) & dmask;
+    const uint64_t m = match_digit(dg, valid, nbits);
+    const uint32_t pos = (valid ? wave_hist[w][dg] : 0u) + (uint32_t)__popcll(m & lt);
+    __builtin_amdgcn_wave_barrier();
+    if (valid) {
+      kout[pos] = key[r];
+      vout[pos] = val[r];
+      if ((m & lt) == 0) wave_hist[w][dg] += (uint32_t)__popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+}  // namespace
+
+int RadixSort::init(int n) {
+  release();
+  max_n = n < 1 ? 1 : n;
+  max_tiles = (max_n + kTile - 1) / kTile;
+  if (hipMalloc(&keys_tmp, sizeof(uint32_t) * max_n) != hipSuccess ||
+      hipMalloc(&vals_tmp, sizeof(int32_t) * max_n) != hipSuccess ||
+      hipMalloc(&table, sizeof(uint32_t) * kBins * max_tiles) != hipSuccess) {
+    (void)hipGetLastError();
+    release();
+    return fail(NBX_ERR_NOMEM, "radix sort buffers for %d keys", n);
+  }
+  return NBX_OK;
+}
+
+void RadixSort::release() {
+  if (keys_tmp) (void)hipFree(keys_tmp);
+  if (vals_tmp) (void)hipFree(vals_tmp);
+  if (table) (void)hipFree(table);
+  keys_tmp = nullptr;
+  vals_tmp = nullptr;
+  table = nullptr;
+}
+
+int RadixSort::sort(uint32_t* keys, int32_t* vals, const int32_t* n_dev, int key_bits,
+                    hipStream_t s) {
+  if (key_bits <= 0) return NBX_OK;
+  uint32_t* ka = keys;
+  int32_t* va = vals;
+  uint32_t* kb = keys_tmp;
+  int32_t* vb = vals_tmp;
+  int passes = 0;
+  for (int shift = 0; shift < key_bits; shift += 8) {
+    const int nbits = key_bits - shift < 8 ? key_bits - shift : 8;
+    hipLaunchKernelGGL(radix_count, dim3(max_tiles), dim3(kThreads), 0, s, ka, n_dev, shift, table,
+                       max_tiles);
+    hipLaunchKernelGGL(radix_scan, dim3(1), dim3(1024), 0, s, table, kBins * max_tiles);
+    hipLaunchKernelGGL(radix_scatter, dim3(max_tiles), dim3(kThreads), 0, s, ka, va, kb, vb, table,
+                       n_dev, shift, nbits, max_tiles);
+    NBX_LAUNCHED();
+    uint32_t* tk = ka;
+    ka = kb;
+    kb = tk;
+    int32_t* tv = va;
+    va = vb;
+    vb = tv;
+    ++passes;
+  }
+  if (passes & 1) {  // result sits in the scratch buffers: copy back
+    NBX_HIP(hipMemcpyAsync(keys, ka, sizeof(uint32_t) * max_n, hipMemcpyDeviceToDevice, s));
+    NBX_HIP(hipMemcpyAsync(vals, va, sizeof(int32_t) * max_n, hipMemcpyDeviceToDevice, s));
+  }
+  return NBX_OK;
+}
+
+}  // namespace nbx

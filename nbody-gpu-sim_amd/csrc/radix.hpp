@@ -1,0 +1,36 @@
+// Stable LSD radix sort of (uint32 key, int32 value) pairs on the GPU.
+//
+// Same result as a stable ascending sort by key (ties keep input order), the
+// semantics of thrust::sort_by_key used by the reference
+// (src/cuda/force_barnes_hut.cu:324-328). 8-bit digits; per pass:
+//   count   - per-tile digit histogram (LDS atomics)
+//   scan    - exclusive scan of the digit-major [256][tiles] table (one workgroup)
+//   scatter - in-tile stable ranking by wave ballots (match-any over the digit
+//             bits, rank = popcount of lower matching lanes), waves own
+//             contiguous sub-tiles so order is kept, then scattered writes.
+// The element count is read from device memory, so a sort can follow a kernel
+// that produced its length without a host round trip.
+#pragma once
+
+#include "common.hpp"
+
+namespace nbx {
+
+struct RadixSort {
+  static constexpr int kWaves = 4;
+  static constexpr int kRounds = 16;
+  static constexpr int kTile = kWaves * kRounds * 64;  // keys per workgroup
+  int max_n = 0;
+  int max_tiles = 0;
+  uint32_t* keys_tmp = nullptr;
+  int32_t* vals_tmp = nullptr;
+  uint32_t* table = nullptr;  // [256][max_tiles]
+
+  int init(int max_n);
+  void release();
+  // Sorts keys/vals in place (result back in keys/vals) over bits [0, key_bits).
+  // n_dev: device pointer holding the element count (<= max_n).
+  int sort(uint32_t* keys, int32_t* vals, const int32_t* n_dev, int key_bits, hipStream_t s);
+};
+
+}  // namespace nbx

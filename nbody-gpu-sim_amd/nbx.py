@@ -39,6 +39,15 @@ class DSParticles(C.Structure):
     _fields_ = [("p", Particles), ("pos_lo_x", F32P), ("pos_lo_y", F32P), ("pos_lo_z", F32P)]
 
 
+# 76-byte node, layout of nbody::OctreeNode (include/nbody/barnes_hut_tree.hpp:9-30)
+NODE_DTYPE = np.dtype([
+    ("center", "<f4", 3), ("half_size", "<f4"), ("com", "<f4", 3), ("total_mass", "<f4"),
+    ("children", "<i4", 8), ("particle_index", "<i4"), ("is_leaf", "u1"), ("pad", "u1", 3),
+    ("particle_count", "<i4"),
+])
+assert NODE_DTYPE.itemsize == 76
+
+
 class NbxError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"nbx status {status}: {msg}")
@@ -88,6 +97,17 @@ _sig("nbx_update_positions", C.c_int, PP, C.c_float, VP)
 _sig("nbx_update_velocities", C.c_int, PP, C.c_float, VP)
 _sig("nbx_verlet_drift", C.c_int, PP, C.c_float, VP)
 _sig("nbx_energy", C.c_int, PP, C.c_float, C.c_float, F64P, F64P, F64P, VP)
+_sig("nbx_bh_create", C.c_int, C.c_size_t, C.POINTER(VP))
+_sig("nbx_bh_destroy", C.c_int, VP)
+_sig("nbx_bh_build", C.c_int, VP, PP, VP)
+_sig("nbx_bh_forces", C.c_int, VP, PP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_bh_node_count", C.c_int, VP, I32P, I32P)
+_sig("nbx_bh_copy_nodes", C.c_int, VP, VP, C.c_size_t)
+_sig("nbx_bh_copy_sort", C.c_int, VP, F32P, U32P, I32P, C.c_size_t)
+_sig("nbx_bh_build_info", C.c_int, VP, I32P, I32P)
+_sig("nbx_bh_set_timing", C.c_int, VP, C.c_int)
+_sig("nbx_bh_set_builder", C.c_int, VP, C.c_int)
+_sig("nbx_bh_phase_ms", C.c_int, VP, F64P)
 _sig("nbx_gen_plummer", C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double, C.c_double,
      *([F32P] * 7))
 _sig("nbx_gen_uniform_sphere", C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double,
@@ -106,7 +126,10 @@ EXPORTED = [
     "nbx_direct_variant_count",
     "nbx_direct_variant_name", "nbx_direct_set_variant", "nbx_direct_autotune",
     "nbx_store_old_accelerations", "nbx_update_positions", "nbx_update_velocities",
-    "nbx_verlet_drift", "nbx_energy", "nbx_gen_plummer", "nbx_gen_uniform_sphere",
+    "nbx_verlet_drift", "nbx_energy", "nbx_bh_create", "nbx_bh_destroy", "nbx_bh_build",
+    "nbx_bh_forces", "nbx_bh_node_count", "nbx_bh_copy_nodes", "nbx_bh_copy_sort",
+    "nbx_bh_build_info", "nbx_bh_set_timing", "nbx_bh_set_builder", "nbx_bh_phase_ms",
+    "nbx_gen_plummer", "nbx_gen_uniform_sphere",
     "nbx_gen_two_plummer", "nbx_gen_ensemble_sizes",
 ]
 
@@ -297,3 +320,62 @@ def variant_names() -> list:
         check(lib.nbx_direct_variant_name(v, buf, 64))
         out.append(buf.value.decode())
     return out
+
+
+BH_PHASES = ("bbox", "morton", "sort", "paths", "tree", "com", "export", "forces")
+
+
+class BHTree:
+    """nbx_bh handle: BarnesHutTree(max_particles) with a 2*max_particles node budget."""
+
+    def __init__(self, max_particles: int):
+        self.h = VP()
+        check(lib.nbx_bh_create(max_particles, C.byref(self.h)))
+
+    def build(self, d: DeviceParticles, stream=None) -> None:
+        check(lib.nbx_bh_build(self.h, d.ptr, stream.h if stream is not None else None))
+
+    def forces(self, d: DeviceParticles, theta: float, G: float, eps: float, stream=None) -> None:
+        check(lib.nbx_bh_forces(self.h, d.ptr, theta, G, eps,
+                                stream.h if stream is not None else None))
+
+    def node_count(self):
+        c, dep = C.c_int32(), C.c_int32()
+        check(lib.nbx_bh_node_count(self.h, C.byref(c), C.byref(dep)))
+        return c.value, dep.value
+
+    def nodes(self) -> np.ndarray:
+        cnt, _ = self.node_count()
+        out = np.zeros(max(cnt, 1), NODE_DTYPE)
+        check(lib.nbx_bh_copy_nodes(self.h, out.ctypes.data, len(out)))
+        return out[:cnt]
+
+    def sort_result(self, n: int):
+        bbox = np.zeros(6, np.float32)
+        codes = np.zeros(n, np.uint32)
+        idx = np.zeros(n, np.int32)
+        check(lib.nbx_bh_copy_sort(self.h, fptr(bbox), codes.ctypes.data_as(U32P),
+                                   idx.ctypes.data_as(I32P), n))
+        return bbox[:3], bbox[3:], codes, idx
+
+    def build_info(self):
+        b, m = C.c_int32(), C.c_int32()
+        check(lib.nbx_bh_build_info(self.h, C.byref(b), C.byref(m)))
+        return {"builder": "sequential" if b.value else "parallel", "depth32_leaves": m.value}
+
+    def set_timing(self, on: bool) -> None:
+        check(lib.nbx_bh_set_timing(self.h, 1 if on else 0))
+
+    def set_builder(self, mode: int) -> None:
+        check(lib.nbx_bh_set_builder(self.h, mode))
+
+    def phase_ms(self) -> dict:
+        ms = (C.c_double * 8)()
+        check(lib.nbx_bh_phase_ms(self.h, ms))
+        return dict(zip(BH_PHASES, list(ms)))
+
+    def __del__(self):
+        try:
+            lib.nbx_bh_destroy(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,166 @@
+"""GPU parity of the Barnes-Hut pipeline against the CPU oracle.
+
+Bit-exact: bounding box, Morton codes, the stable sort permutation, and the
+whole 76-byte node array (numbering, centres, half sizes, centres of mass,
+masses, children, particle indices, leaf flags, counts), node count and max
+depth. Forces: the traversal makes the same accept/open decisions as the
+oracle (same IEEE arithmetic), so accelerations differ only by v_rsq_f32 vs
+1/sqrtf and FMA accumulation: |a_gpu - a_orc| / sum|terms| <= 2e-6 per particle.
+"""
+import numpy as np
+import pytest
+
+import bh_model as M
+from test_bh_oracle import CASES, tree_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_build(nbx, h, capacity=None, builder=0):
+    n = len(h["mass"])
+    d = nbx.DeviceParticles.from_host(h)
+    t = nbx.BHTree(capacity or n)
+    t.set_builder(builder)
+    t.build(d)
+    return d, t
+
+
+def check_exact(nbx, orc, h, capacity=None, builder=0):
+    n = len(h["mass"])
+    d, t = gpu_build(nbx, h, capacity, builder)
+    ref = orc.bh_pipeline(h, capacity)
+    lo, hi, codes, idx = t.sort_result(n)
+    assert np.array_equal(lo, ref["bbox_min"]) and np.array_equal(hi, ref["bbox_max"])
+    assert np.array_equal(codes, ref["codes"]), "Morton codes / sort order differ"
+    assert np.array_equal(idx, ref["sorted_idx"]), "sort permutation differs"
+    cnt, depth = t.node_count()
+    assert cnt == ref["node_count"] and depth == ref["max_depth"], (cnt, depth, ref["node_count"], ref["max_depth"])
+    ok, why = M.same_nodes(t.nodes(), ref["nodes"])
+    assert ok, why
+    return d, t, ref
+
+
+@pytest.mark.parametrize("kind,n", CASES)
+def test_tree_bit_exact(nbx, orc, kind, n):
+    _, t, ref = check_exact(nbx, orc, tree_inputs(nbx, kind, n))
+    info = t.build_info()
+    expect_seq = ref["node_count"] + 33 >= 2 * n
+    assert info["builder"] == ("sequential" if expect_seq else "parallel"), info
+
+
+def test_parallel_builder_handles_depth32_cells(nbx, orc):
+    h = tree_inputs(nbx, "pairs", 4000)
+    _, t, ref = check_exact(nbx, orc, h)
+    assert ref["depth_merges"] == 6 and ref["max_depth"] == 32
+    info = t.build_info()
+    assert info["builder"] == "parallel" and info["depth32_leaves"] == 2
+
+
+def test_sequential_builder_agrees_with_parallel(nbx, orc):
+    h = nbx.gen_plummer(5000, seed=12)
+    _, tp = gpu_build(nbx, h)
+    _, ts = gpu_build(nbx, h, builder=1)
+    assert tp.build_info()["builder"] == "parallel"
+    assert ts.build_info()["builder"] == "sequential"
+    ok, why = M.same_nodes(tp.nodes(), ts.nodes())
+    assert ok, why
+
+
+@pytest.mark.parametrize("n,capacity", [(700, 1000), (3000, 3000)])
+def test_capacity_larger_than_n(nbx, orc, n, capacity):
+    # the node budget is 2 * the capacity the tree was created with (:252-258)
+    check_exact(nbx, orc, nbx.gen_plummer(n, seed=n), capacity=capacity)
+
+
+def test_two_plummer_65536_exact(nbx, orc):
+    check_exact(nbx, orc, nbx.gen_two_plummer(65536, seed=42))
+
+
+def force_check(nbx, orc, h, theta, G, eps, targets=None):
+    d, t = gpu_build(nbx, h)
+    t.forces(d, theta, G, eps)
+    nbx.synchronize()
+    got = d.acc().astype(np.float64)
+    ref = orc.bh_pipeline(h)
+    n = len(h["mass"])
+    tg = np.arange(n) if targets is None else targets
+    a = orc.bh_forces(ref["nodes"], h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], theta, G, eps,
+                      targets=tg).astype(np.float64)
+    # normalise by the sum of |pair terms| bound: G * sum_j m_j / (r^2+eps^2) of the direct sum
+    p = np.stack([h["pos_x"], h["pos_y"], h["pos_z"]], 1).astype(np.float64)
+    m = h["mass"].astype(np.float64)
+    scale = np.array([G * np.sum(m * np.sqrt(((p - p[i]) ** 2).sum(1)) /
+                                 (((p - p[i]) ** 2).sum(1) + eps * eps) ** 1.5) for i in tg])
+    err = np.linalg.norm(got[tg] - a, axis=1) / scale
+    print(f"n={n} theta={theta}: max normalised |gpu - oracle| = {err.max():.2e}")
+    assert err.max() <= 2e-6
+    return got, d, t
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.3, 0.5, 1.0])
+def test_forces_match_oracle(nbx, orc, theta):
+    force_check(nbx, orc, nbx.gen_plummer(3000, seed=3), theta, 1.0, 0.01)
+
+
+def test_forces_degenerate_inputs(nbx, orc):
+    for kind, n in (("coincident", 16), ("collinear", 16), ("pairs", 400), ("plummer", 1)):
+        h = tree_inputs(nbx, kind, n)
+        d, t = gpu_build(nbx, h)
+        t.forces(d, 0.5, 1.0, 0.1)
+        nbx.synchronize()
+        got = d.acc()
+        ref = orc.bh_pipeline(h)
+        a = orc.bh_forces(ref["nodes"], h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 0.5, 1.0, 0.1)
+        assert np.all(np.isfinite(got))
+        assert np.allclose(got, a, rtol=1e-5, atol=1e-6), kind
+    # N=1: the particle's own leaf is skipped (m_eff = 0): zero acceleration
+    h = tree_inputs(nbx, "plummer", 1)
+    d, t = gpu_build(nbx, h)
+    t.forces(d, 0.5, 1.0, 0.1)
+    nbx.synchronize()
+    assert np.all(d.acc() == 0)
+
+
+def test_theta_zero_equals_direct(nbx, orc):
+    h = nbx.gen_plummer(2000, seed=8)
+    d, t = gpu_build(nbx, h)
+    t.forces(d, 0.0, 1.0, 0.01)
+    nbx.synchronize()
+    got = d.acc().astype(np.float64)
+    ref = orc.direct_f64(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 1.0, orc.eps2_of(0.01))
+    e = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(e) < 1e-6 and e.max() < 1e-4
+
+
+def test_rebuild_and_errors(nbx, orc):
+    h = nbx.gen_plummer(2000, seed=1)
+    d, t = gpu_build(nbx, h, capacity=2000)
+    h2 = nbx.gen_plummer(2000, seed=2)
+    d.upload(h2)
+    t.build(d)
+    ok, why = M.same_nodes(t.nodes(), orc.bh_pipeline(h2)["nodes"])
+    assert ok, why
+    big = nbx.DeviceParticles(2001)
+    with pytest.raises(nbx.NbxError):
+        t.build(big)
+    t2 = nbx.BHTree(100)
+    small = nbx.DeviceParticles.from_host(nbx.gen_plummer(100))
+    with pytest.raises(nbx.NbxError):
+        t2.forces(small, 0.5, 1.0, 0.01)
+
+
+def test_two_plummer_1m_exact_and_forces(nbx, orc):
+    # the 1,048,576-body collision workload: Morton codes, permutation and the
+    # full node array bit-exact; forces on 2,048 sample targets vs the oracle
+    h = nbx.gen_two_plummer(1 << 20, seed=42)
+    d, t, ref = check_exact(nbx, orc, h)
+    t.forces(d, 0.5, 1.0, 0.01)
+    nbx.synchronize()
+    got = d.acc().astype(np.float64)
+    tg = np.sort(np.random.default_rng(1).choice(1 << 20, 2048, replace=False))
+    a = orc.bh_forces(ref["nodes"], h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 0.5, 1.0, 0.01,
+                      targets=tg).astype(np.float64)
+    err = np.linalg.norm(got[tg] - a, axis=1) / np.linalg.norm(a, axis=1)
+    print(f"1M two-Plummer: nodes {ref['node_count']}, depth {ref['max_depth']}, "
+          f"force rel err median {np.median(err):.2e} max {err.max():.2e}")
+    assert np.median(err) < 1e-6 and err.max() < 1e-4
