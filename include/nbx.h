@@ -162,6 +162,54 @@ int nbx_bh_set_builder(nbx_bh* t, int mode);
 /* phases: bbox, morton, sort, paths, tree, com, export, forces (8 values, ms). */
 int nbx_bh_phase_ms(nbx_bh* t, double ms[8]);
 
+/* ------------------------------------------------------------ precision form */
+/* Double-single positions (hi + lo FP32 pairs), Kahan-compensated FP32 pair-force
+ * sums, FP64 energies. No reference counterpart (extension of the direct path);
+ * hi arrays are the nbx_particles pos arrays, lo arrays are extra. */
+typedef struct nbx_ds_particles {
+  nbx_particles p; /* pos_* hold the hi parts */
+  float* pos_lo_x;
+  float* pos_lo_y;
+  float* pos_lo_z;
+} nbx_ds_particles;
+int nbx_ds_alloc(nbx_ds_particles* p, size_t count); /* lo parts zeroed */
+int nbx_ds_free(nbx_ds_particles* p);
+int nbx_ds_forces(nbx_ds_particles* d, float G, float eps2, nbx_stream s);
+/* store+drift in double-single: (hi,lo) += v*dt + a*(0.5*dt*dt) with renormalisation. */
+int nbx_ds_drift(nbx_ds_particles* d, float dt, nbx_stream s);
+/* drift, forces, kick. */
+int nbx_ds_step(nbx_ds_particles* d, float G, float eps2, float dt, nbx_stream s);
+int nbx_ds_energy(const nbx_ds_particles* d, float G, float eps2, double* ke, double* pe,
+                  double momentum[3], nbx_stream s);
+
+/* ------------------------------------------------------------ batched ensemble */
+/* Many independent systems of uneven N, concatenated in one SoA set. Forces of
+ * every system in one persistent launch pulling (system, i-tile) items from a
+ * device work queue (largest first); Verlet steps captured in hipGraphs.
+ * No reference counterpart. */
+typedef struct nbx_ensemble nbx_ensemble;
+int nbx_ens_create(int nsys, const int64_t* counts, nbx_ensemble** out);
+int nbx_ens_destroy(nbx_ensemble* e);
+/* Concatenated particle arrays (device) and per-system offsets (host, nsys+1). */
+int nbx_ens_particles(nbx_ensemble* e, nbx_particles* view, int64_t* offsets);
+int nbx_ens_forces(nbx_ensemble* e, float G, float eps2, nbx_stream s);
+/* `steps` Verlet steps; when graph_chunk > 0 the steps are replayed from a
+ * hipGraph holding graph_chunk steps (steps must be a multiple). */
+int nbx_ens_steps(nbx_ensemble* e, float G, float eps2, float dt, int steps, int graph_chunk,
+                  nbx_stream s);
+/* Baseline for the batched form: the same `steps`, advancing each system on
+ * its own with the single-system entry points (nbx_direct_step per system). */
+int nbx_ens_steps_sequential(nbx_ensemble* e, float G, float eps2, float dt, int steps,
+                             nbx_stream s);
+/* Tile shapes: the same (targets per lane, waves per tile) list and order as the
+ * single-system variants (nbx_direct_variant_name). */
+int nbx_ens_variant_count(void);
+/* Choose the ensemble tile shape by timing (synchronous). */
+int nbx_ens_autotune(nbx_ensemble* e, int reps, int* best_variant, double* best_ms);
+int nbx_ens_set_variant(nbx_ensemble* e, int v);
+/* Per-system FP64 energies (host arrays of nsys). Synchronous. */
+int nbx_ens_energy(nbx_ensemble* e, float G, float eps2, double* ke, double* pe, nbx_stream s);
+
 /* ------------------------------------------------------------ input generators */
 /* Deterministic synthetic inputs written to HOST arrays (float), generated in
  * double from SplitMix64 streams and rounded once. Portable across compilers,

@@ -1,0 +1,269 @@
+// Precision form of the direct N^2 path (an extension; the reference has no
+// counterpart — its direct kernel is force_direct.cu:10-85 in plain FP32).
+//
+//  * positions are double-single: x = hi + lo with two FP32 arrays, so the
+//    drift x += v dt + a dt^2/2 does not lose the low bits of small
+//    increments over long runs;
+//  * pair forces stay FP32 (v_rsq_f32); r_ij = (x_j.hi - x_i.hi) + (x_j.lo - x_i.lo);
+//  * each target's sum is Kahan-compensated in FP32 over its whole j chunk;
+//  * FP64 appears only in the cross-wave reduction of the compensated sums, the
+//    hi/lo renormalisation of the drift, and the energy/momentum reductions.
+#include "common.hpp"
+#include "direct_core.hpp"
+#include "verlet.hpp"
+
+namespace nbx {
+
+namespace {
+
+constexpr int kDsBatch = 8;  // 7 scalar loads per body: keep a batch within the SGPR budget
+
+__device__ __forceinline__ double dadd_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a + b;
+}
+__device__ __forceinline__ double dmul_rn(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
+template <bool CHECKED>
+__device__ __forceinline__ void ds_pair(float bxh, float byh, float bzh, float bxl, float byl,
+                                        float bzl, float bm, int j, float xh, float yh, float zh,
+                                        float xl, float yl, float zl, int i, float eps2,
+                                        float (&s)[3], float (&c)[3]) {
+  const float dx = (bxh - xh) + (bxl - xl);
+  const float dy = (byh - yh) + (byl - yl);
+  const float dz = (bzh - zh) + (bzl - zl);
+  float r2 = __builtin_fmaf(dx, dx, eps2);
+  r2 = __builtin_fmaf(dy, dy, r2);
+  r2 = __builtin_fmaf(dz, dz, r2);
+  const float inv = __builtin_amdgcn_rsqf(r2);
+  float f = (bm * inv) * (inv * inv);
+  if (CHECKED) f = (j == i) ? 0.0f : f;
+  const float d[3] = {dx, dy, dz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {  // Kahan: s + (f d) with the lost low part kept in c
+    const float y = __builtin_fmaf(f, d[a], -c[a]);
+    const float t = s[a] + y;
+    c[a] = (t - s[a]) - y;
+    s[a] = t;
+  }
+}
+
+template <int IPL, int WS, bool CHECKED, bool KICK>
+__global__ __launch_bounds__(64 * WS) void ds_forces_kernel(
+    const float* __restrict__ xh, const float* __restrict__ yh, const float* __restrict__ zh,
+    const float* __restrict__ xl, const float* __restrict__ yl, const float* __restrict__ zl,
+    const float* __restrict__ m, int n, double G, float eps2, float* ax, float* ay, float* az,
+    KickArgs kick) {
+  __shared__ double red[WS * 3 * 64 * IPL];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ibase = (int)blockIdx.x * 64 * IPL;
+  float pxh[IPL], pyh[IPL], pzh[IPL], pxl[IPL], pyl[IPL], pzl[IPL];
+  int ii[IPL];
+#pragma unroll
+  for (int q = 0; q < IPL; ++q) {
+    const int i = ibase + q * 64 + lane;
+    ii[q] = i;
+    const int ic = i < n ? i : 0;
+    pxh[q] = xh[ic];
+    pyh[q] = yh[ic];
+    pzh[q] = zh[ic];
+    pxl[q] = xl[ic];
+    pyl[q] = yl[ic];
+    pzl[q] = zl[ic];
+  }
+  float s[IPL][3], c[IPL][3];
+#pragma unroll
+  for (int q = 0; q < IPL; ++q)
+    for (int a = 0; a < 3; ++a) s[q][a] = c[q][a] = 0.0f;
+  const int chunk = direct_chunk(n, WS);
+  const int j0 = min(n, wave * chunk);
+  const int j1 = min(n, j0 + chunk);
+  int j = j0;
+  for (; j + kDsBatch <= j1; j += kDsBatch) {
+#pragma unroll
+    for (int k = 0; k < kDsBatch; ++k) {
+      const int jj = j + k;
+      const float bxh = xh[jj], byh = yh[jj], bzh = zh[jj], bxl = xl[jj], byl = yl[jj],
+                  bzl = zl[jj], bm = m[jj];
+#pragma unroll
+      for (int q = 0; q < IPL; ++q)
+        ds_pair<CHECKED>(bxh, byh, bzh, bxl, byl, bzl, bm, jj, pxh[q], pyh[q], pzh[q], pxl[q],
+                         pyl[q], pzl[q], ii[q], eps2, s[q], c[q]);
+    }
+  }
+  for (; j < j1; ++j) {
+    const float bxh = xh[j], byh = yh[j], bzh = zh[j], bxl = xl[j], byl = yl[j], bzl = zl[j],
+                bm = m[j];
+#pragma unroll
+    for (int q = 0; q < IPL; ++q)
+      ds_pair<CHECKED>(bxh, byh, bzh, bxl, byl, bzl, bm, j, pxh[q], pyh[q], pzh[q], pxl[q],
+                       pyl[q], pzl[q], ii[q], eps2, s[q], c[q]);
+  }
+  constexpr int T = 64 * IPL;
+#pragma unroll
+  for (int q = 0; q < IPL; ++q) {
+    const int sl = q * 64 + lane;
+    for (int a = 0; a < 3; ++a) red[(wave * 3 + a) * T + sl] = (double)s[q][a] - (double)c[q][a];
+  }
+  __syncthreads();
+  for (int sl = threadIdx.x; sl < T; sl += 64 * WS) {
+    const int i = ibase + sl;
+    if (i >= n) continue;
+    double t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int w = 0; w < WS; ++w)
+      for (int a = 0; a < 3; ++a) t[a] += red[(w * 3 + a) * T + sl];
+    const float a0 = (float)(G * t[0]), a1 = (float)(G * t[1]), a2 = (float)(G * t[2]);
+    ax[i] = a0;
+    ay[i] = a1;
+    az[i] = a2;
+    if (KICK) {
+      kick.vx[i] = add_rn(kick.vx[i], mul_rn(add_rn(kick.aox[i], a0), kick.dt_half));
+      kick.vy[i] = add_rn(kick.vy[i], mul_rn(add_rn(kick.aoy[i], a1), kick.dt_half));
+      kick.vz[i] = add_rn(kick.vz[i], mul_rn(add_rn(kick.aoz[i], a2), kick.dt_half));
+    }
+  }
+}
+
+// acc_old <- acc; (hi, lo) += v dt + a dt^2/2 evaluated and renormalised in FP64.
+__global__ void ds_drift_kernel(float* __restrict__ xh, float* __restrict__ yh,
+                                float* __restrict__ zh, float* __restrict__ xl,
+                                float* __restrict__ yl, float* __restrict__ zl,
+                                const float* __restrict__ vx, const float* __restrict__ vy,
+                                const float* __restrict__ vz, const float* __restrict__ ax,
+                                const float* __restrict__ ay, const float* __restrict__ az,
+                                float* __restrict__ aox, float* __restrict__ aoy,
+                                float* __restrict__ aoz, int n, double dt, double dt2_half) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* hi[3] = {xh, yh, zh};
+  float* lo[3] = {xl, yl, zl};
+  const float* v[3] = {vx, vy, vz};
+  const float* a[3] = {ax, ay, az};
+  float* ao[3] = {aox, aoy, aoz};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float ak = a[k][i];
+    ao[k][i] = ak;
+    const double inc = dadd_rn(dmul_rn((double)v[k][i], dt), dmul_rn((double)ak, dt2_half));
+    const double x = dadd_rn(dadd_rn((double)hi[k][i], (double)lo[k][i]), inc);
+    const float h = (float)x;
+    hi[k][i] = h;
+    lo[k][i] = (float)dadd_rn(x, -(double)h);
+  }
+}
+
+constexpr int kIpl = 2, kWs = 8;
+
+int ds_forces_launch(nbx_ds_particles* d, float G, float eps2, const KickArgs* kick,
+                     hipStream_t s) {
+  nbx_particles& p = d->p;
+  if (!(eps2 >= 0.0f) || !(eps2 < 3.0e38f))
+    return fail(NBX_ERR_INVALID, "softening eps2 must be finite and >= 0");
+  const int n = (int)p.count;
+  dim3 grid(div_up(n, 64 * kIpl)), block(64 * kWs);
+  KickArgs k{};
+  if (kick) k = *kick;
+  const bool checked = eps2 < kUncheckedMinEps2;
+#define NBX_DS_LAUNCH(C, K)                                                                    \
+  hipLaunchKernelGGL((ds_forces_kernel<kIpl, kWs, C, K>), grid, block, 0, s, p.pos_x, p.pos_y, \
+                     p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z, p.mass, n, (double)G,    \
+                     eps2, p.acc_x, p.acc_y, p.acc_z, k)
+  if (checked) {
+    if (kick) NBX_DS_LAUNCH(true, true);
+    else NBX_DS_LAUNCH(true, false);
+  } else {
+    if (kick) NBX_DS_LAUNCH(false, true);
+    else NBX_DS_LAUNCH(false, false);
+  }
+#undef NBX_DS_LAUNCH
+  NBX_LAUNCHED();
+  return NBX_OK;
+}
+
+int ds_drift_launch(nbx_ds_particles* d, float dt, hipStream_t s) {
+  nbx_particles& p = d->p;
+  const int n = (int)p.count;
+  const double ddt = (double)dt;
+  hipLaunchKernelGGL(ds_drift_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, p.pos_x, p.pos_y,
+                     p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z, p.vel_x, p.vel_y, p.vel_z,
+                     p.acc_x, p.acc_y, p.acc_z, p.acc_old_x, p.acc_old_y, p.acc_old_z, n, ddt,
+                     0.5 * ddt * ddt);
+  NBX_LAUNCHED();
+  return NBX_OK;
+}
+
+int check_ds(const nbx_ds_particles* d, const char* who) {
+  if (!d) return fail(NBX_ERR_INVALID, "%s: null struct", who);
+  NBX_TRY(check_particles(&d->p, who));
+  if (!d->pos_lo_x || !d->pos_lo_y || !d->pos_lo_z)
+    return fail(NBX_ERR_INVALID, "%s: missing low-part position arrays", who);
+  return NBX_OK;
+}
+
+}  // namespace
+
+}  // namespace nbx
+
+using namespace nbx;
+
+extern "C" {
+
+int nbx_ds_alloc(nbx_ds_particles* d, size_t count) {
+  if (!d) return fail(NBX_ERR_INVALID, "nbx_ds_alloc: null struct");
+  d->pos_lo_x = d->pos_lo_y = d->pos_lo_z = nullptr;
+  NBX_TRY(nbx_particles_alloc(&d->p, count));
+  float** lo[3] = {&d->pos_lo_x, &d->pos_lo_y, &d->pos_lo_z};
+  for (float** f : lo) {
+    if (hipMalloc(reinterpret_cast<void**>(f), std::max<size_t>(count, 1) * sizeof(float)) !=
+            hipSuccess ||
+        hipMemset(*f, 0, std::max<size_t>(count, 1) * sizeof(float)) != hipSuccess) {
+      (void)hipGetLastError();
+      nbx_ds_free(d);
+      return fail(NBX_ERR_NOMEM, "double-single position arrays for %zu particles", count);
+    }
+  }
+  return NBX_OK;
+}
+
+int nbx_ds_free(nbx_ds_particles* d) {
+  if (!d) return NBX_OK;
+  float** lo[3] = {&d->pos_lo_x, &d->pos_lo_y, &d->pos_lo_z};
+  for (float** f : lo) {
+    if (*f) (void)hipFree(*f);
+    *f = nullptr;
+  }
+  return nbx_particles_free(&d->p);
+}
+
+int nbx_ds_forces(nbx_ds_particles* d, float G, float eps2, nbx_stream s) {
+  NBX_TRY(check_ds(d, "nbx_ds_forces"));
+  return ds_forces_launch(d, G, eps2, nullptr, as_stream(s));
+}
+
+int nbx_ds_drift(nbx_ds_particles* d, float dt, nbx_stream s) {
+  NBX_TRY(check_ds(d, "nbx_ds_drift"));
+  return ds_drift_launch(d, dt, as_stream(s));
+}
+
+int nbx_ds_step(nbx_ds_particles* d, float G, float eps2, float dt, nbx_stream s) {
+  NBX_TRY(check_ds(d, "nbx_ds_step"));
+  hipStream_t st = as_stream(s);
+  NBX_TRY(ds_drift_launch(d, dt, st));
+  nbx_particles& p = d->p;
+  KickArgs k{p.vel_x, p.vel_y, p.vel_z, p.acc_old_x, p.acc_old_y, p.acc_old_z, 0.5f * dt};
+  return ds_forces_launch(d, G, eps2, &k, st);
+}
+
+int nbx_ds_energy(const nbx_ds_particles* d, float G, float eps2, double* ke, double* pe,
+                  double momentum[3], nbx_stream s) {
+  NBX_TRY(check_ds(d, "nbx_ds_energy"));
+  return energy_launch(&d->p, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z, G, eps2, ke, pe, momentum,
+                       as_stream(s));
+}
+
+}  // extern "C"

@@ -108,6 +108,22 @@ _sig("nbx_bh_build_info", C.c_int, VP, I32P, I32P)
 _sig("nbx_bh_set_timing", C.c_int, VP, C.c_int)
 _sig("nbx_bh_set_builder", C.c_int, VP, C.c_int)
 _sig("nbx_bh_phase_ms", C.c_int, VP, F64P)
+_sig("nbx_ds_alloc", C.c_int, DSP, C.c_size_t)
+_sig("nbx_ds_free", C.c_int, DSP)
+_sig("nbx_ds_forces", C.c_int, DSP, C.c_float, C.c_float, VP)
+_sig("nbx_ds_drift", C.c_int, DSP, C.c_float, VP)
+_sig("nbx_ds_step", C.c_int, DSP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_ds_energy", C.c_int, DSP, C.c_float, C.c_float, F64P, F64P, F64P, VP)
+_sig("nbx_ens_create", C.c_int, C.c_int, I64P, C.POINTER(VP))
+_sig("nbx_ens_destroy", C.c_int, VP)
+_sig("nbx_ens_particles", C.c_int, VP, PP, I64P)
+_sig("nbx_ens_forces", C.c_int, VP, C.c_float, C.c_float, VP)
+_sig("nbx_ens_steps", C.c_int, VP, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, VP)
+_sig("nbx_ens_steps_sequential", C.c_int, VP, C.c_float, C.c_float, C.c_float, C.c_int, VP)
+_sig("nbx_ens_variant_count", C.c_int)
+_sig("nbx_ens_autotune", C.c_int, VP, C.c_int, C.POINTER(C.c_int), F64P)
+_sig("nbx_ens_set_variant", C.c_int, VP, C.c_int)
+_sig("nbx_ens_energy", C.c_int, VP, C.c_float, C.c_float, F64P, F64P, VP)
 _sig("nbx_gen_plummer", C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double, C.c_double,
      *([F32P] * 7))
 _sig("nbx_gen_uniform_sphere", C.c_int, C.c_size_t, C.c_uint64, C.c_double, C.c_double,
@@ -129,7 +145,10 @@ EXPORTED = [
     "nbx_verlet_drift", "nbx_energy", "nbx_bh_create", "nbx_bh_destroy", "nbx_bh_build",
     "nbx_bh_forces", "nbx_bh_node_count", "nbx_bh_copy_nodes", "nbx_bh_copy_sort",
     "nbx_bh_build_info", "nbx_bh_set_timing", "nbx_bh_set_builder", "nbx_bh_phase_ms",
-    "nbx_gen_plummer", "nbx_gen_uniform_sphere",
+    "nbx_ds_alloc", "nbx_ds_free", "nbx_ds_forces", "nbx_ds_drift", "nbx_ds_step",
+    "nbx_ds_energy", "nbx_ens_create", "nbx_ens_destroy", "nbx_ens_particles", "nbx_ens_forces",
+    "nbx_ens_steps", "nbx_ens_steps_sequential", "nbx_ens_variant_count", "nbx_ens_autotune",
+    "nbx_ens_set_variant", "nbx_ens_energy", "nbx_gen_plummer", "nbx_gen_uniform_sphere",
     "nbx_gen_two_plummer", "nbx_gen_ensemble_sizes",
 ]
 
@@ -379,3 +398,126 @@ class BHTree:
             lib.nbx_bh_destroy(self.h)
         except Exception:
             pass
+
+
+class DSDeviceParticles:
+    """Precision-form particles: nbx_ds_particles (pos hi arrays + lo arrays)."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.s = DSParticles()
+        check(lib.nbx_ds_alloc(C.byref(self.s), n))
+
+    @classmethod
+    def from_host(cls, h: dict, pos64=None) -> "DSDeviceParticles":
+        d = cls(len(h["mass"]))
+        hs = host_struct(h)
+        check(lib.nbx_particles_upload(C.byref(d.s.p), C.byref(hs)))
+        if pos64 is not None:
+            d.set_positions64(*pos64)
+        return d
+
+    def set_positions64(self, x, y, z) -> None:
+        for hi_name, lo_name, v in (("pos_x", "pos_lo_x", x), ("pos_y", "pos_lo_y", y),
+                                    ("pos_z", "pos_lo_z", z)):
+            v = np.asarray(v, np.float64)
+            hi = v.astype(np.float32)
+            lo = (v - hi.astype(np.float64)).astype(np.float32)
+            self._write(getattr(self.s.p, hi_name), hi)
+            self._write(getattr(self.s, lo_name), lo)
+
+    def _write(self, ptr, arr):
+        a = np.ascontiguousarray(arr, np.float32)
+        check(lib.nbx_memcpy_htod(C.cast(ptr, VP), a.ctypes.data, self.n * 4))
+
+    def _read(self, ptr):
+        out = np.empty(self.n, np.float32)
+        check(lib.nbx_memcpy_dtoh(out.ctypes.data, C.cast(ptr, VP), self.n * 4))
+        return out
+
+    def read(self, field: str) -> np.ndarray:
+        if field.startswith("pos_lo"):
+            return self._read(getattr(self.s, field))
+        return self._read(getattr(self.s.p, field))
+
+    def positions64(self):
+        return [self.read(h).astype(np.float64) + self.read(l).astype(np.float64)
+                for h, l in (("pos_x", "pos_lo_x"), ("pos_y", "pos_lo_y"), ("pos_z", "pos_lo_z"))]
+
+    def acc(self) -> np.ndarray:
+        return np.stack([self.read("acc_x"), self.read("acc_y"), self.read("acc_z")], axis=1)
+
+    @property
+    def ptr(self):
+        return C.byref(self.s)
+
+    def __del__(self):
+        try:
+            lib.nbx_ds_free(C.byref(self.s))
+        except Exception:
+            pass
+
+
+class Ensemble:
+    """nbx_ensemble: independent systems concatenated in one device SoA set."""
+
+    def __init__(self, counts):
+        self.counts = np.ascontiguousarray(counts, np.int64)
+        self.nsys = len(self.counts)
+        self.h = VP()
+        check(lib.nbx_ens_create(self.nsys, self.counts.ctypes.data_as(I64P), C.byref(self.h)))
+        self.view = Particles()
+        self.offsets = np.zeros(self.nsys + 1, np.int64)
+        check(lib.nbx_ens_particles(self.h, C.byref(self.view), self.offsets.ctypes.data_as(I64P)))
+        self.total = int(self.offsets[-1])
+
+    def upload(self, h: dict) -> None:
+        hs = host_struct(h)
+        check(lib.nbx_particles_upload(C.byref(self.view), C.byref(hs)))
+
+    def read(self, field: str) -> np.ndarray:
+        out = np.empty(self.total, np.float32)
+        check(lib.nbx_memcpy_dtoh(out.ctypes.data, C.cast(getattr(self.view, field), VP),
+                                  self.total * 4))
+        return out
+
+    def system(self, q: int) -> slice:
+        return slice(int(self.offsets[q]), int(self.offsets[q + 1]))
+
+    def forces(self, G: float, eps2: float, stream=None) -> None:
+        check(lib.nbx_ens_forces(self.h, G, eps2, stream.h if stream is not None else None))
+
+    def steps(self, G, eps2, dt, steps, graph_chunk=0, stream=None) -> None:
+        check(lib.nbx_ens_steps(self.h, G, eps2, dt, steps, graph_chunk,
+                                stream.h if stream is not None else None))
+
+    def steps_sequential(self, G, eps2, dt, steps, stream=None) -> None:
+        check(lib.nbx_ens_steps_sequential(self.h, G, eps2, dt, steps,
+                                           stream.h if stream is not None else None))
+
+    def set_variant(self, v: int) -> None:
+        check(lib.nbx_ens_set_variant(self.h, v))
+
+    def autotune(self, reps=3):
+        b, ms = C.c_int(), C.c_double()
+        check(lib.nbx_ens_autotune(self.h, reps, C.byref(b), C.byref(ms)))
+        return b.value, ms.value
+
+    def energy(self, G, eps2):
+        ke = np.zeros(self.nsys)
+        pe = np.zeros(self.nsys)
+        check(lib.nbx_ens_energy(self.h, G, eps2, ke.ctypes.data_as(F64P), pe.ctypes.data_as(F64P),
+                                 None))
+        return ke, pe
+
+    def __del__(self):
+        try:
+            lib.nbx_ens_destroy(self.h)
+        except Exception:
+            pass
+
+
+def gen_ensemble(counts, seed: int = 42) -> dict:
+    """Concatenated Plummer systems (a=1, M=1 each), system q seeded with seed + q."""
+    parts = [gen_plummer(int(c), seed=seed + q) for q, c in enumerate(counts)]
+    return {f: np.concatenate([p[f] for p in parts]) for f in FIELDS}
