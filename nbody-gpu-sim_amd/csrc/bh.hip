@@ -169,16 +169,37 @@ __global__ __launch_bounds__(kThreads) void bbox_partial(const float* __restrict
   }
 }
 
-__global__ void bbox_final(const float* partial, int nparts, int n, BhState* st) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int nparts, int n,
+                                                       BhState* st) {
   float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-  for (int b = 0; b < nparts; ++b)
+  for (int b = threadIdx.x; b < nparts; b += kThreads)
     for (int a = 0; a < 3; ++a) {
       lo[a] = fminf(lo[a], partial[b * 6 + a]);
       hi[a] = fmaxf(hi[a], partial[b * 6 + 3 + a]);
     }
+  for (int a = 0; a < 3; ++a)
+    for (int off = 32; off > 0; off >>= 1) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+    }
+  __shared__ float sh[6][kThreads / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) {
+      sh[a][w] = lo[a];
+      sh[3 + a][w] = hi[a];
+    }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int a = 0; a < 3; ++a)
+    for (int q = 1; q < kThreads / 64; ++q) {
+      sh[a][0] = fminf(sh[a][0], sh[a][q]);
+      sh[3 + a][0] = fmaxf(sh[3 + a][0], sh[3 + a][q]);
+    }
   float ext[3];
   for (int a = 0; a < 3; ++a) {
+    lo[a] = sh[a][0];
+    hi[a] = sh[3 + a][0];
     st->bmin[a] = lo[a];
     st->bmax[a] = hi[a];
     const float size = add_rn(sub_rn(hi[a], lo[a]), 0.001f);  // :313-315
@@ -302,7 +323,23 @@ __global__ void tree_init(BhState* st, TmpNodes T, Levels L, int max_nodes) {
 }
 
 // Level d, pass 1: count each segment's particles per octant, first rank per
-// octant, and the segment's second-smallest rank.
+// octant, and the segment's second-smallest rank. Device-scope atomics are
+// expensive on this part, so each workgroup takes a contiguous run of
+// elements (spatially coherent: few distinct cells), aggregates in an LDS hash
+// table, and flushes one global atomic per distinct key.
+constexpr int kHash = 2048;   // (segment, octant) slots per workgroup
+constexpr int kHashSeg = 1024;  // segment slots (second-smallest rank)
+
+__device__ __forceinline__ int lds_probe(int* keys, int mask, int key) {
+  unsigned h = ((unsigned)key * 2654435761u) >> 7;
+  for (int p = 0; p < 16; ++p) {
+    const int slot = (int)((h + (unsigned)p) & (unsigned)mask);
+    const int prev = atomicCAS(&keys[slot], -1, key);
+    if (prev == -1 || prev == key) return slot;
+  }
+  return -1;
+}
+
 __global__ __launch_bounds__(kThreads) void level_count(BhState* st, Levels L,
                                                         const uint4* __restrict__ path, int d) {
   const int cur = d & 1, nxt = cur ^ 1;
@@ -311,45 +348,53 @@ __global__ __launch_bounds__(kThreads) void level_count(BhState* st, Levels L,
     st->seg_count[nxt] = 0;
     st->elem_count[nxt] = 0;
   }
-  const int ne = st->elem_count[cur];
   const int ns = st->overflow ? 0 : st->seg_count[cur];
-  if (ns == 0) return;
-  __shared__ int l_cnt[512], l_min[512], l_t2[64];
-  const bool local = ns <= 64;  // few segments: aggregate in LDS first
-  if (local) {
-    for (int k = threadIdx.x; k < 512; k += kThreads) {
-      l_cnt[k] = 0;
-      l_min[k] = INT_MAX;
-    }
-    for (int k = threadIdx.x; k < 64; k += kThreads) l_t2[k] = INT_MAX;
-    __syncthreads();
+  const int ne = ns ? st->elem_count[cur] : 0;
+  const int chunk = (ne + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int begin = (int)blockIdx.x * chunk;
+  const int end = min(ne, begin + chunk);
+  if (begin >= end) return;
+  __shared__ int hk[kHash], hc[kHash], hm[kHash];
+  __shared__ int sk[kHashSeg], s2[kHashSeg];
+  for (int k = threadIdx.x; k < kHash; k += kThreads) {
+    hk[k] = -1;
+    hc[k] = 0;
+    hm[k] = INT_MAX;
   }
-  for (int e = blockIdx.x * kThreads + threadIdx.x; e < ne; e += gridDim.x * kThreads) {
+  for (int k = threadIdx.x; k < kHashSeg; k += kThreads) {
+    sk[k] = -1;
+    s2[k] = INT_MAX;
+  }
+  __syncthreads();
+  for (int e = begin + threadIdx.x; e < end; e += kThreads) {
     const int r = d == 0 ? e : L.elem_rank[cur][e];
     const int s = d == 0 ? 0 : L.elem_seg[cur][e];
     const int o = digit_at(path, r, d);
-    const bool not_first = r != L.seg_t1[cur][s];
-    if (local) {
-      atomicAdd(&l_cnt[s * 8 + o], 1);
-      atomicMin(&l_min[s * 8 + o], r);
-      if (not_first) atomicMin(&l_t2[s], r);
+    const int key = s * 8 + o;
+    const int slot = lds_probe(hk, kHash - 1, key);
+    if (slot >= 0) {
+      atomicAdd(&hc[slot], 1);
+      atomicMin(&hm[slot], r);
     } else {
-      atomicAdd(&L.seg_cnt[cur][s * 8 + o], 1);
-      atomicMin(&L.seg_min[cur][s * 8 + o], r);
-      if (not_first) atomicMin(&L.seg_t2[cur][s], r);
+      atomicAdd(&L.seg_cnt[cur][key], 1);
+      atomicMin(&L.seg_min[cur][key], r);
+    }
+    if (r != L.seg_t1[cur][s]) {
+      const int q = lds_probe(sk, kHashSeg - 1, s);
+      if (q >= 0) atomicMin(&s2[q], r);
+      else atomicMin(&L.seg_t2[cur][s], r);
     }
   }
-  if (local) {
-    __syncthreads();
-    for (int k = threadIdx.x; k < ns * 8; k += kThreads) {
-      if (l_cnt[k]) {
-        atomicAdd(&L.seg_cnt[cur][k], l_cnt[k]);
-        atomicMin(&L.seg_min[cur][k], l_min[k]);
-      }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kHash; k += kThreads) {
+    const int key = hk[k];
+    if (key >= 0) {
+      atomicAdd(&L.seg_cnt[cur][key], hc[k]);
+      atomicMin(&L.seg_min[cur][key], hm[k]);
     }
-    for (int k = threadIdx.x; k < ns; k += kThreads)
-      if (l_t2[k] != INT_MAX) atomicMin(&L.seg_t2[cur][k], l_t2[k]);
   }
+  for (int k = threadIdx.x; k < kHashSeg; k += kThreads)
+    if (sk[k] >= 0 && s2[k] != INT_MAX) atomicMin(&L.seg_t2[cur][sk[k]], s2[k]);
 }
 
 // Level d, pass 2: create the children of every segment (cells at depth d+1).
@@ -449,31 +494,38 @@ __global__ __launch_bounds__(kThreads) void level_children(BhState* st, Levels L
 }
 
 // Level d, pass 3: move each active particle to its child segment (or to its
-// depth-32 leaf's member list).
+// depth-32 leaf's member list). One global atomic per workgroup reserves the
+// output range of its contiguous chunk.
 __global__ __launch_bounds__(kThreads) void level_route(BhState* st, Levels L, TmpNodes T,
                                                         const uint4* __restrict__ path, int d) {
   const int cur = d & 1, nxt = cur ^ 1;
   const int ne = st->overflow ? 0 : st->elem_count[cur];
-  const int lane = threadIdx.x & 63;
-  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  for (int base = blockIdx.x * kThreads; base < ne; base += gridDim.x * kThreads) {
-    const int e = base + threadIdx.x;
-    int r = 0, t = -1;
-    if (e < ne) {
-      r = d == 0 ? e : L.elem_rank[cur][e];
-      const int s = d == 0 ? 0 : L.elem_seg[cur][e];
-      t = L.seg_next[cur][s * 8 + digit_at(path, r, d)];
-    }
-    const uint64_t go = __ballot(t >= 0);
-    int wbase = 0;
-    if (go) {
-      if ((go & lt) == 0 && t >= 0) wbase = atomicAdd(&st->elem_count[nxt], __popcll(go));
-      wbase = __shfl(wbase, __ffsll((unsigned long long)go) - 1, 64);
-    }
+  const int chunk = (ne + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int begin = (int)blockIdx.x * chunk;
+  const int end = min(ne, begin + chunk);
+  if (begin >= end) return;
+  __shared__ int sh_base;
+  // pass A: how many of this chunk continue to the next level
+  int mine = 0;
+  for (int e = begin + threadIdx.x; e < end; e += kThreads) {
+    const int r = d == 0 ? e : L.elem_rank[cur][e];
+    const int s = d == 0 ? 0 : L.elem_seg[cur][e];
+    mine += L.seg_next[cur][s * 8 + digit_at(path, r, d)] >= 0;
+  }
+  int total;
+  int off = block_exclusive_scan(mine, &total);
+  if (threadIdx.x == 0) sh_base = total ? atomicAdd(&st->elem_count[nxt], total) : 0;
+  __syncthreads();
+  off += sh_base;
+  // pass B: write them (each thread revisits its own elements in the same order)
+  for (int e = begin + threadIdx.x; e < end; e += kThreads) {
+    const int r = d == 0 ? e : L.elem_rank[cur][e];
+    const int s = d == 0 ? 0 : L.elem_seg[cur][e];
+    const int t = L.seg_next[cur][s * 8 + digit_at(path, r, d)];
     if (t >= 0) {
-      const int pos = wbase + __popcll(go & lt);
-      L.elem_rank[nxt][pos] = r;
-      L.elem_seg[nxt][pos] = t;
+      L.elem_rank[nxt][off] = r;
+      L.elem_seg[nxt][off] = t;
+      ++off;
     } else if (t <= -2) {
       const int leaf = -(t + 2);
       const int slot = T.member_base[leaf] + atomicAdd(&T.member_fill[leaf], 1);
@@ -1058,11 +1110,12 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   hipStream_t s = as_stream(stream);
   const int n = (int)d->count;
   const int g = grid_for(n);
+  const int gl = grid_for(n, 1024);  // level passes: contiguous chunks per workgroup
   mark(t, 0, s);
   const int bb = std::min(t->bbox_blocks, grid_for(n, 1024));
   hipLaunchKernelGGL(bbox_partial, dim3(bb), dim3(kThreads), 0, s, d->pos_x, d->pos_y, d->pos_z,
                      n, t->bbox_part);
-  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(64), 0, s, t->bbox_part, bb, n, t->st);
+  hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kThreads), 0, s, t->bbox_part, bb, n, t->st);
   NBX_LAUNCHED();
   mark(t, 1, s);
   hipLaunchKernelGGL(morton_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, d->pos_x,
@@ -1077,11 +1130,11 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   mark(t, 4, s);
   hipLaunchKernelGGL(tree_init, dim3(1), dim3(64), 0, s, t->st, t->T, t->L, t->max_nodes);
   for (int lv = 0; lv < kMaxDepth; ++lv) {
-    hipLaunchKernelGGL(level_count, dim3(g), dim3(kThreads), 0, s, t->st, t->L, t->path, lv);
+    hipLaunchKernelGGL(level_count, dim3(gl), dim3(kThreads), 0, s, t->st, t->L, t->path, lv);
     hipLaunchKernelGGL(level_children, dim3(grid_for((size_t)n / 2 + 1)), dim3(kThreads), 0, s,
                        t->st, t->L, t->T, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, lv,
                        t->max_nodes);
-    hipLaunchKernelGGL(level_route, dim3(g), dim3(kThreads), 0, s, t->st, t->L, t->T, t->path,
+    hipLaunchKernelGGL(level_route, dim3(gl), dim3(kThreads), 0, s, t->st, t->L, t->T, t->path,
                        lv);
   }
   hipLaunchKernelGGL(tree_levels_done, dim3(1), dim3(64), 0, s, t->st, t->max_nodes);

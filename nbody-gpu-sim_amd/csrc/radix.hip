@@ -24,28 +24,51 @@ __global__ __launch_bounds__(kThreads) void radix_count(const uint32_t* __restri
   table[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of table[0 .. 256*tiles) in place, one workgroup of 1024.
-__global__ __launch_bounds__(1024) void radix_scan(uint32_t* table, int total) {
-  __shared__ uint32_t part[1024];
-  const int per = (total + 1023) / 1024;
-  const int b = threadIdx.x * per;
-  const int e = min(total, b + per);
-  uint32_t s = 0;
-  for (int i = b; i < e; ++i) s += table[i];
-  part[threadIdx.x] = s;
+// Exclusive scan of the digit-major table, in two launches: each workgroup scans
+// one digit's row over the tiles (and records the row total); then one small
+// workgroup turns the 256 row totals into digit bases, added during scatter.
+__global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* table, int tiles,
+                                                       uint32_t* row_total) {
+  __shared__ uint32_t wsum[4];
+  uint32_t* row = table + (size_t)blockIdx.x * tiles;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (int base = 0; base < tiles; base += 256) {
+    const int i = base + threadIdx.x;
+    const uint32_t v = i < tiles ? row[i] : 0u;
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = carry, tot = 0;
+    for (int q = 0; q < 4; ++q) {
+      if (q < w) before += wsum[q];
+      tot += wsum[q];
+    }
+    if (i < tiles) row[i] = before + incl - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(256) void radix_scan_bases(uint32_t* row_total) {
+  __shared__ uint32_t sh[256];
+  sh[threadIdx.x] = row_total[threadIdx.x];
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int d = 0; d < 256; ++d) {
+      const uint32_t v = sh[d];
+      sh[d] = run;
+      run += v;
+    }
   }
-  uint32_t run = part[threadIdx.x] - s;  // exclusive prefix of this chunk
-  for (int i = b; i < e; ++i) {
-    const uint32_t v = table[i];
-    table[i] = run;
-    run += v;
-  }
+  __syncthreads();
+  row_total[threadIdx.x] = sh[threadIdx.x];
 }
 
 __device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid, int nbits) {
@@ -60,8 +83,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid, int 
 
 __global__ __launch_bounds__(kThreads) void radix_scatter(
     const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    int32_t* __restrict__ vout, const uint32_t* __restrict__ table, const int32_t* __restrict__ n_dev,
-    int shift, int nbits, int tiles) {
+    int32_t* __restrict__ vout, const uint32_t* __restrict__ table,
+    const uint32_t* __restrict__ bases, const int32_t* __restrict__ n_dev, int shift, int nbits,
+    int tiles) {
   constexpr int R = RadixSort::kRounds;
   constexpr int W = RadixSort::kWaves;
   __shared__ uint32_t wave_hist[W][kBins];
@@ -96,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter(
   // per-wave starting offsets: tile offset + earlier waves of the tile
   {
     const int d = threadIdx.x;  // kThreads == kBins
-    uint32_t run = table[d * tiles + blockIdx.x];
+    uint32_t run = bases[d] + table[d * tiles + blockIdx.x];
     for (int q = 0; q < W; ++q) {
       const uint32_t c = wave_hist[q][d];
       wave_hist[q][d] = run;
@@ -129,7 +153,8 @@ int RadixSort::init(int n) {
   max_tiles = (max_n + kTile - 1) / kTile;
   if (hipMalloc(&keys_tmp, sizeof(uint32_t) * max_n) != hipSuccess ||
       hipMalloc(&vals_tmp, sizeof(int32_t) * max_n) != hipSuccess ||
-      hipMalloc(&table, sizeof(uint32_t) * kBins * max_tiles) != hipSuccess) {
+      hipMalloc(&table, sizeof(uint32_t) * kBins * max_tiles) != hipSuccess ||
+      hipMalloc(&bases, sizeof(uint32_t) * kBins) != hipSuccess) {
     (void)hipGetLastError();
     release();
     return fail(NBX_ERR_NOMEM, "radix sort buffers for %d keys", n);
@@ -141,6 +166,8 @@ void RadixSort::release() {
   if (keys_tmp) (void)hipFree(keys_tmp);
   if (vals_tmp) (void)hipFree(vals_tmp);
   if (table) (void)hipFree(table);
+  if (bases) (void)hipFree(bases);
+  bases = nullptr;
   keys_tmp = nullptr;
   vals_tmp = nullptr;
   table = nullptr;
@@ -158,9 +185,10 @@ int RadixSort::sort(uint32_t* keys, int32_t* vals, const int32_t* n_dev, int key
     const int nbits = key_bits - shift < 8 ? key_bits - shift : 8;
     hipLaunchKernelGGL(radix_count, dim3(max_tiles), dim3(kThreads), 0, s, ka, n_dev, shift, table,
                        max_tiles);
-    hipLaunchKernelGGL(radix_scan, dim3(1), dim3(1024), 0, s, table, kBins * max_tiles);
+    hipLaunchKernelGGL(radix_scan_rows, dim3(kBins), dim3(256), 0, s, table, max_tiles, bases);
+    hipLaunchKernelGGL(radix_scan_bases, dim3(1), dim3(256), 0, s, bases);
     hipLaunchKernelGGL(radix_scatter, dim3(max_tiles), dim3(kThreads), 0, s, ka, va, kb, vb, table,
-                       n_dev, shift, nbits, max_tiles);
+                       bases, n_dev, shift, nbits, max_tiles);
     NBX_LAUNCHED();
     uint32_t* tk = ka;
     ka = kb;

@@ -4,7 +4,8 @@
 // semantics of thrust::sort_by_key used by the reference
 // (src/cuda/force_barnes_hut.cu:324-328). 8-bit digits; per pass:
 //   count   - per-tile digit histogram (LDS atomics)
-//   scan    - exclusive scan of the digit-major [256][tiles] table (one workgroup)
+//   scan    - exclusive scan of each digit's row over the tiles (one workgroup
+//             per digit), then the 256 digit bases
 //   scatter - in-tile stable ranking by wave ballots (match-any over the digit
 //             bits, rank = popcount of lower matching lanes), waves own
 //             contiguous sub-tiles so order is kept, then scattered writes.
@@ -25,6 +26,7 @@ struct RadixSort {
   uint32_t* keys_tmp = nullptr;
   int32_t* vals_tmp = nullptr;
   uint32_t* table = nullptr;  // [256][max_tiles]
+  uint32_t* bases = nullptr;  // [256] digit bases of the current pass
 
   int init(int max_n);
   void release();
