@@ -1,0 +1,69 @@
+// Force-method operators of the engine's C++ API, mirroring the reference's
+// include/nbody/force_calculator.hpp (build-workbench/nbody-gpu-sim). Each
+// computeForces call runs HIP kernels through the nbx C ABI on the calling
+// thread's null stream, asynchronously.
+#pragma once
+
+#include <memory>
+
+#include "nbody/types.hpp"
+
+namespace nbody {
+
+class ForceCalculator {
+ public:
+  virtual ~ForceCalculator() = default;
+  virtual void computeForces(ParticleData* d_particles) = 0;
+  virtual ForceMethod getMethod() const = 0;
+
+  void setSofteningParameter(float eps) {
+    softening_eps_ = eps;
+    softening_eps2_ = eps * eps;
+  }
+  void setGravitationalConstant(float G) { G_ = G; }
+  float getSofteningParameter() const noexcept { return softening_eps_; }
+  float getGravitationalConstant() const noexcept { return G_; }
+
+ protected:
+  float softening_eps_ = 0.1f;
+  float softening_eps2_ = 0.01f;
+  float G_ = 1.0f;
+};
+
+class DirectForceCalculator : public ForceCalculator {
+ public:
+  explicit DirectForceCalculator(int block_size = 256);
+  void computeForces(ParticleData* d_particles) override;
+  ForceMethod getMethod() const noexcept override { return ForceMethod::DIRECT_N2; }
+  void setBlockSize(int size) { block_size_ = size; }
+  int getBlockSize() const noexcept { return block_size_; }
+
+ private:
+  int block_size_;
+};
+
+class BarnesHutCalculator : public ForceCalculator {
+ public:
+  explicit BarnesHutCalculator(float theta = 0.5f);
+  ~BarnesHutCalculator();
+  void computeForces(ParticleData* d_particles) override;
+  ForceMethod getMethod() const noexcept override { return ForceMethod::BARNES_HUT; }
+  void setTheta(float theta) { theta_ = theta; }
+  float getTheta() const noexcept { return theta_; }
+  BarnesHutTree* getTree() noexcept { return tree_.get(); }
+
+ private:
+  std::unique_ptr<BarnesHutTree> tree_;
+  float theta_;
+};
+
+// DIRECT_N2 and BARNES_HUT; SPATIAL_HASH throws ValidationException (not built).
+std::unique_ptr<ForceCalculator> createForceCalculator(ForceMethod method,
+                                                       const SimulationConfig& config);
+
+// Host pair function: the acceleration on a particle at p1 due to mass m2 at p2
+// (Plummer-softened), evaluated in IEEE FP32 like the reference's CPU reference.
+Vec3 computeGravitationalAccelerationCPU(const Vec3& p1, const Vec3& p2, float m2, float G,
+                                         float eps);
+
+}  // namespace nbody
