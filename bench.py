@@ -1,18 +1,28 @@
 #!/usr/bin/env python3
-"""Headline benchmark: direct N^2 FP32 gravity with Velocity Verlet on one MI355X.
+"""Benchmarks of the N-body engine on one MI355X.
 
-Workload: uniform-density sphere of radius 1, N = 262,144 equal masses 1/N,
+Headline (the JSON line's `value`): direct N^2 FP32 gravity with Velocity
+Verlet on a uniform-density sphere of radius 1, N = 262,144 equal masses 1/N,
 Gaussian velocities sigma = 0.05 per axis, G = 1, eps = 0.005, dt = 5e-4
-(synthetic, seed 42). A step is one Velocity Verlet step
-(store+drift kernel, then the force kernel with the kick in its epilogue),
-all on one HIP stream with the inputs resident in HBM.
+(synthetic, seed 42). A step is one Velocity Verlet step (store+drift kernel,
+then the force kernel with the kick in its epilogue) on one HIP stream, with
+the inputs resident in HBM. Reported as whole-job interactions/s, with the
+roofline of the force kernel (timed per launch with HIP events on its stream;
+20 flop per interaction against the 157.3 TFLOP/s FP32 peak), its HBM traffic
+from the committed rocprofv3 counter summary, the FP64-oracle force error on
+sample targets, and the CPU baseline (the oracle's restatement of the
+reference CPU force path, timed on this host on a bounded sample).
 
-Prints one JSON line with the whole-job rate in Ginteractions/s, the roofline
-of the dominant kernel (the direct force kernel, timed per launch with HIP
-events on its stream; 20 flop per interaction against the 157.3 TFLOP/s FP32
-vector peak), the FP64-oracle force error on sample targets, and the CPU
-baseline (the oracle's restatement of the reference CPU force path, timed on
-this host on a bounded sample).
+The same line carries the other two parts of the metric:
+  barnes_hut - ms/step at N = 1,048,576 (two Plummer spheres, theta 0.5) with
+               per-phase times and achieved GB/s of the HBM-bound phases;
+  ensemble   - 512 Plummer systems, N_s log-uniform in [256, 16384]: batched
+               (persistent launch, hipGraph chunks of 100 steps) vs the same
+               steps advancing each system alone; per-system forces are
+               bit-identical to the standalone kernel.
+
+--only {direct,barnes-hut,ensemble,precision} runs one part (for profiling);
+precision compares plain FP32 with the double-single/Kahan form at N=65,536.
 """
 import argparse
 import json
@@ -29,6 +39,7 @@ sys.path.insert(0, os.path.join(ROOT, "nbody-gpu-sim_amd"))
 METRIC = "direct N² Ginteractions/s; BH ms/step @1M; batched-ensemble speedup @fixed err"
 FLOP_PER_INTERACTION = 20.0
 PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "direct_forces_uniform_sphere_n262144_traffic.md")
 
 
@@ -36,8 +47,18 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    return orc
+
+
+def eps2_of(eps):
+    return float(np.float32(np.float32(eps) * np.float32(eps)))
+
+
 def traffic_from_profile():
-    """HBM bytes per launch of the force kernel, from the committed rocprofv3 --pmc summary."""
+    """HBM bytes per launch of the force kernel, from the committed rocprofv3 summary."""
     try:
         text = open(TRAFFIC_PROFILE).read()
     except OSError:
@@ -46,92 +67,48 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(h, G, eps, n_targets):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import orc
+# ------------------------------------------------------------------ direct
 
-    n = len(h["mass"])
-    t = np.sort(np.random.default_rng(7).choice(n, n_targets, replace=False))
-    threads = orc.max_threads()
-    t0 = time.perf_counter()
-    orc.direct_f32(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], G, eps, targets=t,
-                   nthreads=threads)
-    sec = time.perf_counter() - t0
-    rate = n_targets * (n - 1) / sec / 1e9
-    return {"value": rate, "unit": "Ginteractions/s", "cores": threads, "kind": "port",
-            "sample": f"{n_targets} of {n} targets x all {n} sources, one FP32 force "
-                      f"evaluation (reference CPU pair function summed in j order), "
-                      f"{sec:.2f} s wall"}
-
-
-def force_error(nbx, d, h_now, G, eps, n_targets=256):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import orc
-
-    n = len(h_now["mass"])
-    t = np.sort(np.random.default_rng(42).choice(n, min(n, n_targets), replace=False))
-    ref = orc.direct_f64(h_now["pos_x"], h_now["pos_y"], h_now["pos_z"], h_now["mass"], G,
-                         orc.eps2_of(eps), targets=t)
-    got = d.acc()[t].astype(np.float64)
-    e = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
-    return {"targets": int(len(t)), "median": float(np.median(e)),
-            "p99": float(np.quantile(e, 0.99)), "max": float(e.max())}
-
-
-def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=262144)
-    ap.add_argument("--variant", type=int, default=-1, help="force a direct tile variant")
-    ap.add_argument("--autotune", action="store_true", help="time all tile variants first")
-    ap.add_argument("--cpu-targets", type=int, default=16384)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this engine runs on a single MI355X "
-              "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
-        return 2
-    import nbx
-
+def run_direct(nbx, args):
     n, G, eps, dt = args.n, 1.0, 0.005, 5e-4
-    eps2 = float(np.float32(np.float32(eps) * np.float32(eps)))
-    info = nbx.device_info()
-    log(f"device {info['name']} ({info['cu']} CUs, {info['clock_khz'] / 1e3:.0f} MHz)")
-
+    e2 = eps2_of(eps)
+    if args.calibrate_copy:
+        C = nbx.C
+        nb = 256 << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        nbx.check(nbx.lib.nbx_malloc(C.byref(a), nb))
+        nbx.check(nbx.lib.nbx_malloc(C.byref(b), nb))
+        nbx.check(nbx.lib.nbx_memcpy_dtod(b, a, nb))
+        nbx.synchronize()
+        nbx.check(nbx.lib.nbx_free(a))
+        nbx.check(nbx.lib.nbx_free(b))
     if args.autotune:
         best, ms = nbx.C.c_int(), nbx.C.c_double()
         nbx.check(nbx.lib.nbx_direct_autotune(n, 3, nbx.C.byref(best), nbx.C.byref(ms)))
         log(f"autotune: best variant {nbx.variant_names()[best.value]} {ms.value:.3f} ms")
     if args.variant >= 0:
         nbx.check(nbx.lib.nbx_direct_set_variant(args.variant))
-
     h = nbx.gen_uniform_sphere(n, seed=42, R=1.0, M=1.0, sigma=0.05)
     d = nbx.DeviceParticles.from_host(h)
     s = nbx.Stream()
     lib = nbx.lib
-    nbx.check(lib.nbx_direct_forces(d.ptr, G, eps2, 256, s.h))  # a(0)
+    nbx.check(lib.nbx_direct_forces(d.ptr, G, e2, 256, s.h))  # a(0)
 
     def step(ev_a=None, ev_b=None):
         nbx.check(lib.nbx_verlet_drift(d.ptr, dt, s.h))
         if ev_a is not None:
             ev_a.record(s)
-        nbx.check(lib.nbx_direct_forces_kick(d.ptr, G, eps2, dt, s.h))
+        nbx.check(lib.nbx_direct_forces_kick(d.ptr, G, e2, dt, s.h))
         if ev_b is not None:
             ev_b.record(s)
 
     for _ in range(args.warmup):
         step()
     s.synchronize()
-    log(f"warmup done ({args.warmup} steps)")
-
     K = args.steps
     fa = [nbx.Event() for _ in range(K)]
     fb = [nbx.Event() for _ in range(K)]
     e0, e1 = nbx.Event(), nbx.Event()
-    s.synchronize()
     wall0 = time.perf_counter()
     e0.record(s)
     for k in range(K):
@@ -141,57 +118,332 @@ def main() -> int:
     wall = time.perf_counter() - wall0
     total_ms = e0.elapsed_ms(e1)
     force_ms = np.array([fa[k].elapsed_ms(fb[k]) for k in range(K)])
-    log(f"timed {K} steps: {total_ms:.3f} ms (wall {wall * 1e3:.3f} ms), force kernel "
-        f"mean {force_ms.mean():.3f} ms min {force_ms.min():.3f} ms")
-
+    log(f"direct: {K} steps {total_ms:.3f} ms (wall {wall * 1e3:.3f} ms), force kernel mean "
+        f"{force_ms.mean():.3f} ms")
     inter = float(n) * float(n - 1)
     value = K * inter / (total_ms / 1e3) / 1e9
     achieved = FLOP_PER_INTERACTION * inter / (force_ms.mean() / 1e3) / 1e12
-    traffic = traffic_from_profile()
 
-    h_now = d.download()
-    err = force_error(nbx, d, h_now, G, eps)
-    log(f"force error vs FP64 on {err['targets']} targets: median {err['median']:.2e} "
-        f"p99 {err['p99']:.2e} max {err['max']:.2e}")
+    err = None
+    if not args.no_check:
+        orc = oracle()
+        hn = d.download()
+        t = np.sort(np.random.default_rng(42).choice(n, min(n, 256), replace=False))
+        ref = orc.direct_f64(hn["pos_x"], hn["pos_y"], hn["pos_z"], hn["mass"], G, e2, targets=t)
+        got = d.acc()[t].astype(np.float64)
+        e = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        err = {"targets": int(len(t)), "median": float(np.median(e)),
+               "p99": float(np.quantile(e, 0.99)), "max": float(e.max())}
+        log(f"direct force error vs FP64: {err}")
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(h, G, eps, args.cpu_targets)
-        log(f"cpu baseline: {cpu['value']:.4f} Ginteractions/s on {cpu['cores']} threads")
-
+        orc = oracle()
+        t = np.sort(np.random.default_rng(7).choice(n, args.cpu_targets, replace=False))
+        threads = orc.max_threads()
+        t0 = time.perf_counter()
+        orc.direct_f32(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], G, eps, targets=t,
+                       nthreads=threads)
+        sec = time.perf_counter() - t0
+        cpu = {"value": args.cpu_targets * (n - 1) / sec / 1e9, "unit": "Ginteractions/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_targets} of {n} targets x all {n} sources: one FP32 force "
+                         f"evaluation with the reference CPU pair function summed in j order "
+                         f"(oracle), {sec:.2f} s wall on {threads} threads"}
+        log(f"direct cpu baseline: {cpu['value']:.3f} Ginteractions/s")
     names = nbx.variant_names()
-    variant = args.variant
-    if variant < 0:
-        # the variant the dispatcher picked for this size
-        variant = None
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "Ginteractions/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": args.warmup,
-        "ms_per_step": total_ms / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
+    v = args.variant if args.variant >= 0 else None
+    return {
+        "value": value, "ms_per_step": total_ms / K,
         "config": {"workload": "direct N^2 FP32 + Velocity Verlet, uniform sphere R=1, "
-                               "N=262144, m=1/N, sigma_v=0.05, eps=0.005, dt=5e-4",
+                               f"N={n}, m=1/N, sigma_v=0.05, eps=0.005, dt=5e-4",
                    "n": n, "eps": eps, "dt": dt, "G": G, "parallelism": "single GPU",
-                   "tile_variant": names[variant] if variant is not None else "auto"},
+                   "tile_variant": names[v] if v is not None else "auto"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": traffic,
+                     "traffic": traffic_from_profile(),
                      "kernel": "direct_forces_kernel (forces + kick epilogue)",
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "interactions_per_launch": inter,
                      "launch_ms_mean": float(force_ms.mean())},
         "force_error_fp64": err,
         "cpu_baseline": cpu,
+    }
+
+
+# ------------------------------------------------------------------ Barnes-Hut
+
+# algorithmic bytes per particle of the HBM-bound build phases
+BH_BYTES = {"bbox": 12.0, "morton": 20.0, "paths": 4.0 + 12.0 + 16.0}
+
+
+def run_bh(nbx, args):
+    n, G, eps, theta, dt = args.bh_n, 1.0, 0.01, 0.5, 1e-3
+    h = nbx.gen_two_plummer(n, seed=42)
+    d = nbx.DeviceParticles.from_host(h)
+    t = nbx.BHTree(n)
+    s = nbx.Stream()
+    t.build(d, s)
+    t.forces(d, theta, G, eps, s)
+
+    def step():
+        nbx.check(nbx.lib.nbx_verlet_drift(d.ptr, dt, s.h))
+        t.build(d, s)
+        t.forces(d, theta, G, eps, s)
+        nbx.check(nbx.lib.nbx_update_velocities(d.ptr, dt, s.h))
+
+    step()
+    s.synchronize()
+    K = args.bh_steps
+    e0, e1 = nbx.Event(), nbx.Event()
+    e0.record(s)
+    for _ in range(K):
+        step()
+    e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_ms(e1) / K
+    # one more step with phase events for the breakdown
+    t.set_timing(True)
+    step()
+    s.synchronize()
+    phases = t.phase_ms()
+    t.set_timing(False)
+    nodes, depth = t.node_count()
+    gbs = {k: BH_BYTES[k] * n / (phases[k] * 1e-3) / 1e9 for k in BH_BYTES if phases[k] > 0}
+    log(f"barnes-hut N={n}: {ms:.3f} ms/step, phases {phases}, nodes {nodes}, depth {depth}")
+    out = {"n": n, "theta": theta, "eps": eps, "dt": dt, "steps": K, "ms_per_step": ms,
+           "workload": "two Plummer spheres (a=1, M=1 each, truncated at 99.9% mass) at x=+-5, "
+                       "relative velocity 0.5 along x",
+           "phases_ms": phases, "nodes": nodes, "max_depth": depth,
+           "builder": t.build_info()["builder"],
+           "achieved_GBps": gbs, "hbm_peak_GBps": PEAK_HBM_GBS}
+    if not args.no_cpu_baseline:
+        orc = oracle()
+        hn = d.download()
+        x, y, z, m = hn["pos_x"], hn["pos_y"], hn["pos_z"], hn["mass"]
+        t0 = time.perf_counter()
+        tree = orc.bh_pipeline(hn)
+        build_s = time.perf_counter() - t0
+        sample = min(n, args.bh_cpu_targets)
+        tg = np.sort(np.random.default_rng(3).choice(n, sample, replace=False))
+        threads = orc.max_threads()
+        t0 = time.perf_counter()
+        orc.bh_forces(tree["nodes"], x, y, z, m, theta, G, eps, targets=tg, nthreads=threads)
+        trav_s = time.perf_counter() - t0
+        est = (build_s + trav_s * n / sample) * 1e3
+        out["cpu_baseline"] = {
+            "value": est, "unit": "ms/step", "cores": threads, "kind": "port",
+            "sample": f"oracle build (bbox, Morton, stable sort, single-thread insertion + COM, "
+                      f"as the reference host build) {build_s * 1e3:.0f} ms + traversal of "
+                      f"{sample} of {n} targets on {threads} threads {trav_s * 1e3:.0f} ms, "
+                      f"scaled to all targets"}
+        log(f"barnes-hut cpu baseline: {est:.1f} ms/step")
+    return out
+
+
+# ------------------------------------------------------------------ ensemble
+
+def run_ensemble(nbx, args):
+    counts = nbx.gen_ensemble_sizes(512, seed=42, nmin=256, nmax=16384)
+    h = nbx.gen_ensemble(counts, seed=1000)
+    G, eps, dt = 1.0, 0.01, 1e-3
+    e2 = eps2_of(eps)
+    pairs = int((counts * (counts - 1)).sum())
+    ens = nbx.Ensemble(counts)
+    ens.upload(h)
+    s = nbx.Stream()
+    v, tune_ms = ens.autotune(2)
+    ens.forces(G, e2, s)
+    chunk = args.ens_chunk
+    ens.steps(G, e2, dt, chunk, chunk, s)  # capture + first replay
+    s.synchronize()
+    e0, e1 = nbx.Event(), nbx.Event()
+    K = args.ens_steps
+    e0.record(s)
+    ens.steps(G, e2, dt, K, chunk, s)
+    e1.record(s)
+    s.synchronize()
+    batched = e0.elapsed_ms(e1) / K
+    # baseline: the same steps, one system at a time with the single-system entry points
+    seq = nbx.Ensemble(counts)
+    seq.upload(h)
+    seq.forces(G, e2, s)
+    seq.steps_sequential(G, e2, dt, 1, s)
+    s.synchronize()
+    Ks = max(1, args.ens_seq_steps)
+    e0.record(s)
+    seq.steps_sequential(G, e2, dt, Ks, s)
+    e1.record(s)
+    s.synchronize()
+    sequential = e0.elapsed_ms(e1) / Ks
+    err = None
+    if not args.no_check:
+        orc = oracle()
+        ens.forces(G, e2, s)
+        s.synchronize()
+        acc = np.stack([ens.read("acc_x"), ens.read("acc_y"), ens.read("acc_z")], 1)
+        pos = {k: ens.read(k) for k in ("pos_x", "pos_y", "pos_z", "mass")}
+        worst = 0.0
+        for q in np.linspace(0, len(counts) - 1, 8).astype(int):
+            sl = ens.system(int(q))
+            ref = orc.direct_f64(pos["pos_x"][sl], pos["pos_y"][sl], pos["pos_z"][sl],
+                                 pos["mass"][sl], G, e2)
+            e = np.linalg.norm(acc[sl] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+            worst = max(worst, float(np.quantile(e, 0.99)))
+        err = {"systems_checked": 8, "worst_p99_rel_err_fp64": worst}
+    names = nbx.variant_names()
+    out = {"systems": int(len(counts)), "total_particles": int(counts.sum()),
+           "interactions_per_step": pairs, "batched_ms_per_step": batched,
+           "sequential_ms_per_step": sequential, "speedup": sequential / batched,
+           "batched_Ginteractions_per_s": pairs / batched / 1e6,
+           "graph_chunk": chunk, "tile_variant": names[v], "force_error": err,
+           "workload": "512 Plummer systems (a=1, M=1 each), N_s log-uniform in [256, 16384], "
+                       "eps=0.01, dt=1e-3"}
+    log(f"ensemble: batched {batched:.3f} ms/step, sequential {sequential:.3f} ms/step, "
+        f"speedup {out['speedup']:.2f}, tile {names[v]}")
+    return out
+
+
+# ------------------------------------------------------------------ precision
+
+def run_precision(nbx, args):
+    n, G, eps, dt = args.prec_n, 1.0, 0.01, 2.5e-4
+    e2 = eps2_of(eps)
+    K = args.prec_steps
+    every = max(1, K // 20)
+    h = nbx.gen_plummer(n, seed=42)
+    plain = nbx.DeviceParticles.from_host(h)
+    ds = nbx.DSDeviceParticles.from_host(h)
+    s = nbx.Stream()
+    C = nbx.C
+
+    def ds_energy():
+        ke, pe = C.c_double(), C.c_double()
+        mom = (C.c_double * 3)()
+        nbx.check(nbx.lib.nbx_ds_energy(ds.ptr, G, e2, C.byref(ke), C.byref(pe), mom, s.h))
+        return ke.value + pe.value, list(mom)
+
+    def plain_energy():
+        e = nbx.energy(plain, G, eps)
+        return e["ke"] + e["pe"], e["mom"]
+
+    orc = oracle()
+    tg = np.sort(np.random.default_rng(5).choice(n, 512, replace=False))
+
+    def force_err(acc, pos):
+        ref = orc.direct_f64_pos64(pos[0], pos[1], pos[2], h["mass"], G, e2, targets=tg)
+        e = np.linalg.norm(acc[tg] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        return {"median": float(np.median(e)), "p99": float(np.quantile(e, 0.99))}
+
+    res = {}
+    for name in ("plain", "double_single"):
+        if name == "plain":
+            nbx.check(nbx.lib.nbx_direct_forces(plain.ptr, G, e2, 256, s.h))
+            energy = plain_energy
+            def step(): nbx.check(nbx.lib.nbx_direct_step(plain.ptr, G, e2, dt, s.h))
+            def state():
+                a = plain.acc().astype(np.float64)
+                return a, [plain.read(k).astype(np.float64) for k in ("pos_x", "pos_y", "pos_z")]
+        else:
+            nbx.check(nbx.lib.nbx_ds_forces(ds.ptr, G, e2, s.h))
+            energy = ds_energy
+            def step(): nbx.check(nbx.lib.nbx_ds_step(ds.ptr, G, e2, dt, s.h))
+            def state(): return ds.acc().astype(np.float64), ds.positions64()
+        s.synchronize()
+        E0, P0 = energy()
+        a0, p0 = state()
+        err0 = force_err(a0, p0)
+        drift = []
+        e_start, e_stop = nbx.Event(), nbx.Event()
+        run_ms = 0.0
+        for k in range(0, K, every):
+            e_start.record(s)
+            for _ in range(min(every, K - k)):
+                step()
+            e_stop.record(s)
+            run_ms += e_start.elapsed_ms(e_stop)
+            E, P = energy()
+            drift.append([k + min(every, K - k), (E - E0) / abs(E0)])
+        a1, p1 = state()
+        res[name] = {"ms_per_step": run_ms / K, "E0": E0,
+                     "rel_energy_drift_final": drift[-1][1],
+                     "rel_energy_drift_max": max(abs(v) for _, v in drift),
+                     "momentum_final": energy()[1], "force_err_start": err0,
+                     "force_err_end": force_err(a1, p1), "drift_series": drift}
+        log(f"precision {name}: {res[name]['ms_per_step']:.3f} ms/step, final drift "
+            f"{res[name]['rel_energy_drift_final']:.3e}, max {res[name]['rel_energy_drift_max']:.3e}")
+    return {"n": n, "steps": K, "dt": dt, "eps": eps,
+            "workload": "Plummer sphere a=1, M=1, FP32 pair forces; double_single = hi/lo "
+                        "positions, Kahan sums, FP64 renormalised drift", **res}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--only", choices=["direct", "barnes-hut", "ensemble", "precision"])
+    ap.add_argument("--variant", type=int, default=-1, help="force a direct tile variant")
+    ap.add_argument("--autotune", action="store_true", help="time all direct tile variants first")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="one 256 MiB device copy first (reference launch for counter scaling)")
+    ap.add_argument("--cpu-targets", type=int, default=16384)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--bh-n", type=int, default=1 << 20)
+    ap.add_argument("--bh-steps", type=int, default=5)
+    ap.add_argument("--bh-cpu-targets", type=int, default=65536)
+    ap.add_argument("--ens-steps", type=int, default=100)
+    ap.add_argument("--ens-chunk", type=int, default=100)
+    ap.add_argument("--ens-seq-steps", type=int, default=5)
+    ap.add_argument("--prec-n", type=int, default=65536)
+    ap.add_argument("--prec-steps", type=int, default=10000)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this engine runs on a single MI355X "
+              "(no multi-GPU path is built); run with --gpus 1", file=sys.stderr)
+        return 2
+    import nbx
+
+    info = nbx.device_info()
+    log(f"device {info['name']} ({info['cu']} CUs, {info['clock_khz'] / 1e3:.0f} MHz)")
+    if args.only == "precision":
+        print(json.dumps({"precision": run_precision(nbx, args), "device": info["name"]}),
+              flush=True)
+        return 0
+    if args.only == "barnes-hut":
+        print(json.dumps({"barnes_hut": run_bh(nbx, args), "device": info["name"]}), flush=True)
+        return 0
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": run_ensemble(nbx, args), "device": info["name"]}),
+              flush=True)
+        return 0
+    direct = run_direct(nbx, args)
+    line = {
+        "metric": METRIC,
+        "value": direct["value"],
+        "unit": "Ginteractions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": direct["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": direct["config"],
+        "roofline": direct["roofline"],
+        "cpu_baseline": direct["cpu_baseline"],
+        "force_error_fp64": direct["force_error_fp64"],
         "device": info["name"],
     }
+    if args.only != "direct":
+        line["barnes_hut"] = run_bh(nbx, args)
+        line["ensemble"] = run_ensemble(nbx, args)
     print(json.dumps(line), flush=True)
     return 0
 
