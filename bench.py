@@ -128,7 +128,7 @@ def run_direct(nbx, args):
     if not args.no_check:
         orc = oracle()
         hn = d.download()
-        t = np.sort(np.random.default_rng(42).choice(n, min(n, 256), replace=False))
+        t = np.sort(np.random.default_rng(42).choice(n, min(n, args.check_targets), replace=False))
         ref = orc.direct_f64(hn["pos_x"], hn["pos_y"], hn["pos_z"], hn["mass"], G, e2, targets=t)
         got = d.acc()[t].astype(np.float64)
         e = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
@@ -152,17 +152,23 @@ def run_direct(nbx, args):
                          f"(oracle), {sec:.2f} s wall on {threads} threads"}
         log(f"direct cpu baseline: {cpu['value']:.3f} Ginteractions/s")
     names = nbx.variant_names()
-    v = args.variant if args.variant >= 0 else None
+    v = args.variant if args.variant >= 0 else int(nbx.lib.nbx_direct_variant_for(n))
+    sym = names[v].startswith("sym_")
     return {
         "value": value, "ms_per_step": total_ms / K,
         "config": {"workload": "direct N^2 FP32 + Velocity Verlet, uniform sphere R=1, "
                                f"N={n}, m=1/N, sigma_v=0.05, eps=0.005, dt=5e-4",
                    "n": n, "eps": eps, "dt": dt, "G": G, "parallelism": "single GPU",
-                   "tile_variant": names[v] if v is not None else "auto"},
+                   "tile_variant": names[v]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profile(),
-                     "kernel": "direct_forces_kernel (forces + kick epilogue)",
+                     "kernel": ("direct_sym_kernel + sym_reduce_kernel (forces + kick)" if sym
+                                else "direct_forces_kernel (forces + kick epilogue)"),
+                     "evaluation": ("pair-symmetric: each of the N(N-1)/2 pairs evaluated once "
+                                    "and applied to both bodies; achieved counts the N(N-1) "
+                                    "interactions of the reference's algorithm" if sym else
+                                    "one-sided: N(N-1) pair evaluations"),
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "interactions_per_launch": inter,
                      "launch_ms_mean": float(force_ms.mean())},
@@ -386,6 +392,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--only", choices=["direct", "barnes-hut", "ensemble", "precision"])
     ap.add_argument("--variant", type=int, default=-1, help="force a direct tile variant")
+    ap.add_argument("--check-targets", type=int, default=4096,
+                    help="sample targets of the FP64 force check")
     ap.add_argument("--autotune", action="store_true", help="time all direct tile variants first")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 256 MiB device copy first (reference launch for counter scaling)")

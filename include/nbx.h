@@ -102,11 +102,14 @@ int nbx_direct_step(nbx_particles* d, float G, float eps2, float dt, nbx_stream 
  * v += (a_old + a_new)*(0.5f*dt) in its epilogue (after nbx_verlet_drift). */
 int nbx_direct_forces_kick(nbx_particles* d, float G, float eps2, float dt, nbx_stream s);
 
-/* Tile variants of the direct kernel (i per lane, waves splitting j). */
+/* Tile variants of the direct kernel, named iplI_nbB_wsW: I targets per lane,
+ * stage groups of B bodies, W waves splitting the j range of one tile. */
 int nbx_direct_variant_count(void);
 int nbx_direct_variant_name(int v, char* buf, size_t len);
 /* -1 restores the autotuned choice. */
 int nbx_direct_set_variant(int v);
+/* The variant a call on n bodies launches now (forced, tuned or default). */
+int nbx_direct_variant_for(size_t n);
 /* Time every variant on a synthetic input of n bodies; remember the fastest for
  * n's size class (used by later calls). Synchronous. best_ms may be NULL. */
 int nbx_direct_autotune(size_t n, int reps, int* best_variant, double* best_ms);

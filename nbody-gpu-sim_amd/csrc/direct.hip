@@ -7,31 +7,37 @@
 
 #include "common.hpp"
 #include "direct_core.hpp"
+#include "direct_sym.hpp"
 #include "verlet.hpp"
 
 namespace nbx {
 
-template <int IPL, int WS, bool CHECKED, bool KICK>
+template <int IPL, int NB, int WS, bool CHECKED, bool KICK>
 __global__ __launch_bounds__(64 * WS) void direct_forces_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     const float* __restrict__ m, int n, int i0, int iend, double G, float eps2, float* ax,
     float* ay, float* az, KickArgs kick) {
   __shared__ double red[WS * 3 * 64 * IPL];
   const int ibase = i0 + (int)blockIdx.x * 64 * IPL;
-  direct_tile<IPL, WS, CHECKED, KICK>(x, y, z, m, n, ibase, iend, G, eps2, ax, ay, az, kick, red);
+  direct_tile<IPL, NB, WS, CHECKED, KICK>(x, y, z, m, n, ibase, iend, G, eps2, ax, ay, az, kick,
+                                          red);
 }
 
 struct DirectVariant {
   const char* name;
   int ipl;
-  int ws;
+  int nb;  // bodies per stage group (one-sided), 0 for the symmetric kernel
+  int ws;  // waves per workgroup
+  // Super-block size of the symmetric kernel, 0 for one-sided shapes. Symmetric
+  // shapes only evaluate whole systems ([i0, iend) = [0, n)).
+  int sym_block;
   // launches over targets [i0, iend) of the system d
-  void (*launch)(const nbx_particles* d, int i0, int iend, double G, float eps2, bool checked,
-                 const KickArgs* kick, hipStream_t s);
+  int (*launch)(const nbx_particles* d, int i0, int iend, double G, float eps2, bool checked,
+                const KickArgs* kick, hipStream_t s);
 };
 
-template <int IPL, int WS>
-static void launch_variant(const nbx_particles* d, int i0, int iend, double G, float eps2,
+template <int IPL, int NB, int WS>
+static int launch_variant(const nbx_particles* d, int i0, int iend, double G, float eps2,
                            bool checked, const KickArgs* kick, hipStream_t s) {
   const int n = (int)d->count;
   const int tiles = (int)div_up((size_t)(iend - i0), 64 * IPL);
@@ -40,31 +46,41 @@ static void launch_variant(const nbx_particles* d, int i0, int iend, double G, f
   if (kick) k = *kick;
   if (checked) {
     if (kick)
-      hipLaunchKernelGGL((direct_forces_kernel<IPL, WS, true, true>), grid, block, 0, s, d->pos_x,
+      hipLaunchKernelGGL((direct_forces_kernel<IPL, NB, WS, true, true>), grid, block, 0, s, d->pos_x,
                          d->pos_y, d->pos_z, d->mass, n, i0, iend, G, eps2, d->acc_x, d->acc_y,
                          d->acc_z, k);
     else
-      hipLaunchKernelGGL((direct_forces_kernel<IPL, WS, true, false>), grid, block, 0, s,
+      hipLaunchKernelGGL((direct_forces_kernel<IPL, NB, WS, true, false>), grid, block, 0, s,
                          d->pos_x, d->pos_y, d->pos_z, d->mass, n, i0, iend, G, eps2, d->acc_x,
                          d->acc_y, d->acc_z, k);
   } else {
     if (kick)
-      hipLaunchKernelGGL((direct_forces_kernel<IPL, WS, false, true>), grid, block, 0, s,
+      hipLaunchKernelGGL((direct_forces_kernel<IPL, NB, WS, false, true>), grid, block, 0, s,
                          d->pos_x, d->pos_y, d->pos_z, d->mass, n, i0, iend, G, eps2, d->acc_x,
                          d->acc_y, d->acc_z, k);
     else
-      hipLaunchKernelGGL((direct_forces_kernel<IPL, WS, false, false>), grid, block, 0, s,
+      hipLaunchKernelGGL((direct_forces_kernel<IPL, NB, WS, false, false>), grid, block, 0, s,
                          d->pos_x, d->pos_y, d->pos_z, d->mass, n, i0, iend, G, eps2, d->acc_x,
                          d->acc_y, d->acc_z, k);
   }
+  NBX_LAUNCHED();
+  return NBX_OK;
 }
 
-static const DirectVariant kVariants[] = {
-    {"ipl1_ws4", 1, 4, launch_variant<1, 4>},  {"ipl1_ws8", 1, 8, launch_variant<1, 8>},
-    {"ipl1_ws16", 1, 16, launch_variant<1, 16>}, {"ipl2_ws2", 2, 2, launch_variant<2, 2>},
-    {"ipl2_ws4", 2, 4, launch_variant<2, 4>},  {"ipl2_ws8", 2, 8, launch_variant<2, 8>},
-    {"ipl4_ws2", 4, 2, launch_variant<4, 2>},  {"ipl4_ws4", 4, 4, launch_variant<4, 4>},
-};
+template <int IPL, int W>
+static int launch_sym(const nbx_particles* d, int, int, double G, float eps2, bool checked,
+                      const KickArgs* kick, hipStream_t s) {
+  return direct_sym_launch<IPL, W>(d, G, eps2, checked, kick, s);
+}
+
+// One-sided shapes first, in the order the batched kernel shares, then the
+// symmetric ones.
+#define NBX_DIRECT_V(I, B, W) {"ipl" #I "_nb" #B "_ws" #W, I, B, W, 0, launch_variant<I, B, W>},
+#define NBX_SYM_V(I, W) {"sym_ipl" #I "_ws" #W, I, 0, W, 64 * I * W, launch_sym<I, W>},
+static const DirectVariant kVariants[] = {NBX_DIRECT_VARIANTS(NBX_DIRECT_V)
+                                              NBX_SYM_VARIANTS(NBX_SYM_V)};
+#undef NBX_DIRECT_V
+#undef NBX_SYM_V
 constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
 
 // Autotuned choice per size class (floor(log2 n)); -1 = not tuned.
@@ -79,12 +95,22 @@ static int size_class(size_t n) {
   return c;
 }
 
-// Default when nothing was tuned, from a sweep on MI355X (ipl = targets per
-// lane, ws = waves splitting the j range of one tile).
+// Defaults when nothing was tuned, from sweeps on MI355X (uniform-sphere
+// bodies, equal masses): the one-sided kernel for small systems and for calls
+// the symmetric kernel cannot take, the symmetric one from 2^16 bodies.
+#define NBX_COUNT_V(I, B, W) +1
+constexpr int kNumOneSided = 0 NBX_DIRECT_VARIANTS(NBX_COUNT_V);
+#undef NBX_COUNT_V
+
+static int one_sided_default(size_t n) {
+  if (n >= (size_t(1) << 17)) return 1;  // ipl1_nb4_ws8
+  if (n >= (size_t(1) << 15)) return 3;  // ipl1_nb8_ws8
+  return 4;                              // ipl1_nb8_ws16
+}
+
 static int default_variant(size_t n) {
-  if (n >= (size_t(1) << 17)) return 7;  // ipl4_ws4
-  if (n >= (size_t(1) << 15)) return 5;  // ipl2_ws8
-  return 2;                              // ipl1_ws16
+  if (n >= (size_t(1) << 16)) return kNumOneSided + 1;  // sym_ipl4_ws4
+  return one_sided_default(n);
 }
 
 int direct_pick_variant(size_t n) {
@@ -104,11 +130,13 @@ int direct_launch(const nbx_particles* d, size_t i0, size_t ni, float G, float e
   if (ni == 0) return NBX_OK;
   if (!(eps2 >= 0.0f) || !std::isfinite(eps2))
     return fail(NBX_ERR_INVALID, "softening eps2 must be finite and >= 0 (got %g)", (double)eps2);
-  const int v = variant >= 0 ? variant : direct_pick_variant(d->count);
+  int v = variant >= 0 ? variant : direct_pick_variant(d->count);
+  if (kVariants[v].sym_block &&
+      (i0 != 0 || ni != d->count ||
+       direct_sym_partial_bytes(d->count, kVariants[v].sym_block) > kSymMaxPartialBytes))
+    v = one_sided_default(d->count);
   const bool checked = eps2 < kUncheckedMinEps2;
-  kVariants[v].launch(d, (int)i0, (int)(i0 + ni), (double)G, eps2, checked, kick, s);
-  NBX_LAUNCHED();
-  return NBX_OK;
+  return kVariants[v].launch(d, (int)i0, (int)(i0 + ni), (double)G, eps2, checked, kick, s);
 }
 
 }  // namespace nbx
@@ -162,6 +190,8 @@ int nbx_direct_set_variant(int v) {
   g_forced = v;
   return NBX_OK;
 }
+
+int nbx_direct_variant_for(size_t n) { return direct_pick_variant(n); }
 
 int nbx_direct_autotune(size_t n, int reps, int* best_variant, double* best_ms) {
   if (n == 0 || n > (size_t)INT32_MAX) return fail(NBX_ERR_INVALID, "autotune size %zu", n);

@@ -24,7 +24,7 @@
 namespace nbx {
 namespace {
 
-template <int IPL, int WS, bool CHECKED, bool KICK>
+template <int IPL, int NB, int WS, bool CHECKED, bool KICK>
 __global__ __launch_bounds__(64 * WS) void ens_forces_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     const float* __restrict__ m, const int32_t* __restrict__ sys_off,
@@ -50,8 +50,9 @@ __global__ __launch_bounds__(64 * WS) void ens_forces_kernel(
       k.aoy += off;
       k.aoz += off;
     }
-    direct_tile<IPL, WS, CHECKED, KICK>(x + off, y + off, z + off, m + off, n, w.y * 64 * IPL, n,
-                                        G, eps2, ax + off, ay + off, az + off, k, red);
+    direct_tile<IPL, NB, WS, CHECKED, KICK>(x + off, y + off, z + off, m + off, n,
+                                            w.y * 64 * IPL, n, G, eps2, ax + off, ay + off,
+                                            az + off, k, red);
   }
   if (threadIdx.x == 0) {
     __threadfence();
@@ -117,18 +118,16 @@ struct EnsVariant {
   void* k[2][2];  // [checked][kick]
 };
 
-#define NBX_ENS_V(I, W)                                                                        \
-  {                                                                                            \
-    "ipl" #I "_ws" #W, I, W, {                                                                 \
-      {(void*)ens_forces_kernel<I, W, false, false>, (void*)ens_forces_kernel<I, W, false, true>}, \
-      {(void*)ens_forces_kernel<I, W, true, false>, (void*)ens_forces_kernel<I, W, true, true>}  \
-    }                                                                                          \
-  }
-// Same (ipl, ws) list and order as the single-system variants (direct.hip).
-static const EnsVariant kEnsVariants[] = {
-    NBX_ENS_V(1, 4), NBX_ENS_V(1, 8), NBX_ENS_V(1, 16), NBX_ENS_V(2, 2),
-    NBX_ENS_V(2, 4), NBX_ENS_V(2, 8), NBX_ENS_V(4, 2),  NBX_ENS_V(4, 4),
-};
+#define NBX_ENS_V(I, B, W)                                                        \
+  {"ipl" #I "_nb" #B "_ws" #W,                                                    \
+   I,                                                                             \
+   W,                                                                             \
+   {{(void*)ens_forces_kernel<I, B, W, false, false>,                             \
+     (void*)ens_forces_kernel<I, B, W, false, true>},                             \
+    {(void*)ens_forces_kernel<I, B, W, true, false>,                              \
+     (void*)ens_forces_kernel<I, B, W, true, true>}}},
+// Same tile list and order as the single-system variants (direct.hip).
+static const EnsVariant kEnsVariants[] = {NBX_DIRECT_VARIANTS(NBX_ENS_V)};
 #undef NBX_ENS_V
 constexpr int kEnsNumVariants = (int)(sizeof(kEnsVariants) / sizeof(kEnsVariants[0]));
 
@@ -296,7 +295,7 @@ int nbx_ens_create(int nsys, const int64_t* counts, nbx_ensemble** out) {
     rc = fail(NBX_ERR_HIP, "energy block table upload failed");
   if (rc == NBX_OK && hipStreamCreateWithFlags(&e->own_stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(NBX_ERR_HIP, "stream creation failed");
-  e->variant = 2;  // ipl1_ws16: smallest tile, best for systems of a few thousand bodies
+  e->variant = 4;  // ipl1_nb8_ws16: smallest tile, best for systems of a few thousand bodies
   if (rc != NBX_OK) {
     ens_release(e);
     delete e;

@@ -91,6 +91,7 @@ _sig("nbx_direct_forces_kick", C.c_int, PP, C.c_float, C.c_float, C.c_float, VP)
 _sig("nbx_direct_variant_count", C.c_int)
 _sig("nbx_direct_variant_name", C.c_int, C.c_int, C.c_char_p, C.c_size_t)
 _sig("nbx_direct_set_variant", C.c_int, C.c_int)
+_sig("nbx_direct_variant_for", C.c_int, C.c_size_t)
 _sig("nbx_direct_autotune", C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_int), F64P)
 _sig("nbx_store_old_accelerations", C.c_int, PP, VP)
 _sig("nbx_update_positions", C.c_int, PP, C.c_float, VP)
@@ -140,7 +141,8 @@ EXPORTED = [
     "nbx_particles_free", "nbx_particles_upload", "nbx_particles_download", "nbx_direct_forces",
     "nbx_direct_forces_range", "nbx_direct_step", "nbx_direct_forces_kick",
     "nbx_direct_variant_count",
-    "nbx_direct_variant_name", "nbx_direct_set_variant", "nbx_direct_autotune",
+    "nbx_direct_variant_name", "nbx_direct_set_variant", "nbx_direct_variant_for",
+    "nbx_direct_autotune",
     "nbx_store_old_accelerations", "nbx_update_positions", "nbx_update_velocities",
     "nbx_verlet_drift", "nbx_energy", "nbx_bh_create", "nbx_bh_destroy", "nbx_bh_build",
     "nbx_bh_forces", "nbx_bh_node_count", "nbx_bh_copy_nodes", "nbx_bh_copy_sort",

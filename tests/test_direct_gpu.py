@@ -101,8 +101,34 @@ def test_ragged_sizes(nbx, orc, n):
     check_fp64(nbx, orc, h, 1.0, 0.01)
 
 
-def test_every_variant_matches(nbx, orc):
+def mixed_masses(h, seed):
+    """Masses drawn from [0.1, 2), as the reference initialisers do for min_mass < max_mass."""
+    n = len(h["mass"])
+    h["mass"][:] = np.random.default_rng(seed).uniform(0.1, 2.0, n).astype(np.float32) / np.float32(n)
+    return h
+
+
+@pytest.mark.parametrize("n", [4096, 20000])
+def test_mixed_masses_fp64(nbx, orc, n):
+    # per-pair mass factor (folds whose masses differ)
+    h = mixed_masses(nbx.gen_plummer(n, seed=5), seed=n)
+    check_fp64(nbx, orc, h, 1.0, 0.01)
+
+
+@pytest.mark.parametrize("where", [0, 1500, 4095])
+def test_one_heavy_body(nbx, orc, where):
+    # equal masses except one: the j ranges holding it take the per-pair path,
+    # the others apply the shared mass once per fold
+    h = nbx.gen_plummer(4096, seed=11)
+    h["mass"][where] = np.float32(50.0) * h["mass"][0]
+    check_fp64(nbx, orc, h, 1.0, 0.01)
+
+
+@pytest.mark.parametrize("masses", ["equal", "mixed"])
+def test_every_variant_matches(nbx, orc, masses):
     h = nbx.gen_plummer(3000, seed=9)
+    if masses == "mixed":
+        mixed_masses(h, seed=3)
     names = nbx.variant_names()
     results = []
     for v in range(len(names)):
@@ -155,3 +181,105 @@ def test_uniform_sphere_262144_sample(nbx, orc):
     t = np.sort(np.random.default_rng(42).choice(262144, 4096, replace=False))
     got = run_forces(nbx, h, 1.0, 0.005)
     check_fp64(nbx, orc, h, 1.0, 0.005, targets=t, got=got)
+
+
+# ---------------------------------------------------------------- pair-symmetric kernel
+# Whole-system evaluations with each unordered pair computed once and applied to both
+# bodies (direct_sym.hip). Same FP64 bounds as the one-sided kernel.
+
+def sym_variants(nbx):
+    return [(v, nm) for v, nm in enumerate(nbx.variant_names()) if nm.startswith("sym_")]
+
+
+def sym_block(name):
+    ipl, ws = (int(t[3:] if t.startswith("ipl") else t[2:]) for t in name.split("_")[1:])
+    return 64 * ipl * ws
+
+
+@pytest.mark.parametrize("blocks", [0.6, 1, 2, 2.4, 3, 4, 4.5, 7])
+def test_sym_super_block_counts(nbx, orc, blocks):
+    # K = ceil(n / S) super-blocks: one (diagonal only), even and odd K, ragged last block
+    for v, name in sym_variants(nbx):
+        n = max(3, int(blocks * sym_block(name)))
+        if n > 20000:
+            continue
+        h = nbx.gen_plummer(n, seed=n + v)
+        check_fp64(nbx, orc, h, 1.0, 0.01, variant=v)
+
+
+@pytest.mark.parametrize("masses", ["mixed", "one_heavy"])
+def test_sym_masses(nbx, orc, masses):
+    for v, name in sym_variants(nbx):
+        n = 3 * sym_block(name) + 77
+        h = nbx.gen_plummer(n, seed=21)
+        if masses == "mixed":
+            mixed_masses(h, seed=v)
+        else:
+            h["mass"][n // 2] = np.float32(40.0) * h["mass"][0]
+        check_fp64(nbx, orc, h, 1.0, 0.01, variant=v)
+
+
+def test_sym_zero_softening(nbx, orc):
+    # eps = 0: the diagonal tiles zero the self pair by index; off-diagonal tiles have none
+    for v, name in sym_variants(nbx):
+        n = 2 * sym_block(name) + 5
+        h = nbx.gen_uniform_sphere(n, seed=8)
+        got = run_forces(nbx, h, 1.0, 0.0, variant=v)
+        assert np.all(np.isfinite(got)), name
+        t = np.random.default_rng(1).choice(n, 512, False)
+        ref = orc.direct_f64(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 1.0, 0.0, t)
+        st = error_stats(got[t], ref)
+        assert st["p99"] <= 1e-5, (name, st)
+
+
+def test_sym_deterministic(nbx):
+    for v, name in sym_variants(nbx):
+        h = nbx.gen_uniform_sphere(3 * sym_block(name) + 1, seed=3)
+        a = run_forces(nbx, h, 1.0, 0.01, variant=v)
+        b = run_forces(nbx, h, 1.0, 0.01, variant=v)
+        assert np.array_equal(a, b), name
+
+
+def test_sym_kick_epilogue(nbx):
+    # v_new = v + (a_old + a_new) * (0.5 dt), IEEE single rounding, with a_new as returned
+    dt = np.float32(1e-3)
+    for v, name in sym_variants(nbx):
+        n = 2 * sym_block(name) + 300
+        h = nbx.gen_plummer(n, seed=17)
+        rng = np.random.default_rng(2)
+        for f in ("acc_old_x", "acc_old_y", "acc_old_z"):
+            h[f][:] = rng.normal(size=n).astype(np.float32)
+        d = nbx.DeviceParticles.from_host(h)
+        for f in ("acc_old_x", "acc_old_y", "acc_old_z"):
+            d.write(f, h[f])  # upload() leaves acc_old alone, as the reference copy does
+        nbx.check(nbx.lib.nbx_direct_set_variant(v))
+        try:
+            nbx.check(nbx.lib.nbx_direct_forces_kick(d.ptr, 1.0, float(np.float32(0.01) ** 2),
+                                                     float(dt), None))
+            nbx.synchronize()
+        finally:
+            nbx.check(nbx.lib.nbx_direct_set_variant(-1))
+        half = np.float32(np.float32(0.5) * dt)
+        for c in "xyz":
+            a_new = d.read(f"acc_{c}")
+            s = (h[f"acc_old_{c}"] + a_new).astype(np.float32)
+            want = (h[f"vel_{c}"] + (s * half).astype(np.float32)).astype(np.float32)
+            assert np.array_equal(d.read(f"vel_{c}"), want), (name, c)
+
+
+def test_sym_range_call_uses_one_sided(nbx):
+    # target sub-ranges are evaluated by the one-sided kernel whatever variant is forced
+    v, name = sym_variants(nbx)[0]
+    h = nbx.gen_plummer(2 * sym_block(name), seed=6)
+    eps2 = float(np.float32(0.01) * np.float32(0.01))
+    res = []
+    for forced in (v, -1):
+        d = nbx.DeviceParticles.from_host(h)
+        nbx.check(nbx.lib.nbx_direct_set_variant(forced))
+        try:
+            nbx.check(nbx.lib.nbx_direct_forces_range(d.ptr, 100, 900, 1.0, eps2, None))
+            nbx.synchronize()
+        finally:
+            nbx.check(nbx.lib.nbx_direct_set_variant(-1))
+        res.append(d.acc())
+    assert np.array_equal(res[0], res[1])
