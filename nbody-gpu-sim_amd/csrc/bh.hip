@@ -44,7 +44,6 @@ namespace nbx {
 namespace {
 
 constexpr int kMaxDepth = 32;  // force_barnes_hut.cu:24
-constexpr int kStack = 256;    // force_barnes_hut.cu:28
 constexpr int kThreads = 256;
 
 // 76-byte node, byte layout of nbody::OctreeNode (barnes_hut_tree.hpp:9-30).
@@ -272,6 +271,8 @@ struct TmpNodes {
   float4* com;      // centre of mass, total mass
   int32_t* member_base;  // depth-32 multi leaves: slot range start (else -1)
   int32_t* member_fill;
+  int32_t* size;  // cells in the subtree (the cell included)
+  int32_t* pre;   // position in the traversal order
 };
 
 struct Levels {
@@ -633,8 +634,12 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
   if (st->use_seq) return;
   const int b = st->level_start[d], e = st->level_start[d + 1];
   for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
-    if (T.info[t] & (1u << 11)) continue;
+    if (T.info[t] & (1u << 11)) {
+      T.size[t] = 1;
+      continue;
+    }
     float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    int size = 1;
     for (int o = 0; o < 8; ++o) {
       const int c = T.child[t * 8 + o];
       if (c < 0) continue;
@@ -643,6 +648,7 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
       sx = add_rn(sx, mul_rn(q.x, q.w));
       sy = add_rn(sy, mul_rn(q.y, q.w));
       sz = add_rn(sz, mul_rn(q.z, q.w));
+      size += T.size[c];
     }
     float4 r = make_float4(0.f, 0.f, 0.f, tm);
     if (tm > 0.0f) {
@@ -651,10 +657,40 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
       r.z = div_rn(sz, tm);
     }
     T.com[t] = r;
+    T.size[t] = size;
   }
 }
 
-// Writes the reference-layout node array from the parallel builder's cells.
+// Traversal layout, one level per launch from the root down: cells in the
+// order the reference's stack walk visits them (:160-241: children pushed
+// 0..7, so popped 7..0), i.e. depth-first preorder with children 7 down to 0.
+// Record p holds (centre of mass, mass) and (cell centre, half size with the
+// leaf flag in its sign); skip[p] is the position after the cell's subtree.
+__device__ __forceinline__ float4 trav_box(float4 b, bool leaf) {
+  return make_float4(b.x, b.y, b.z, leaf ? -b.w - 0.0f : b.w);
+}
+
+__global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict__ st, TmpNodes T,
+                                                      int d, float4* rec, int32_t* skip) {
+  if (st->use_seq) return;
+  const int b = st->level_start[d], e = st->level_start[d + 1];
+  for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
+    const int pt = d == 0 ? 0 : T.pre[t];
+    const bool leaf = (T.info[t] >> 11) & 1u;
+    rec[2 * pt] = T.com[t];
+    rec[2 * pt + 1] = trav_box(T.box[t], leaf);
+    skip[pt] = pt + T.size[t];
+    if (leaf) continue;
+    int next = pt + 1;
+    for (int o = 7; o >= 0; --o) {
+      const int c = T.child[t * 8 + o];
+      if (c < 0) continue;
+      T.pre[c] = next;
+      next += T.size[c];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void emit_nodes(const BhState* __restrict__ st, TmpNodes T,
                                                        const int32_t* __restrict__ final_of,
                                                        RefNode* out) {
@@ -831,34 +867,63 @@ __global__ void seq_build(BhState* st, const int32_t* __restrict__ sidx, const f
   st->depth_merges = nd;
 }
 
-// Compact traversal layout from the reference-layout nodes.
-__global__ __launch_bounds__(kThreads) void pack_traversal(const BhState* __restrict__ st,
-                                                           const RefNode* __restrict__ nodes,
-                                                           float4* com4, float4* box4,
-                                                           int4* child4) {
+// Traversal layout when the sequential builder produced the nodes (use_seq):
+// one thread walks them as the reference does (:160-241) and numbers them in
+// visiting order, then sizes the subtrees in reverse order (a node's children
+// always come after it). pre_of, size and order are scratch of max_nodes ints.
+__global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __restrict__ nodes,
+                             int32_t* pre_of, int32_t* size, int32_t* order, float4* rec,
+                             int32_t* skip) {
+  if (!st->use_seq || threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nn = st->final_count;
-  for (int f = blockIdx.x * kThreads + threadIdx.x; f < nn; f += gridDim.x * kThreads) {
-    const RefNode& q = nodes[f];
-    com4[f] = make_float4(q.comx, q.comy, q.comz, q.mass);
-    // half size carries the leaf flag in its sign bit (half >= 0 always)
-    box4[f] = make_float4(q.cx, q.cy, q.cz, q.leaf ? -q.half - 0.0f : q.half);
-    child4[2 * f] = make_int4(q.child[0], q.child[1], q.child[2], q.child[3]);
-    child4[2 * f + 1] = make_int4(q.child[4], q.child[5], q.child[6], q.child[7]);
+  if (nn <= 0) return;
+  int stack[8 * (kMaxDepth + 2)];
+  int sp = 0, cnt = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const int v = stack[--sp];
+    pre_of[v] = cnt;
+    order[cnt++] = v;
+    if (nodes[v].leaf) continue;
+    for (int o = 0; o < 8; ++o) {
+      const int c = nodes[v].child[o];
+      if (c >= 0 && sp < 8 * (kMaxDepth + 2)) stack[sp++] = c;
+    }
+  }
+  for (int p = cnt - 1; p >= 0; --p) {
+    const int v = order[p];
+    const RefNode& q = nodes[v];
+    int sz = 1;
+    if (!q.leaf)
+      for (int o = 0; o < 8; ++o)
+        if (q.child[o] >= 0) sz += size[q.child[o]];
+    size[v] = sz;
+    rec[2 * p] = make_float4(q.comx, q.comy, q.comz, q.mass);
+    rec[2 * p + 1] = make_float4(q.cx, q.cy, q.cz, q.leaf ? -q.half - 0.0f : q.half);
+    skip[p] = p + sz;
   }
 }
 
 // ---------------------------------------------------------------- 7: traversal
 
+// Wave-coherent traversal over the preorder layout. The 64 lanes of a wave
+// hold 64 consecutive particles in Morton order and walk the union of their
+// trees in the reference's visiting order (:160-241). State: the current
+// record v and the mask M of lanes that need it. Each lane takes exactly the
+// reference's accept/open decisions in the same IEEE arithmetic. If some lanes
+// open v, the walk descends to v + 1 (its first child in this order) with
+// those lanes, remembering (skip[v], M); otherwise it continues at skip[v].
+// Arriving at a remembered position restores its mask. Positions only grow,
+// so the walk takes at most one step per record, and the remembered entries
+// are ancestors of v: at most the tree depth (<= 33), kept one per lane in
+// VGPRs. A lane's interactions are summed in the reference's order.
 __global__ __launch_bounds__(kThreads) void bh_traverse(
-    const float4* __restrict__ com4, const float4* __restrict__ box4,
-    const int4* __restrict__ child4, const BhState* __restrict__ st,
-    const int32_t* __restrict__ order, const float* __restrict__ x, const float* __restrict__ y,
-    const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
-    float eps2, float* ax, float* ay, float* az) {
-  __shared__ int32_t st_node[kThreads / 64][kStack];
-  __shared__ uint64_t st_mask[kThreads / 64][kStack];
+    const float4* __restrict__ rec, const int32_t* __restrict__ skip,
+    const BhState* __restrict__ st, const int32_t* __restrict__ order,
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    const float* __restrict__ m, int n, float theta2, float G, float eps2, float* ax, float* ay,
+    float* az) {
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
   const int r = blockIdx.x * kThreads + threadIdx.x;
   const bool valid = r < n;
   const int i = valid ? order[r] : 0;
@@ -866,38 +931,44 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
   const int nn = st->final_count;
   float sx = 0.f, sy = 0.f, sz = 0.f;
   const uint64_t me = 1ull << lane;
-  const uint64_t live = __ballot(valid);
-  if (lane == 0) {
-    st_node[w][0] = 0;
-    st_mask[w][0] = live;
-  }
-  int sp = 1;
-  int budget = nn + 1;  // every node is pushed at most once per wave
-  while (sp > 0 && budget-- > 0) {
-    --sp;
-    __builtin_amdgcn_wave_barrier();
-    const int node = __builtin_amdgcn_readfirstlane(st_node[w][sp]);
-    const uint64_t mk = st_mask[w][sp];
-    const uint32_t mlo = __builtin_amdgcn_readfirstlane((uint32_t)mk);
-    const uint32_t mhi = __builtin_amdgcn_readfirstlane((uint32_t)(mk >> 32));
-    const uint64_t mask = ((uint64_t)mhi << 32) | mlo;
-    if (node < 0 || node >= nn) continue;
-    const float4 c4 = com4[node];
-    if (c4.w == 0.0f) continue;  // :172-173
-    const float4 b4 = box4[node];
+  uint64_t M = __ballot(valid);
+  int keep_at = 0;          // lane k: position of remembered entry k
+  uint32_t keep_lo = 0, keep_hi = 0;  // lane k: its mask
+  int depth = 0;
+  int v = 0;
+  int budget = nn + 1;
+  while (v < nn && budget-- > 0) {
+    while (depth > 0 && __builtin_amdgcn_readlane(keep_at, depth - 1) == v) {
+      --depth;
+      M = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)keep_hi, depth) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)keep_lo, depth);
+    }
+    // One 32-byte record plus its skip, waited for together.
+    const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 32u;
+    const float4 c4 = *reinterpret_cast<const float4*>(rp);
+    const float4 b4 = *reinterpret_cast<const float4*>(rp + 16);
+    const int after =
+        *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(skip) + (uint32_t)v * 4u);
+    asm volatile("" ::"s"(c4.x), "s"(c4.y), "s"(c4.z), "s"(c4.w), "s"(b4.x), "s"(b4.y),
+                 "s"(b4.z), "s"(b4.w), "s"(after));
+    if (c4.w == 0.0f) {  // :172-173
+      v = after;
+      continue;
+    }
+    const bool act = (M & me) != 0;
     const bool leaf = __builtin_signbit(b4.w);
     const float h = fabsf(b4.w);
-    bool open = false;
-    if (mask & me) {
-      const bool contains =
-          fabsf(xi - b4.x) <= h && fabsf(yi - b4.y) <= h && fabsf(zi - b4.z) <= h;  // :180-182
-      float meff = c4.w, cx = c4.x, cy = c4.y, cz = c4.z;
-      bool interact = true;
-      if (leaf) {
-        if (contains) {  // :190-200
+    const bool contains = (fabsf(xi - b4.x) <= h) & (fabsf(yi - b4.y) <= h) &
+                          (fabsf(zi - b4.z) <= h);  // :180-182
+    float meff = c4.w, cx = c4.x, cy = c4.y, cz = c4.z;
+    bool skip_self = false;
+    if (leaf) {
+      const bool rm = act && contains;  // :190-200, the particle's own leaf
+      if (__ballot(rm)) {
+        if (rm) {
           meff = sub_rn(c4.w, mi);
           if (meff <= 0.0f) {
-            interact = false;
+            skip_self = true;
           } else {
             cx = div_rn(sub_rn(mul_rn(c4.x, c4.w), mul_rn(xi, mi)), meff);
             cy = div_rn(sub_rn(mul_rn(c4.y, c4.w), mul_rn(yi, mi)), meff);
@@ -905,38 +976,42 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
           }
         }
       }
-      const float dx = sub_rn(cx, xi), dy = sub_rn(cy, yi), dz = sub_rn(cz, zi);
-      const float d2 =
-          add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
-      if (!leaf) {  // :218-224
-        const float size2 = mul_rn(mul_rn(4.0f, h), h);
-        if (contains || !(div_rn(size2, d2) < theta2)) {
-          interact = false;
-          open = true;
-        }
-      }
-      if (interact) {
-        const float inv = __builtin_amdgcn_rsqf(d2);
-        const float f = mul_rn(G, meff) * (inv * inv * inv);
-        sx = __builtin_fmaf(f, dx, sx);
-        sy = __builtin_fmaf(f, dy, sy);
-        sz = __builtin_fmaf(f, dz, sz);
-      }
     }
-    const uint64_t om = __ballot(open);
-    if (om != 0 && sp + 8 <= kStack) {  // :234-240, children 0..7 pushed in order
-      const int4 c0 = child4[2 * node], c1 = child4[2 * node + 1];
-      const int ch[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (ch[c] >= 0) {
-          if (lane == 0) {
-            st_node[w][sp] = ch[c];
-            st_mask[w][sp] = om;
-          }
-          ++sp;
-        }
+    const float dx = sub_rn(cx, xi), dy = sub_rn(cy, yi), dz = sub_rn(cz, zi);
+    const float d2 = add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
+    bool open = false;
+    if (!leaf) {  // :218-224, open when contains || !(size2 / d2 < theta2)
+      const float size2 = mul_rn(mul_rn(4.0f, h), h);
+      // v_rcp_f32 is within 1 ulp, so size2 * rcp(d2) is within 2 ulp of the
+      // rounded quotient; outside a 2^-20 band around theta2 the comparison
+      // is decided, inside it the IEEE division decides.
+      const float qa = size2 * __builtin_amdgcn_rcpf(d2);
+      bool below = qa < theta2 * (1.0f - 0x1p-20f);
+      const bool above = qa > theta2 * (1.0f + 0x1p-20f);
+      const bool exact = act && !contains && !below && !above;
+      if (__ballot(exact)) {
+        if (exact) below = div_rn(size2, d2) < theta2;
       }
+      open = act && (contains || !below);
+    }
+    const bool interact = act && !open && !skip_self;
+    const float inv = __builtin_amdgcn_rsqf(d2);
+    const float fi = mul_rn(G, meff) * (inv * inv * inv);
+    const float f = interact ? fi : 0.0f;
+    sx = __builtin_fmaf(f, dx, sx);
+    sy = __builtin_fmaf(f, dy, sy);
+    sz = __builtin_fmaf(f, dz, sz);
+    const uint64_t om = __ballot(open);
+    if (om) {  // :234-240
+      const bool mine = lane == depth;
+      keep_at = mine ? after : keep_at;
+      keep_lo = mine ? (uint32_t)M : keep_lo;
+      keep_hi = mine ? (uint32_t)(M >> 32) : keep_hi;
+      ++depth;
+      M = om;
+      v = v + 1;
+    } else {
+      v = after;
     }
   }
   if (valid) {
@@ -967,9 +1042,8 @@ struct nbx_bh {
   int32_t* nvals = nullptr;
   int32_t* final_of = nullptr;
   nbx::RefNode* nodes = nullptr;
-  float4* com4 = nullptr;
-  float4* box4 = nullptr;
-  int4* child4 = nullptr;
+  float4* trav_rec = nullptr;  // 2 records per cell, traversal order
+  int32_t* trav_skip = nullptr;
   nbx::RadixSort sort_particles;
   nbx::RadixSort sort_nodes;
   bool built = false;
@@ -1078,9 +1152,10 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->nvals, NN);
   A(&t->final_of, NN);
   A(&t->nodes, NN);
-  A(&t->com4, NN);
-  A(&t->box4, NN);
-  A(&t->child4, NN * 2);
+  A(&t->T.size, NN);
+  A(&t->T.pre, NN);
+  A(&t->trav_rec, NN * 2);
+  A(&t->trav_skip, NN);
   if (rc == NBX_OK) rc = t->sort_particles.init((int)P);
   if (rc == NBX_OK) rc = t->sort_nodes.init((int)NN);
   for (auto& e : t->ev)
@@ -1163,8 +1238,11 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
                      t->nodes);
   hipLaunchKernelGGL(seq_build, dim3(1), dim3(64), 0, s, t->st, t->idx, d->pos_x, d->pos_y,
                      d->pos_z, d->mass, t->nodes, t->max_nodes);
-  hipLaunchKernelGGL(pack_traversal, dim3(gn), dim3(kThreads), 0, s, t->st, t->nodes, t->com4,
-                     t->box4, t->child4);
+  for (int lv = 0; lv <= kMaxDepth; ++lv)
+    hipLaunchKernelGGL(pre_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv, t->trav_rec,
+                       t->trav_skip);
+  hipLaunchKernelGGL(seq_preorder, dim3(1), dim3(64), 0, s, t->st, t->nodes, t->T.pre, t->T.size,
+                     t->T.k, t->trav_rec, t->trav_skip);
   NBX_LAUNCHED();
   mark(t, 7, s);
   t->built = true;
@@ -1180,8 +1258,8 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   hipStream_t s = as_stream(stream);
   const int n = (int)d->count;
   mark(t, 8, s);
-  hipLaunchKernelGGL(bh_traverse, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->com4,
-                     t->box4, t->child4, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, n,
+  hipLaunchKernelGGL(bh_traverse, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->trav_rec,
+                     t->trav_skip, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, n,
                      theta * theta, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   mark(t, 9, s);
