@@ -906,17 +906,27 @@ __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __re
 
 // ---------------------------------------------------------------- 7: traversal
 
-// Wave-coherent traversal over the preorder layout. The 64 lanes of a wave
-// hold 64 consecutive particles in Morton order and walk the union of their
-// trees in the reference's visiting order (:160-241). State: the current
-// record v and the mask M of lanes that need it. Each lane takes exactly the
+// Wave-coherent traversal over the preorder layout. A wave holds 64 * PPL
+// consecutive particles in Morton order (PPL slots of 64 per lane) and walks
+// the union of their trees in the reference's visiting order (:160-241).
+// State: the current record v and, per slot, the mask M of lanes that need it. Each lane takes exactly the
 // reference's accept/open decisions in the same IEEE arithmetic. If some lanes
 // open v, the walk descends to v + 1 (its first child in this order) with
-// those lanes, remembering (skip[v], M); otherwise it continues at skip[v].
+// those lanes, remembering (skip[v], masks); otherwise it continues at skip[v].
+// With PPL = 2 two particles per lane share the walk's scalar work, but the
+// union of 128 particles' trees grows enough that PPL = 1 is faster on MI355X
+// (1M two-Plummer traversal: 6.7 ms vs 7.0 ms; PPL = 4: 9.3 ms).
 // Arriving at a remembered position restores its mask. Positions only grow,
 // so the walk takes at most one step per record, and the remembered entries
 // are ancestors of v: at most the tree depth (<= 33), kept one per lane in
 // VGPRs. A lane's interactions are summed in the reference's order.
+// Particles per lane: a wave walks the union of 64 * kTravPPL particles' trees.
+#ifndef NBX_BH_PPL
+#define NBX_BH_PPL 1
+#endif
+constexpr int kTravPPL = NBX_BH_PPL;
+
+template <int PPL>
 __global__ __launch_bounds__(kThreads) void bh_traverse(
     const float4* __restrict__ rec, const int32_t* __restrict__ skip,
     const BhState* __restrict__ st, const int32_t* __restrict__ order,
@@ -924,24 +934,42 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
     const float* __restrict__ m, int n, float theta2, float G, float eps2, float* ax, float* ay,
     float* az) {
   const int lane = threadIdx.x & 63;
-  const int r = blockIdx.x * kThreads + threadIdx.x;
-  const bool valid = r < n;
-  const int i = valid ? order[r] : 0;
-  const float xi = x[i], yi = y[i], zi = z[i], mi = m[i];
+  // wave w: particles [w * 64 * PPL, (w + 1) * 64 * PPL) in Morton order;
+  // slot q of lane l holds particle base + q * 64 + l
+  const int base = (blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)) * 64 * PPL;
+  int ip[PPL];
+  bool valid[PPL];
+  float xi[PPL], yi[PPL], zi[PPL], mi[PPL], sx[PPL], sy[PPL], sz[PPL];
+  uint64_t M[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) {
+    const int r = base + q * 64 + lane;
+    valid[q] = r < n;
+    ip[q] = valid[q] ? order[r] : 0;
+    xi[q] = x[ip[q]];
+    yi[q] = y[ip[q]];
+    zi[q] = z[ip[q]];
+    mi[q] = m[ip[q]];
+    sx[q] = sy[q] = sz[q] = 0.f;
+    M[q] = __ballot(valid[q]);
+  }
   const int nn = st->final_count;
-  float sx = 0.f, sy = 0.f, sz = 0.f;
   const uint64_t me = 1ull << lane;
-  uint64_t M = __ballot(valid);
-  int keep_at = 0;          // lane k: position of remembered entry k
-  uint32_t keep_lo = 0, keep_hi = 0;  // lane k: its mask
+  int keep_at = 0;                          // lane k: position of remembered entry k
+  uint32_t keep_lo[PPL], keep_hi[PPL];      // lane k: its masks
+#pragma unroll
+  for (int q = 0; q < PPL; ++q) keep_lo[q] = keep_hi[q] = 0;
   int depth = 0;
   int v = 0;
   int budget = nn + 1;
+  const float theta2_hi = theta2 * (1.0f + 0x1p-20f), theta2_lo = theta2 * (1.0f - 0x1p-20f);
   while (v < nn && budget-- > 0) {
     while (depth > 0 && __builtin_amdgcn_readlane(keep_at, depth - 1) == v) {
       --depth;
-      M = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)keep_hi, depth) << 32) |
-          (uint32_t)__builtin_amdgcn_readlane((int)keep_lo, depth);
+#pragma unroll
+      for (int q = 0; q < PPL; ++q)
+        M[q] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)keep_hi[q], depth) << 32) |
+               (uint32_t)__builtin_amdgcn_readlane((int)keep_lo[q], depth);
     }
     // One 32-byte record plus its skip, waited for together.
     const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 32u;
@@ -955,70 +983,80 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
       v = after;
       continue;
     }
-    const bool act = (M & me) != 0;
     const bool leaf = __builtin_signbit(b4.w);
     const float h = fabsf(b4.w);
-    const bool contains = (fabsf(xi - b4.x) <= h) & (fabsf(yi - b4.y) <= h) &
-                          (fabsf(zi - b4.z) <= h);  // :180-182
-    float meff = c4.w, cx = c4.x, cy = c4.y, cz = c4.z;
-    bool skip_self = false;
-    if (leaf) {
-      const bool rm = act && contains;  // :190-200, the particle's own leaf
-      if (__ballot(rm)) {
-        if (rm) {
-          meff = sub_rn(c4.w, mi);
-          if (meff <= 0.0f) {
-            skip_self = true;
-          } else {
-            cx = div_rn(sub_rn(mul_rn(c4.x, c4.w), mul_rn(xi, mi)), meff);
-            cy = div_rn(sub_rn(mul_rn(c4.y, c4.w), mul_rn(yi, mi)), meff);
-            cz = div_rn(sub_rn(mul_rn(c4.z, c4.w), mul_rn(zi, mi)), meff);
+    const float size2 = mul_rn(mul_rn(4.0f, h), h);
+    uint64_t om[PPL];
+    uint64_t any = 0;
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+      // Lane sets as 64-bit masks: each per-lane test is one compare, and the
+      // sets combine with a few scalar operations.
+      const float far =
+          fmaxf(fmaxf(fabsf(xi[q] - b4.x), fabsf(yi[q] - b4.y)), fabsf(zi[q] - b4.z));
+      const uint64_t cm = __ballot(far <= h) & M[q];  // :180-182, particle inside the cell
+      float meff = c4.w, cx = c4.x, cy = c4.y, cz = c4.z;
+      uint64_t gone = 0;  // lanes whose own leaf holds no other mass
+      if (leaf && cm) {   // :190-200, the particle's own leaf
+        if (cm & me) {
+          meff = sub_rn(c4.w, mi[q]);
+          if (meff > 0.0f) {
+            cx = div_rn(sub_rn(mul_rn(c4.x, c4.w), mul_rn(xi[q], mi[q])), meff);
+            cy = div_rn(sub_rn(mul_rn(c4.y, c4.w), mul_rn(yi[q], mi[q])), meff);
+            cz = div_rn(sub_rn(mul_rn(c4.z, c4.w), mul_rn(zi[q], mi[q])), meff);
           }
         }
+        gone = __ballot(meff <= 0.0f) & cm;
       }
-    }
-    const float dx = sub_rn(cx, xi), dy = sub_rn(cy, yi), dz = sub_rn(cz, zi);
-    const float d2 = add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
-    bool open = false;
-    if (!leaf) {  // :218-224, open when contains || !(size2 / d2 < theta2)
-      const float size2 = mul_rn(mul_rn(4.0f, h), h);
-      // v_rcp_f32 is within 1 ulp, so size2 * rcp(d2) is within 2 ulp of the
-      // rounded quotient; outside a 2^-20 band around theta2 the comparison
-      // is decided, inside it the IEEE division decides.
-      const float qa = size2 * __builtin_amdgcn_rcpf(d2);
-      bool below = qa < theta2 * (1.0f - 0x1p-20f);
-      const bool above = qa > theta2 * (1.0f + 0x1p-20f);
-      const bool exact = act && !contains && !below && !above;
-      if (__ballot(exact)) {
-        if (exact) below = div_rn(size2, d2) < theta2;
+      const float dx = sub_rn(cx, xi[q]), dy = sub_rn(cy, yi[q]), dz = sub_rn(cz, zi[q]);
+      const float d2 =
+          add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
+      om[q] = 0;
+      if (!leaf) {  // :218-224, open when contains || !(size2 / d2 < theta2)
+        // v_rcp_f32 is within 1 ulp, so size2 * rcp(d2) is within 2 ulp of the
+        // rounded quotient; outside a 2^-20 band around theta2 the comparison
+        // is decided, inside it the IEEE division decides.
+        const float qa = size2 * __builtin_amdgcn_rcpf(d2);
+        const uint64_t hm = __ballot(qa > theta2_hi);
+        const uint64_t lm = __ballot(qa < theta2_lo);
+        om[q] = (cm | hm) & M[q];
+        const uint64_t am = M[q] & ~(cm | hm | lm);
+        if (am) {
+          const bool mine = (am & me) != 0;
+          om[q] |= __ballot(mine && !(div_rn(size2, d2) < theta2)) & am;
+        }
       }
-      open = act && (contains || !below);
+      const uint64_t im = M[q] & ~om[q] & ~gone;  // lanes that take the node's force
+      const float inv = __builtin_amdgcn_rsqf(d2);
+      const float fi = mul_rn(G, meff) * (inv * inv * inv);
+      const float f = (im & me) ? fi : 0.0f;
+      sx[q] = __builtin_fmaf(f, dx, sx[q]);
+      sy[q] = __builtin_fmaf(f, dy, sy[q]);
+      sz[q] = __builtin_fmaf(f, dz, sz[q]);
+      any |= om[q];
     }
-    const bool interact = act && !open && !skip_self;
-    const float inv = __builtin_amdgcn_rsqf(d2);
-    const float fi = mul_rn(G, meff) * (inv * inv * inv);
-    const float f = interact ? fi : 0.0f;
-    sx = __builtin_fmaf(f, dx, sx);
-    sy = __builtin_fmaf(f, dy, sy);
-    sz = __builtin_fmaf(f, dz, sz);
-    const uint64_t om = __ballot(open);
-    if (om) {  // :234-240
+    if (any) {  // :234-240
       const bool mine = lane == depth;
       keep_at = mine ? after : keep_at;
-      keep_lo = mine ? (uint32_t)M : keep_lo;
-      keep_hi = mine ? (uint32_t)(M >> 32) : keep_hi;
+#pragma unroll
+      for (int q = 0; q < PPL; ++q) {
+        keep_lo[q] = mine ? (uint32_t)M[q] : keep_lo[q];
+        keep_hi[q] = mine ? (uint32_t)(M[q] >> 32) : keep_hi[q];
+        M[q] = om[q];
+      }
       ++depth;
-      M = om;
       v = v + 1;
     } else {
       v = after;
     }
   }
-  if (valid) {
-    ax[i] = sx;
-    ay[i] = sy;
-    az[i] = sz;
-  }
+#pragma unroll
+  for (int q = 0; q < PPL; ++q)
+    if (valid[q]) {
+      ax[ip[q]] = sx[q];
+      ay[ip[q]] = sy[q];
+      az[ip[q]] = sz[q];
+    }
 }
 
 }  // namespace
@@ -1258,9 +1296,9 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   hipStream_t s = as_stream(stream);
   const int n = (int)d->count;
   mark(t, 8, s);
-  hipLaunchKernelGGL(bh_traverse, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->trav_rec,
-                     t->trav_skip, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, n,
-                     theta * theta, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
+  hipLaunchKernelGGL(bh_traverse<kTravPPL>, dim3(div_up(n, kThreads * kTravPPL)), dim3(kThreads),
+                     0, s, t->trav_rec, t->trav_skip, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z,
+                     d->mass, n, theta * theta, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   mark(t, 9, s);
   return NBX_OK;
