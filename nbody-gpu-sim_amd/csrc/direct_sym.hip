@@ -32,213 +32,11 @@
 #include <tuple>
 
 #include "common.hpp"
-#include "direct_core.hpp"
 #include "direct_sym.hpp"
+#include "direct_sym_core.hpp"
 
 namespace nbx {
 namespace {
-
-constexpr float kPadPos = 1.0e18f;
-
-// Rotation by one lane (lane l receives lane l-1's value) through the LDS
-// crossbar: ds_bpermute_b32 moves 64 dwords without a VALU slot. (A DPP
-// wave_ror:1 move does the same on the VALU; measured on MI355X, the whole
-// kernel runs about 30 % faster with ds_bpermute.) src_addr = ((l + 63) & 63) * 4.
-__device__ __forceinline__ float ror1(float v, int src_addr) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(src_addr, __float_as_int(v)));
-}
-
-struct SymGeom {
-  int n;        // bodies
-  int S;        // bodies per super-block
-  int K;        // super-blocks
-  int Dmax;     // floor(K / 2)
-  long npad;    // K * S
-  float* pi;    // i-side partials [Dmax + 1][3][npad]
-  float* pj;    // j-side partials [Dmax][3][npad]
-};
-
-// Off-diagonal tiles of row A: distances 1 .. sym_rowlen(A, K).
-__host__ __device__ inline int sym_rowlen(int A, int K) {
-  if (K & 1) return (K - 1) / 2;
-  return A < K / 2 ? K / 2 : K / 2 - 1;
-}
-
-// 64 rotation steps of one chunk (xj.. one body per lane) against the wave's
-// IPL targets per lane. UNI: all masses equal, so the mass is left out of both
-// force factors (the caller scales by it). JSIDE: also accumulate the reaction
-// on the chunk's bodies (g*). CHECKED: zero the self pair, which only a
-// diagonal chunk holding the targets of slot self_q has, at step 0.
-template <int IPL, bool UNI, bool JSIDE, bool CHECKED>
-__device__ __forceinline__ void sym_chunk(int src_addr, float& xj, float& yj, float& zj, float& mj,
-                                          const float (&xi)[IPL], const float (&yi)[IPL],
-                                          const float (&zi)[IPL], const float (&mi)[IPL],
-                                          float eps2, int self_q, float (&fx)[IPL],
-                                          float (&fy)[IPL], float (&fz)[IPL], float& gx,
-                                          float& gy, float& gz) {
-#pragma unroll 2
-  for (int s = 0; s < 64; ++s) {
-    float dx[IPL], dy[IPL], dz[IPL], w[IPL];
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-      dx[q] = xj - xi[q];
-      dy[q] = yj - yi[q];
-      dz[q] = zj - zi[q];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-      float r2 = __builtin_fmaf(dx[q], dx[q], eps2);
-      r2 = __builtin_fmaf(dy[q], dy[q], r2);
-      w[q] = __builtin_fmaf(dz[q], dz[q], r2);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) w[q] = __builtin_amdgcn_rsqf(w[q]);
-    __builtin_amdgcn_sched_barrier(0);
-    float wj[IPL];
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-      const float inv = w[q];
-      float c = (inv * inv) * inv;
-      if (CHECKED) c = (s == 0 && q == self_q) ? 0.0f : c;
-      if (UNI) {
-        w[q] = c;
-        wj[q] = c;
-      } else {
-        w[q] = mj * c;
-        wj[q] = mi[q] * c;
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-      fx[q] = __builtin_fmaf(w[q], dx[q], fx[q]);
-      fy[q] = __builtin_fmaf(w[q], dy[q], fy[q]);
-      fz[q] = __builtin_fmaf(w[q], dz[q], fz[q]);
-      if (JSIDE) {
-        gx = __builtin_fmaf(-wj[q], dx[q], gx);
-        gy = __builtin_fmaf(-wj[q], dy[q], gy);
-        gz = __builtin_fmaf(-wj[q], dz[q], gz);
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    xj = ror1(xj, src_addr);
-    yj = ror1(yj, src_addr);
-    zj = ror1(zj, src_addr);
-    if (!UNI) mj = ror1(mj, src_addr);
-    if (JSIDE) {
-      gx = ror1(gx, src_addr);
-      gy = ror1(gy, src_addr);
-      gz = ror1(gz, src_addr);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-__device__ __forceinline__ void load_body(const float* __restrict__ x,
-                                          const float* __restrict__ y,
-                                          const float* __restrict__ z,
-                                          const float* __restrict__ m, int j, int n, float& bx,
-                                          float& by, float& bz, float& bm) {
-  const bool ok = j < n;
-  bx = ok ? x[j] : kPadPos;
-  by = ok ? y[j] : kPadPos;
-  bz = ok ? z[j] : kPadPos;
-  bm = ok ? m[j] : 0.0f;
-}
-
-// One tile: targets of this wave against every chunk of super-block C.
-template <int IPL, int W, bool UNI, bool JSIDE, bool CHECKED>
-__device__ __forceinline__ void sym_tile(const float* __restrict__ x,
-                                         const float* __restrict__ y,
-                                         const float* __restrict__ z,
-                                         const float* __restrict__ m, const SymGeom& g, int A,
-                                         int C, int d, float mref, float eps2,
-                                         const float (&xi)[IPL], const float (&yi)[IPL],
-                                         const float (&zi)[IPL], const float (&mi)[IPL],
-                                         float (*red)[W][3][64]) {
-  constexpr int B = 64 * IPL;
-  constexpr int S = W * B;
-  constexpr int kChunks = S / 64;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int jbase = C * S;
-  const int src_addr = ((lane + 63) & 63) * 4;
-  double sx[IPL], sy[IPL], sz[IPL];
-#pragma unroll
-  for (int q = 0; q < IPL; ++q) sx[q] = sy[q] = sz[q] = 0.0;
-  float nx, ny, nz, nm;
-  load_body(x, y, z, m, jbase + lane, g.n, nx, ny, nz, nm);
-  for (int c = 0; c < kChunks; ++c) {
-    float xj = nx, yj = ny, zj = nz, mj = nm;
-    if (c + 1 < kChunks) load_body(x, y, z, m, jbase + (c + 1) * 64 + lane, g.n, nx, ny, nz, nm);
-    float fx[IPL], fy[IPL], fz[IPL];
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) fx[q] = fy[q] = fz[q] = 0.0f;
-    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-    int self_q = -1;
-    if (CHECKED) {
-      const int q = c - wave * IPL;
-      self_q = (q >= 0 && q < IPL) ? q : -1;
-    }
-    sym_chunk<IPL, UNI, JSIDE, CHECKED>(src_addr, xj, yj, zj, mj, xi, yi, zi, mi, eps2, self_q,
-                                        fx, fy, fz, gx, gy, gz);
-#pragma unroll
-    for (int q = 0; q < IPL; ++q) {
-      if (UNI) {
-        sx[q] += (double)fx[q] * (double)mref;
-        sy[q] += (double)fy[q] * (double)mref;
-        sz[q] += (double)fz[q] * (double)mref;
-      } else {
-        sx[q] += (double)fx[q];
-        sy[q] += (double)fy[q];
-        sz[q] += (double)fz[q];
-      }
-    }
-    if (JSIDE) {
-      float(*buf)[3][64] = red[c & 1];
-      buf[wave][0][lane] = gx;
-      buf[wave][1][lane] = gy;
-      buf[wave][2][lane] = gz;
-      __syncthreads();
-      // 3 components x 64 bodies, whatever the workgroup size
-      for (int r = threadIdx.x; r < 192; r += 64 * W) {
-        const int k = r >> 6, l = r & 63;
-        double t = 0.0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) t += (double)buf[w][k][l];
-        if (UNI) t *= (double)mref;
-        g.pj[((long)(d - 1) * 3 + k) * g.npad + jbase + c * 64 + l] = (float)t;
-      }
-      // red[c & 1] is rewritten at chunk c + 2, after every thread has
-      // passed the barrier of chunk c + 1, so one barrier per chunk suffices.
-    }
-  }
-  const int ibase = A * S + wave * B;
-#pragma unroll
-  for (int q = 0; q < IPL; ++q) {
-    const long i = ibase + q * 64 + lane;
-    g.pi[((long)d * 3 + 0) * g.npad + i] = (float)sx[q];
-    g.pi[((long)d * 3 + 1) * g.npad + i] = (float)sy[q];
-    g.pi[((long)d * 3 + 2) * g.npad + i] = (float)sz[q];
-  }
-}
-
-template <int IPL, int W, bool UNI, bool CHECKED>
-__device__ __forceinline__ void sym_dispatch(const float* __restrict__ x,
-                                             const float* __restrict__ y,
-                                             const float* __restrict__ z,
-                                             const float* __restrict__ m, const SymGeom& g, int A,
-                                             int C, int d, float mref, float eps2,
-                                             const float (&xi)[IPL], const float (&yi)[IPL],
-                                             const float (&zi)[IPL], const float (&mi)[IPL],
-                                             float (*red)[W][3][64]) {
-  if (d == 0)
-    sym_tile<IPL, W, UNI, false, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
-  else
-    sym_tile<IPL, W, UNI, true, false>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
-}
 
 template <int IPL, int W, bool CHECKED>
 __global__ __launch_bounds__(64 * W) void direct_sym_kernel(const float* __restrict__ x,
@@ -246,68 +44,18 @@ __global__ __launch_bounds__(64 * W) void direct_sym_kernel(const float* __restr
                                                             const float* __restrict__ z,
                                                             const float* __restrict__ m,
                                                             SymGeom g, float eps2) {
-  constexpr int B = 64 * IPL;
-  constexpr int S = W * B;
   __shared__ float red[2][W][3][64];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int t = blockIdx.x;
   int A, d;
-  if (t < g.K) {
-    A = t;
-    d = 0;
-  } else {
-    A = (t - g.K) / g.Dmax;
-    d = (t - g.K) % g.Dmax + 1;
-    if (d > sym_rowlen(A, g.K)) return;  // whole workgroup: tile owned by the other row
-  }
-  const int C = (A + d) % g.K;
-  float xi[IPL], yi[IPL], zi[IPL], mi[IPL];
-#pragma unroll
-  for (int q = 0; q < IPL; ++q)
-    load_body(x, y, z, m, A * S + wave * B + q * 64 + lane, g.n, xi[q], yi[q], zi[q], mi[q]);
-  // Do all real bodies of A and C share one mass?
-  const float mref = m[A * S];
-  bool same = true;
-  for (int k = threadIdx.x; k < S; k += 64 * W) {
-    const int ia = A * S + k, jc = C * S + k;
-    if (ia < g.n) same = same && (m[ia] == mref);
-    if (jc < g.n) same = same && (m[jc] == mref);
-  }
-  if (__syncthreads_and(same))
-    sym_dispatch<IPL, W, true, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
-  else
-    sym_dispatch<IPL, W, false, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
+  if (!sym_tile_of(g, (int)blockIdx.x, A, d)) return;  // whole workgroup: the other row's tile
+  sym_run_tile<IPL, W, CHECKED>(x, y, z, m, g, A, d, eps2, red);
 }
 
-// a_b = G * (sum of b's i-side partials, d ascending, then its j-side
-// partials, d ascending), in FP64; optional kick as in direct_tile().
 template <bool KICK>
 __global__ __launch_bounds__(256) void sym_reduce_kernel(SymGeom g, double G, float* ax,
                                                          float* ay, float* az, KickArgs kick) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= g.n) return;
-  const int C = b / g.S;
-  double t[3] = {0.0, 0.0, 0.0};
-  const int DC = sym_rowlen(C, g.K);
-  for (int d = 0; d <= DC; ++d)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) t[k] += (double)g.pi[((long)d * 3 + k) * g.npad + b];
-  for (int d = 1; d <= g.Dmax; ++d) {
-    const int A = (C - d + g.K) % g.K;
-    if (d > sym_rowlen(A, g.K)) continue;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) t[k] += (double)g.pj[((long)(d - 1) * 3 + k) * g.npad + b];
-  }
-  const float a0 = (float)(G * t[0]), a1 = (float)(G * t[1]), a2 = (float)(G * t[2]);
-  ax[b] = a0;
-  ay[b] = a1;
-  az[b] = a2;
-  if (KICK) {
-    kick.vx[b] = add_rn(kick.vx[b], mul_rn(add_rn(kick.aox[b], a0), kick.dt_half));
-    kick.vy[b] = add_rn(kick.vy[b], mul_rn(add_rn(kick.aoy[b], a1), kick.dt_half));
-    kick.vz[b] = add_rn(kick.vz[b], mul_rn(add_rn(kick.aoz[b], a2), kick.dt_half));
-  }
+  sym_reduce_body<KICK>(g, b, G, ax, ay, az, kick);
 }
 
 // Partial buffers, one set per (device, stream): calls on one stream are
@@ -318,34 +66,25 @@ std::map<std::pair<int, hipStream_t>, Scratch> g_sym_scratch;
 
 }  // namespace
 
-size_t direct_sym_partial_bytes(size_t n, int S) {
-  const size_t K = (n + S - 1) / S;
-  const size_t D = K / 2;
-  return (2 * D + 1) * 3 * K * (size_t)S * sizeof(float);
-}
+size_t direct_sym_partial_bytes(size_t n, int S) { return sym_partial_floats(n, S) * sizeof(float); }
 
 template <int IPL, int W>
 int direct_sym_launch(const nbx_particles* d, double G, float eps2, bool checked,
                       const KickArgs* kick, hipStream_t s) {
   constexpr int S = 64 * IPL * W;
   const int n = (int)d->count;
-  SymGeom g{};
-  g.n = n;
-  g.S = S;
-  g.K = (n + S - 1) / S;
-  g.Dmax = g.K / 2;
-  g.npad = (long)g.K * S;
   const size_t bytes = direct_sym_partial_bytes((size_t)n, S);
+  float* part = nullptr;
   {
     int dev = 0;
     NBX_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_sym_mu);
     Scratch& sc = g_sym_scratch[std::make_pair(dev, s)];
     NBX_TRY(sc.ensure(bytes));
-    g.pi = (float*)sc.ptr;
-    g.pj = g.pi + (size_t)(g.Dmax + 1) * 3 * g.npad;
+    part = (float*)sc.ptr;
   }
-  const unsigned tiles = (unsigned)(g.K + (size_t)g.K * g.Dmax);
+  const SymGeom g = sym_geom(n, S, part);
+  const unsigned tiles = (unsigned)sym_tiles(g);
   if (checked)
     hipLaunchKernelGGL((direct_sym_kernel<IPL, W, true>), dim3(tiles), dim3(64 * W), 0, s,
                        d->pos_x, d->pos_y, d->pos_z, d->mass, g, eps2);

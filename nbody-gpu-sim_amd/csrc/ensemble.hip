@@ -19,6 +19,8 @@
 
 #include "common.hpp"
 #include "direct_core.hpp"
+#include "direct_sym.hpp"
+#include "direct_sym_core.hpp"
 #include "verlet.hpp"
 
 namespace nbx {
@@ -62,6 +64,68 @@ __global__ __launch_bounds__(64 * WS) void ens_forces_kernel(
       __threadfence();
     }
   }
+}
+
+// Pair-symmetric batched forces: workgroups pull (system, tile) items, largest
+// systems first, and run the tiles of the single-system symmetric kernel on
+// that system's slice (sym_run_tile), writing its partial sums. The queue
+// resets as in ens_forces_kernel.
+template <int IPL, int W, bool CHECKED>
+__global__ __launch_bounds__(64 * W) void ens_sym_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    const float* __restrict__ m, const int32_t* __restrict__ sys_off,
+    const int32_t* __restrict__ sys_n, const int64_t* __restrict__ part_off, float* part,
+    const int2* __restrict__ items, int nitems, int* queue, int* done, float eps2) {
+  constexpr int S = 64 * IPL * W;
+  __shared__ float red[2][W][3][64];
+  __shared__ int item_sh;
+  for (;;) {
+    if (threadIdx.x == 0) item_sh = atomicAdd(queue, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(item_sh);
+    __syncthreads();
+    if (it >= nitems) break;
+    const int2 w = items[it];
+    const int off = sys_off[w.x];
+    const SymGeom g = sym_geom(sys_n[w.x], S, part + part_off[w.x]);
+    int A, d;
+    sym_tile_of(g, w.y, A, d);  // items hold only owned tiles
+    sym_run_tile<IPL, W, CHECKED>(x + off, y + off, z + off, m + off, g, A, d, eps2, red);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(done, 1) == (int)gridDim.x - 1) {  // every workgroup is past the queue
+      atomicExch(queue, 0);
+      atomicExch(done, 0);
+      __threadfence();
+    }
+  }
+}
+
+// Accelerations (and kick) from the partial sums, one workgroup per
+// (system, block of 256 bodies).
+template <bool KICK>
+__global__ __launch_bounds__(256) void ens_sym_reduce(
+    const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_first,
+    const int32_t* __restrict__ sys_off, const int32_t* __restrict__ sys_n,
+    const int64_t* __restrict__ part_off, float* part, int S, double G, float* ax, float* ay,
+    float* az, KickArgs kick) {
+  const int s = blk_sys[blockIdx.x];
+  const int b = blk_first[blockIdx.x] + threadIdx.x;
+  const int n = sys_n[s];
+  if (b >= n) return;
+  const int off = sys_off[s];
+  const SymGeom g = sym_geom(n, S, part + part_off[s]);
+  KickArgs k = kick;
+  if (KICK) {
+    k.vx += off;
+    k.vy += off;
+    k.vz += off;
+    k.aox += off;
+    k.aoy += off;
+    k.aoz += off;
+  }
+  sym_reduce_body<KICK>(g, b, G, ax + off, ay + off, az + off, k);
 }
 
 // Per-system FP64 kinetic energy and softened potential, one workgroup per
@@ -115,20 +179,31 @@ __global__ __launch_bounds__(256) void ens_energy_kernel(
 struct EnsVariant {
   const char* name;
   int ipl, ws;
-  void* k[2][2];  // [checked][kick]
+  int sym_block;  // 0: one-sided tiles; else the symmetric super-block size
+  void* k[2][2];  // [checked][kick]; symmetric shapes apply the kick in ens_sym_reduce
 };
 
 #define NBX_ENS_V(I, B, W)                                                        \
   {"ipl" #I "_nb" #B "_ws" #W,                                                    \
    I,                                                                             \
    W,                                                                             \
+   0,                                                                             \
    {{(void*)ens_forces_kernel<I, B, W, false, false>,                             \
      (void*)ens_forces_kernel<I, B, W, false, true>},                             \
     {(void*)ens_forces_kernel<I, B, W, true, false>,                              \
      (void*)ens_forces_kernel<I, B, W, true, true>}}},
+#define NBX_ENS_SYM_V(I, W)                                                       \
+  {"sym_ipl" #I "_ws" #W,                                                         \
+   I,                                                                             \
+   W,                                                                             \
+   64 * I * W,                                                                    \
+   {{(void*)ens_sym_kernel<I, W, false>, (void*)ens_sym_kernel<I, W, false>},     \
+    {(void*)ens_sym_kernel<I, W, true>, (void*)ens_sym_kernel<I, W, true>}}},
 // Same tile list and order as the single-system variants (direct.hip).
-static const EnsVariant kEnsVariants[] = {NBX_DIRECT_VARIANTS(NBX_ENS_V)};
+static const EnsVariant kEnsVariants[] = {NBX_DIRECT_VARIANTS(NBX_ENS_V)
+                                              NBX_SYM_VARIANTS(NBX_ENS_SYM_V)};
 #undef NBX_ENS_V
+#undef NBX_ENS_SYM_V
 constexpr int kEnsNumVariants = (int)(sizeof(kEnsVariants) / sizeof(kEnsVariants[0]));
 
 }  // namespace
@@ -145,6 +220,8 @@ struct nbx_ensemble {
   int variant = 0;
   int2* d_items[nbx::kEnsNumVariants] = {};
   int nitems[nbx::kEnsNumVariants] = {};
+  int64_t* d_part_off[nbx::kEnsNumVariants] = {};  // symmetric shapes: per-system partial offset
+  float* d_part = nullptr;                          // symmetric partial sums (largest shape)
   int grid[nbx::kEnsNumVariants][2][2] = {};
   hipStream_t own_stream = nullptr;
   // captured step chunk
@@ -174,12 +251,30 @@ int ens_forces_launch(nbx_ensemble* e, float G, float eps2, const KickArgs* kick
   const nbx_particles& p = e->p;
   const double Gd = (double)G;
   int nitems = e->nitems[v];
+  int* done = e->d_queue + 1;
+  if (V.sym_block) {
+    void* args[] = {(void*)&p.pos_x,       (void*)&p.pos_y,       (void*)&p.pos_z,
+                    (void*)&p.mass,        (void*)&e->d_off,      (void*)&e->d_n,
+                    (void*)&e->d_part_off[v], (void*)&e->d_part,  (void*)&e->d_items[v],
+                    (void*)&nitems,        (void*)&e->d_queue,    (void*)&done,
+                    (void*)&eps2};
+    NBX_HIP(hipLaunchKernel(V.k[chk][kk], dim3(e->grid[v][chk][kk]), dim3(64 * V.ws), args, 0, s));
+    const int S = V.sym_block;
+    if (kick)
+      hipLaunchKernelGGL(ens_sym_reduce<true>, dim3(e->eblocks), dim3(256), 0, s, e->d_blk_sys,
+                         e->d_blk_first, e->d_off, e->d_n, e->d_part_off[v], e->d_part, S, Gd,
+                         p.acc_x, p.acc_y, p.acc_z, k);
+    else
+      hipLaunchKernelGGL(ens_sym_reduce<false>, dim3(e->eblocks), dim3(256), 0, s, e->d_blk_sys,
+                         e->d_blk_first, e->d_off, e->d_n, e->d_part_off[v], e->d_part, S, Gd,
+                         p.acc_x, p.acc_y, p.acc_z, k);
+    NBX_LAUNCHED();
+    return NBX_OK;
+  }
   void* args[] = {(void*)&p.pos_x,  (void*)&p.pos_y, (void*)&p.pos_z,   (void*)&p.mass,
                   (void*)&e->d_off, (void*)&e->d_n,  (void*)&e->d_items[v], (void*)&nitems,
-                  (void*)&e->d_queue, nullptr,       (void*)&Gd,        (void*)&eps2,
+                  (void*)&e->d_queue, (void*)&done,  (void*)&Gd,        (void*)&eps2,
                   (void*)&p.acc_x,  (void*)&p.acc_y, (void*)&p.acc_z,   (void*)&k};
-  int* done = e->d_queue + 1;
-  args[9] = (void*)&done;
   NBX_HIP(hipLaunchKernel(V.k[chk][kk], dim3(e->grid[v][chk][kk]), dim3(64 * V.ws), args, 0, s));
   return NBX_OK;
 }
@@ -200,6 +295,9 @@ void ens_release(nbx_ensemble* e) {
     if (b) (void)hipFree(b);
   for (auto* it : e->d_items)
     if (it) (void)hipFree(it);
+  for (auto* it : e->d_part_off)
+    if (it) (void)hipFree(it);
+  if (e->d_part) (void)hipFree(e->d_part);
   if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
 }
 
@@ -258,11 +356,34 @@ int nbx_ens_create(int nsys, const int64_t* counts, nbx_ensemble** out) {
   hipDeviceProp_t prop;
   if (rc == NBX_OK && hipGetDeviceProperties(&prop, 0) != hipSuccess)
     rc = fail(NBX_ERR_HIP, "hipGetDeviceProperties failed");
+  size_t part_floats = 0;
   for (int v = 0; v < kEnsNumVariants && rc == NBX_OK; ++v) {
-    const int tile = 64 * kEnsVariants[v].ipl;
+    const EnsVariant& V = kEnsVariants[v];
     std::vector<int2> items;
-    for (int s : order)
-      for (int t = 0; t * tile < n[s]; ++t) items.push_back(make_int2(s, t));
+    if (!V.sym_block) {
+      const int tile = 64 * V.ipl;
+      for (int s : order)
+        for (int t = 0; t * tile < n[s]; ++t) items.push_back(make_int2(s, t));
+    } else {
+      // owned tiles of every system, diagonal tiles first; partial sums back to back
+      std::vector<int64_t> poff(nsys);
+      size_t at = 0;
+      for (int s = 0; s < nsys; ++s) {
+        poff[s] = (int64_t)at;
+        at += sym_partial_floats((size_t)n[s], V.sym_block);
+      }
+      part_floats = std::max(part_floats, at);
+      for (int s : order) {
+        const int K = (n[s] + V.sym_block - 1) / V.sym_block, D = K / 2;
+        for (int t = 0; t < K + K * D; ++t)
+          if (t < K || (t - K) % D + 1 <= sym_rowlen((t - K) / D, K))
+            items.push_back(make_int2(s, t));
+      }
+      dmal((void**)&e->d_part_off[v], sizeof(int64_t) * nsys);
+      if (rc == NBX_OK && hipMemcpy(e->d_part_off[v], poff.data(), sizeof(int64_t) * nsys,
+                                    hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(NBX_ERR_HIP, "partial offset upload failed");
+    }
     e->nitems[v] = (int)items.size();
     dmal((void**)&e->d_items[v], sizeof(int2) * items.size());
     if (rc == NBX_OK &&
@@ -272,12 +393,13 @@ int nbx_ens_create(int nsys, const int64_t* counts, nbx_ensemble** out) {
     for (int c = 0; c < 2 && rc == NBX_OK; ++c)
       for (int k = 0; k < 2 && rc == NBX_OK; ++k) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kEnsVariants[v].k[c][k],
-                                                         64 * kEnsVariants[v].ws, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, V.k[c][k], 64 * V.ws, 0) !=
+            hipSuccess)
           rc = fail(NBX_ERR_HIP, "occupancy query failed");
         e->grid[v][c][k] = std::max(1, per_cu) * prop.multiProcessorCount;
       }
   }
+  if (part_floats) dmal((void**)&e->d_part, sizeof(float) * part_floats);
   // energy blocks: (system, first target) pairs
   std::vector<int32_t> bs, bf;
   for (int s = 0; s < nsys; ++s)

@@ -40,7 +40,10 @@ def standalone(nbx, h, sl, variant):
     return d.acc()
 
 
-@pytest.mark.parametrize("variant", [0, 4, 6, 7])
+# one-sided shapes 0, 4, 6, 7 and the pair-symmetric shapes 8 (sym_ipl8_ws4) and
+# 9 (sym_ipl4_ws4): the systems of 1, 65 and up to 3000 bodies give 1 to 3
+# super-blocks, so diagonal-only, even and odd tile sets all occur
+@pytest.mark.parametrize("variant", [0, 4, 6, 7, 8, 9])
 def test_batched_forces_bit_identical_to_standalone(nbx, small, variant):
     counts, h = small
     ens = nbx.Ensemble(counts)
@@ -54,10 +57,12 @@ def test_batched_forces_bit_identical_to_standalone(nbx, small, variant):
         assert np.array_equal(acc[ens.system(q)], ref), f"system {q} (n={counts[q]})"
 
 
-def test_batched_forces_vs_fp64(nbx, orc, small):
+@pytest.mark.parametrize("variant", [4, 9])
+def test_batched_forces_vs_fp64(nbx, orc, small, variant):
     counts, h = small
     ens = nbx.Ensemble(counts)
     ens.upload(h)
+    ens.set_variant(variant)
     ens.forces(G, e2())
     nbx.synchronize()
     acc = np.stack([ens.read("acc_x"), ens.read("acc_y"), ens.read("acc_z")], 1).astype(np.float64)
@@ -72,12 +77,19 @@ def test_batched_forces_vs_fp64(nbx, orc, small):
     assert np.all(acc[ens.system(0)] == 0)  # lone particle
 
 
-def test_graph_steps_equal_eager_and_sequential(nbx, small):
+def test_variant_names_match_single_system(nbx):
+    # a batched shape index names the same tile shape as the single-system one
+    assert nbx.lib.nbx_ens_variant_count() == nbx.lib.nbx_direct_variant_count()
+    assert nbx.variant_names()[9] == "sym_ipl4_ws4" and nbx.variant_names()[8] == "sym_ipl8_ws4"
+
+
+@pytest.mark.parametrize("variant", [2, 9])
+def test_graph_steps_equal_eager_and_sequential(nbx, small, variant):
     counts, h = small
     a, b, c = nbx.Ensemble(counts), nbx.Ensemble(counts), nbx.Ensemble(counts)
     for e in (a, b, c):
         e.upload(h)
-        e.set_variant(2)
+        e.set_variant(variant)
         e.forces(G, e2())
     nbx.synchronize()
     s = nbx.Stream()
@@ -89,7 +101,7 @@ def test_graph_steps_equal_eager_and_sequential(nbx, small):
         s.synchronize()
         for f in ("pos_x", "vel_z", "acc_y"):
             assert np.array_equal(a.read(f), b.read(f)), (rep, f)
-    nbx.check(nbx.lib.nbx_direct_set_variant(2))
+    nbx.check(nbx.lib.nbx_direct_set_variant(variant))
     try:
         c.steps_sequential(G, e2(), DT, 12, stream=s)
         s.synchronize()
