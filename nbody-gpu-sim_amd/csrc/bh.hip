@@ -665,13 +665,15 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
 // order the reference's stack walk visits them (:160-241: children pushed
 // 0..7, so popped 7..0), i.e. depth-first preorder with children 7 down to 0.
 // Record p holds (centre of mass, mass) and (cell centre, half size with the
-// leaf flag in its sign); skip[p] is the position after the cell's subtree.
+// leaf flag in its sign); aux[p] holds the position after the cell's subtree
+// and the opening-test size (2h)^2 = 4 h h, rounded as the traversal's test
+// (:218-224) rounds it.
 __device__ __forceinline__ float4 trav_box(float4 b, bool leaf) {
   return make_float4(b.x, b.y, b.z, leaf ? -b.w - 0.0f : b.w);
 }
 
 __global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict__ st, TmpNodes T,
-                                                      int d, float4* rec, int32_t* skip) {
+                                                      int d, float4* rec, int2* aux) {
   if (st->use_seq) return;
   const int b = st->level_start[d], e = st->level_start[d + 1];
   for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
@@ -679,7 +681,8 @@ __global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict_
     const bool leaf = (T.info[t] >> 11) & 1u;
     rec[2 * pt] = T.com[t];
     rec[2 * pt + 1] = trav_box(T.box[t], leaf);
-    skip[pt] = pt + T.size[t];
+    aux[pt] = make_int2(pt + T.size[t],
+                        __float_as_int(mul_rn(mul_rn(4.0f, T.box[t].w), T.box[t].w)));
     if (leaf) continue;
     int next = pt + 1;
     for (int o = 7; o >= 0; --o) {
@@ -873,7 +876,7 @@ __global__ void seq_build(BhState* st, const int32_t* __restrict__ sidx, const f
 // always come after it). pre_of, size and order are scratch of max_nodes ints.
 __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __restrict__ nodes,
                              int32_t* pre_of, int32_t* size, int32_t* order, float4* rec,
-                             int32_t* skip) {
+                             int2* aux) {
   if (!st->use_seq || threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nn = st->final_count;
   if (nn <= 0) return;
@@ -900,7 +903,7 @@ __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __re
     size[v] = sz;
     rec[2 * p] = make_float4(q.comx, q.comy, q.comz, q.mass);
     rec[2 * p + 1] = make_float4(q.cx, q.cy, q.cz, q.leaf ? -q.half - 0.0f : q.half);
-    skip[p] = p + sz;
+    aux[p] = make_int2(p + sz, __float_as_int(mul_rn(mul_rn(4.0f, q.half), q.half)));
   }
 }
 
@@ -928,7 +931,7 @@ constexpr int kTravPPL = NBX_BH_PPL;
 
 template <int PPL>
 __global__ __launch_bounds__(kThreads) void bh_traverse(
-    const float4* __restrict__ rec, const int32_t* __restrict__ skip,
+    const float4* __restrict__ rec, const int2* __restrict__ aux,
     const BhState* __restrict__ st, const int32_t* __restrict__ order,
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     const float* __restrict__ m, int n, float theta2, float G, float eps2, float* ax, float* ay,
@@ -971,21 +974,22 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
         M[q] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)keep_hi[q], depth) << 32) |
                (uint32_t)__builtin_amdgcn_readlane((int)keep_lo[q], depth);
     }
-    // One 32-byte record plus its skip, waited for together.
+    // One 32-byte record plus (skip, (2h)^2), waited for together.
     const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 32u;
     const float4 c4 = *reinterpret_cast<const float4*>(rp);
     const float4 b4 = *reinterpret_cast<const float4*>(rp + 16);
-    const int after =
-        *reinterpret_cast<const int32_t*>(reinterpret_cast<const char*>(skip) + (uint32_t)v * 4u);
+    const int2 ax2 =
+        *reinterpret_cast<const int2*>(reinterpret_cast<const char*>(aux) + (uint32_t)v * 8u);
+    const int after = ax2.x;
+    const float size2 = __int_as_float(ax2.y);
     asm volatile("" ::"s"(c4.x), "s"(c4.y), "s"(c4.z), "s"(c4.w), "s"(b4.x), "s"(b4.y),
-                 "s"(b4.z), "s"(b4.w), "s"(after));
+                 "s"(b4.z), "s"(b4.w), "s"(after), "s"(size2));
     if (c4.w == 0.0f) {  // :172-173
       v = after;
       continue;
     }
     const bool leaf = __builtin_signbit(b4.w);
     const float h = fabsf(b4.w);
-    const float size2 = mul_rn(mul_rn(4.0f, h), h);
     uint64_t om[PPL];
     uint64_t any = 0;
 #pragma unroll
@@ -1013,12 +1017,11 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
           add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
       om[q] = 0;
       if (!leaf) {  // :218-224, open when contains || !(size2 / d2 < theta2)
-        // v_rcp_f32 is within 1 ulp, so size2 * rcp(d2) is within 2 ulp of the
-        // rounded quotient; outside a 2^-20 band around theta2 the comparison
-        // is decided, inside it the IEEE division decides.
-        const float qa = size2 * __builtin_amdgcn_rcpf(d2);
-        const uint64_t hm = __ballot(qa > theta2_hi);
-        const uint64_t lm = __ballot(qa < theta2_lo);
+        // size2 against d2 * theta2 (1 -+ 2^-20): each product is within half
+        // an ulp, so outside that band the rounded quotient size2 / d2 is
+        // certainly below or above theta2; inside it the IEEE division decides.
+        const uint64_t hm = __ballot(size2 > d2 * theta2_hi);
+        const uint64_t lm = __ballot(size2 < d2 * theta2_lo);
         om[q] = (cm | hm) & M[q];
         const uint64_t am = M[q] & ~(cm | hm | lm);
         if (am) {
@@ -1029,7 +1032,8 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
       const uint64_t im = M[q] & ~om[q] & ~gone;  // lanes that take the node's force
       const float inv = __builtin_amdgcn_rsqf(d2);
       const float fi = mul_rn(G, meff) * (inv * inv * inv);
-      const float f = (im & me) ? fi : 0.0f;
+      float f;  // fi on the lanes of im, else 0: one select on the lane mask
+      asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(f) : "v"(fi), "s"(im));
       sx[q] = __builtin_fmaf(f, dx, sx[q]);
       sy[q] = __builtin_fmaf(f, dy, sy[q]);
       sz[q] = __builtin_fmaf(f, dz, sz[q]);
@@ -1081,7 +1085,7 @@ struct nbx_bh {
   int32_t* final_of = nullptr;
   nbx::RefNode* nodes = nullptr;
   float4* trav_rec = nullptr;  // 2 records per cell, traversal order
-  int32_t* trav_skip = nullptr;
+  int2* trav_aux = nullptr;  // per record: skip position, (2h)^2
   nbx::RadixSort sort_particles;
   nbx::RadixSort sort_nodes;
   bool built = false;
@@ -1193,7 +1197,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->T.size, NN);
   A(&t->T.pre, NN);
   A(&t->trav_rec, NN * 2);
-  A(&t->trav_skip, NN);
+  A(&t->trav_aux, NN);
   if (rc == NBX_OK) rc = t->sort_particles.init((int)P);
   if (rc == NBX_OK) rc = t->sort_nodes.init((int)NN);
   for (auto& e : t->ev)
@@ -1278,9 +1282,9 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
                      d->pos_z, d->mass, t->nodes, t->max_nodes);
   for (int lv = 0; lv <= kMaxDepth; ++lv)
     hipLaunchKernelGGL(pre_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv, t->trav_rec,
-                       t->trav_skip);
+                       t->trav_aux);
   hipLaunchKernelGGL(seq_preorder, dim3(1), dim3(64), 0, s, t->st, t->nodes, t->T.pre, t->T.size,
-                     t->T.k, t->trav_rec, t->trav_skip);
+                     t->T.k, t->trav_rec, t->trav_aux);
   NBX_LAUNCHED();
   mark(t, 7, s);
   t->built = true;
@@ -1297,7 +1301,7 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   const int n = (int)d->count;
   mark(t, 8, s);
   hipLaunchKernelGGL(bh_traverse<kTravPPL>, dim3(div_up(n, kThreads * kTravPPL)), dim3(kThreads),
-                     0, s, t->trav_rec, t->trav_skip, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z,
+                     0, s, t->trav_rec, t->trav_aux, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z,
                      d->mass, n, theta * theta, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   mark(t, 9, s);
