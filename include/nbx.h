@@ -178,6 +178,9 @@ typedef struct nbx_ds_particles {
 int nbx_ds_alloc(nbx_ds_particles* p, size_t count); /* lo parts zeroed */
 int nbx_ds_free(nbx_ds_particles* p);
 int nbx_ds_forces(nbx_ds_particles* d, float G, float eps2, nbx_stream s);
+/* Kernel of nbx_ds_forces / nbx_ds_step: -1 pair-symmetric from 2^15 bodies (the
+ * default), 0 one-sided, 1 pair-symmetric. */
+int nbx_ds_set_kernel(int kind);
 /* store+drift in double-single: (hi,lo) += v*dt + a*(0.5*dt*dt) with renormalisation. */
 int nbx_ds_drift(nbx_ds_particles* d, float dt, nbx_stream s);
 /* drift, forces, kick. */

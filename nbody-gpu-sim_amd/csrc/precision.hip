@@ -8,8 +8,13 @@
 //  * each target's sum is Kahan-compensated in FP32 over its whole j chunk;
 //  * FP64 appears only in the cross-wave reduction of the compensated sums, the
 //    hi/lo renormalisation of the drift, and the energy/momentum reductions.
+#include <map>
+#include <mutex>
+
 #include "common.hpp"
 #include "direct_core.hpp"
+#include "direct_sym.hpp"
+#include "direct_sym_core.hpp"
 #include "verlet.hpp"
 
 namespace nbx {
@@ -129,6 +134,216 @@ __global__ __launch_bounds__(64 * WS) void ds_forces_kernel(
   }
 }
 
+// ---- pair-symmetric form (the default from 2^15 bodies) --------------------
+// The tiles, partial sums and reduction of the single-system symmetric kernel
+// (direct_sym_core.hpp) with double-single positions and Kahan-compensated
+// sums on both sides of every pair: the targets' sums per chunk, and the
+// reactions, whose sums and compensations rotate with their bodies.
+
+__device__ __forceinline__ void load_ds_body(const float* __restrict__ xh,
+                                             const float* __restrict__ yh,
+                                             const float* __restrict__ zh,
+                                             const float* __restrict__ xl,
+                                             const float* __restrict__ yl,
+                                             const float* __restrict__ zl,
+                                             const float* __restrict__ m, int j, int n,
+                                             float (&b)[7]) {
+  const bool ok = j < n;
+  b[0] = ok ? xh[j] : kPadPos;
+  b[1] = ok ? yh[j] : kPadPos;
+  b[2] = ok ? zh[j] : kPadPos;
+  b[3] = ok ? xl[j] : 0.0f;
+  b[4] = ok ? yl[j] : 0.0f;
+  b[5] = ok ? zl[j] : 0.0f;
+  b[6] = ok ? m[j] : 0.0f;
+}
+
+// s + v with the lost low part kept in c (Kahan).
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+template <int IPL, bool UNI, bool JSIDE, bool CHECKED>
+__device__ __forceinline__ void ds_sym_chunk(int src_addr, float (&b)[7],
+                                             const float (&ih)[IPL][3], const float (&il)[IPL][3],
+                                             const float (&mi)[IPL], float eps2, int self_q,
+                                             float (&s)[IPL][3], float (&c)[IPL][3],
+                                             float (&g)[3], float (&cg)[3]) {
+#pragma unroll 2
+  for (int st = 0; st < 64; ++st) {
+    float d[IPL][3], w[IPL], wj[IPL];
+#pragma unroll
+    for (int q = 0; q < IPL; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) d[q][a] = (b[a] - ih[q][a]) + (b[3 + a] - il[q][a]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+      float r2 = __builtin_fmaf(d[q][0], d[q][0], eps2);
+      r2 = __builtin_fmaf(d[q][1], d[q][1], r2);
+      w[q] = __builtin_fmaf(d[q][2], d[q][2], r2);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) w[q] = __builtin_amdgcn_rsqf(w[q]);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < IPL; ++q) {
+      const float inv = w[q];
+      float f = (inv * inv) * inv;
+      if (CHECKED) f = (st == 0 && q == self_q) ? 0.0f : f;
+      w[q] = UNI ? f : b[6] * f;
+      wj[q] = UNI ? f : mi[q] * f;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < IPL; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        kahan_add(s[q][a], c[q][a], w[q] * d[q][a]);
+        if (JSIDE) kahan_add(g[a], cg[a], -(wj[q] * d[q][a]));
+      }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) b[k] = ror1(b[k], src_addr);
+    if (!UNI) b[6] = ror1(b[6], src_addr);
+    if (JSIDE)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        g[a] = ror1(g[a], src_addr);
+        cg[a] = ror1(cg[a], src_addr);
+      }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int IPL, int W, bool UNI, bool JSIDE, bool CHECKED>
+__device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
+                                            const float* __restrict__ yh,
+                                            const float* __restrict__ zh,
+                                            const float* __restrict__ xl,
+                                            const float* __restrict__ yl,
+                                            const float* __restrict__ zl,
+                                            const float* __restrict__ m, const SymGeom& g, int A,
+                                            int C, int dd, float mref, float eps2,
+                                            const float (&ih)[IPL][3], const float (&il)[IPL][3],
+                                            const float (&mi)[IPL], double (*red)[W][3][64]) {
+  constexpr int B = 64 * IPL;
+  constexpr int S = W * B;
+  constexpr int kChunks = S / 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int jbase = C * S;
+  const int src_addr = ((lane + 63) & 63) * 4;
+  double sx[IPL][3];
+#pragma unroll
+  for (int q = 0; q < IPL; ++q)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sx[q][a] = 0.0;
+  float nb[7];
+  load_ds_body(xh, yh, zh, xl, yl, zl, m, jbase + lane, g.n, nb);
+  for (int ch = 0; ch < kChunks; ++ch) {
+    float b[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) b[k] = nb[k];
+    if (ch + 1 < kChunks) load_ds_body(xh, yh, zh, xl, yl, zl, m, jbase + (ch + 1) * 64 + lane, g.n, nb);
+    float s[IPL][3], c[IPL][3], gs[3] = {0.f, 0.f, 0.f}, gc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < IPL; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) s[q][a] = c[q][a] = 0.0f;
+    int self_q = -1;
+    if (CHECKED) {
+      const int q = ch - wave * IPL;
+      self_q = (q >= 0 && q < IPL) ? q : -1;
+    }
+    ds_sym_chunk<IPL, UNI, JSIDE, CHECKED>(src_addr, b, ih, il, mi, eps2, self_q, s, c, gs, gc);
+#pragma unroll
+    for (int q = 0; q < IPL; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double v = (double)s[q][a] - (double)c[q][a];
+        sx[q][a] += UNI ? v * (double)mref : v;
+      }
+    if (JSIDE) {
+      double(*buf)[3][64] = red[ch & 1];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) buf[wave][a][lane] = (double)gs[a] - (double)gc[a];
+      __syncthreads();
+      for (int r = threadIdx.x; r < 192; r += 64 * W) {
+        const int k = r >> 6, l = r & 63;
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) t += buf[w][k][l];
+        if (UNI) t *= (double)mref;
+        g.pj[((long)(dd - 1) * 3 + k) * g.npad + jbase + ch * 64 + l] = (float)t;
+      }
+    }
+  }
+  const int ibase = A * S + wave * B;
+#pragma unroll
+  for (int q = 0; q < IPL; ++q)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      g.pi[((long)dd * 3 + a) * g.npad + ibase + q * 64 + lane] = (float)sx[q][a];
+}
+
+template <int IPL, int W, bool CHECKED>
+__global__ __launch_bounds__(64 * W) void ds_sym_kernel(
+    const float* __restrict__ xh, const float* __restrict__ yh, const float* __restrict__ zh,
+    const float* __restrict__ xl, const float* __restrict__ yl, const float* __restrict__ zl,
+    const float* __restrict__ m, SymGeom g, float eps2) {
+  constexpr int B = 64 * IPL;
+  constexpr int S = W * B;
+  __shared__ double red[2][W][3][64];
+  int A, dd;
+  if (!sym_tile_of(g, (int)blockIdx.x, A, dd)) return;  // whole workgroup: the other row's tile
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int C = (A + dd) % g.K;
+  float ih[IPL][3], il[IPL][3], mi[IPL];
+#pragma unroll
+  for (int q = 0; q < IPL; ++q) {
+    float b[7];
+    load_ds_body(xh, yh, zh, xl, yl, zl, m, A * S + wave * B + q * 64 + lane, g.n, b);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      ih[q][a] = b[a];
+      il[q][a] = b[3 + a];
+    }
+    mi[q] = b[6];
+  }
+  const float mref = m[A * S];
+  bool same = true;
+  for (int k = threadIdx.x; k < S; k += 64 * W) {
+    const int ia = A * S + k, jc = C * S + k;
+    if (ia < g.n) same = same && (m[ia] == mref);
+    if (jc < g.n) same = same && (m[jc] == mref);
+  }
+  const bool uni = __syncthreads_and(same);
+#define NBX_DS_TILE(U, J, CK) \
+  ds_sym_tile<IPL, W, U, J, CK>(xh, yh, zh, xl, yl, zl, m, g, A, C, dd, mref, eps2, ih, il, mi, red)
+  if (uni) {
+    if (dd == 0) NBX_DS_TILE(true, false, CHECKED);
+    else NBX_DS_TILE(true, true, false);
+  } else {
+    if (dd == 0) NBX_DS_TILE(false, false, CHECKED);
+    else NBX_DS_TILE(false, true, false);
+  }
+#undef NBX_DS_TILE
+}
+
+template <bool KICK>
+__global__ __launch_bounds__(256) void ds_sym_reduce(SymGeom g, double G, float* ax, float* ay,
+                                                     float* az, KickArgs kick) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= g.n) return;
+  sym_reduce_body<KICK>(g, b, G, ax, ay, az, kick);
+}
+
 // acc_old <- acc; (hi, lo) += v dt + a dt^2/2 evaluated and renormalised in FP64.
 __global__ void ds_drift_kernel(float* __restrict__ xh, float* __restrict__ yh,
                                 float* __restrict__ zh, float* __restrict__ xl,
@@ -157,7 +372,52 @@ __global__ void ds_drift_kernel(float* __restrict__ xh, float* __restrict__ yh,
   }
 }
 
-constexpr int kIpl = 2, kWs = 8;
+constexpr int kIpl = 2, kWs = 8;           // one-sided shape
+constexpr int kSymIpl = 4, kSymW = 4;      // pair-symmetric shape, super-blocks of 1,024
+constexpr int kSymS = 64 * kSymIpl * kSymW;
+
+// -1: symmetric from 2^15 bodies (partials permitting), 0: one-sided, 1: symmetric.
+int g_ds_kernel = -1;
+
+std::mutex g_ds_mu;
+std::map<std::pair<int, hipStream_t>, Scratch> g_ds_scratch;
+
+int ds_sym_launch(nbx_ds_particles* d, float G, float eps2, const KickArgs* kick,
+                  hipStream_t s) {
+  nbx_particles& p = d->p;
+  const int n = (int)p.count;
+  float* part = nullptr;
+  {
+    int dev = 0;
+    NBX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_ds_mu);
+    Scratch& sc = g_ds_scratch[std::make_pair(dev, s)];
+    NBX_TRY(sc.ensure(sym_partial_floats((size_t)n, kSymS) * sizeof(float)));
+    part = (float*)sc.ptr;
+  }
+  const SymGeom g = sym_geom(n, kSymS, part);
+  const unsigned tiles = (unsigned)sym_tiles(g);
+  if (eps2 < kUncheckedMinEps2)
+    hipLaunchKernelGGL((ds_sym_kernel<kSymIpl, kSymW, true>), dim3(tiles), dim3(64 * kSymW), 0,
+                       s, p.pos_x, p.pos_y, p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z,
+                       p.mass, g, eps2);
+  else
+    hipLaunchKernelGGL((ds_sym_kernel<kSymIpl, kSymW, false>), dim3(tiles), dim3(64 * kSymW), 0,
+                       s, p.pos_x, p.pos_y, p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z,
+                       p.mass, g, eps2);
+  NBX_LAUNCHED();
+  KickArgs k{};
+  if (kick) k = *kick;
+  const unsigned rb = div_up((size_t)n, 256);
+  if (kick)
+    hipLaunchKernelGGL(ds_sym_reduce<true>, dim3(rb), dim3(256), 0, s, g, (double)G, p.acc_x,
+                       p.acc_y, p.acc_z, k);
+  else
+    hipLaunchKernelGGL(ds_sym_reduce<false>, dim3(rb), dim3(256), 0, s, g, (double)G, p.acc_x,
+                       p.acc_y, p.acc_z, k);
+  NBX_LAUNCHED();
+  return NBX_OK;
+}
 
 int ds_forces_launch(nbx_ds_particles* d, float G, float eps2, const KickArgs* kick,
                      hipStream_t s) {
@@ -165,6 +425,9 @@ int ds_forces_launch(nbx_ds_particles* d, float G, float eps2, const KickArgs* k
   if (!(eps2 >= 0.0f) || !(eps2 < 3.0e38f))
     return fail(NBX_ERR_INVALID, "softening eps2 must be finite and >= 0");
   const int n = (int)p.count;
+  const bool fits = sym_partial_floats((size_t)n, kSymS) * sizeof(float) <= kSymMaxPartialBytes;
+  const bool sym = g_ds_kernel == 1 ? fits : (g_ds_kernel == -1 && n >= (1 << 15) && fits);
+  if (sym) return ds_sym_launch(d, G, eps2, kick, s);
   dim3 grid(div_up(n, 64 * kIpl)), block(64 * kWs);
   KickArgs k{};
   if (kick) k = *kick;
@@ -243,6 +506,12 @@ int nbx_ds_free(nbx_ds_particles* d) {
 int nbx_ds_forces(nbx_ds_particles* d, float G, float eps2, nbx_stream s) {
   NBX_TRY(check_ds(d, "nbx_ds_forces"));
   return ds_forces_launch(d, G, eps2, nullptr, as_stream(s));
+}
+
+int nbx_ds_set_kernel(int kind) {
+  if (kind < -1 || kind > 1) return fail(NBX_ERR_INVALID, "nbx_ds_set_kernel: kind %d", kind);
+  g_ds_kernel = kind;
+  return NBX_OK;
 }
 
 int nbx_ds_drift(nbx_ds_particles* d, float dt, nbx_stream s) {

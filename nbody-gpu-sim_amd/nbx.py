@@ -112,6 +112,7 @@ _sig("nbx_bh_phase_ms", C.c_int, VP, F64P)
 _sig("nbx_ds_alloc", C.c_int, DSP, C.c_size_t)
 _sig("nbx_ds_free", C.c_int, DSP)
 _sig("nbx_ds_forces", C.c_int, DSP, C.c_float, C.c_float, VP)
+_sig("nbx_ds_set_kernel", C.c_int, C.c_int)
 _sig("nbx_ds_drift", C.c_int, DSP, C.c_float, VP)
 _sig("nbx_ds_step", C.c_int, DSP, C.c_float, C.c_float, C.c_float, VP)
 _sig("nbx_ds_energy", C.c_int, DSP, C.c_float, C.c_float, F64P, F64P, F64P, VP)
@@ -147,7 +148,8 @@ EXPORTED = [
     "nbx_verlet_drift", "nbx_energy", "nbx_bh_create", "nbx_bh_destroy", "nbx_bh_build",
     "nbx_bh_forces", "nbx_bh_node_count", "nbx_bh_copy_nodes", "nbx_bh_copy_sort",
     "nbx_bh_build_info", "nbx_bh_set_timing", "nbx_bh_set_builder", "nbx_bh_phase_ms",
-    "nbx_ds_alloc", "nbx_ds_free", "nbx_ds_forces", "nbx_ds_drift", "nbx_ds_step",
+    "nbx_ds_alloc", "nbx_ds_free", "nbx_ds_forces", "nbx_ds_set_kernel", "nbx_ds_drift",
+    "nbx_ds_step",
     "nbx_ds_energy", "nbx_ens_create", "nbx_ens_destroy", "nbx_ens_particles", "nbx_ens_forces",
     "nbx_ens_steps", "nbx_ens_steps_sequential", "nbx_ens_variant_count", "nbx_ens_autotune",
     "nbx_ens_set_variant", "nbx_ens_energy", "nbx_gen_plummer", "nbx_gen_uniform_sphere",

@@ -17,7 +17,15 @@ def eps2(orc, eps):
     return orc.eps2_of(eps)
 
 
-def test_ds_forces_match_fp64(nbx, orc):
+@pytest.fixture(params=[0, 1], ids=["one_sided", "pair_symmetric"])
+def ds_kernel(nbx, request):
+    """Runs a test with each double-single force kernel forced."""
+    nbx.check(nbx.lib.nbx_ds_set_kernel(request.param))
+    yield request.param
+    nbx.check(nbx.lib.nbx_ds_set_kernel(-1))
+
+
+def test_ds_forces_match_fp64(nbx, orc, ds_kernel):
     h = nbx.gen_plummer(4096, seed=42)
     rng = np.random.default_rng(0)
     x64 = [h[k].astype(np.float64) + rng.uniform(-1, 1, 4096) * 2.0 ** -26 * np.abs(h[k])
@@ -35,8 +43,9 @@ def test_ds_forces_match_fp64(nbx, orc):
     assert np.median(e) <= 3e-7 and np.quantile(e, 0.99) <= 1e-6
 
 
-def test_ds_forces_ragged_and_zero_softening(nbx, orc):
-    for n, eps in ((1, 0.01), (65, 0.01), (1000, 0.0)):
+def test_ds_forces_ragged_and_zero_softening(nbx, orc, ds_kernel):
+    # 1000 and 2500 bodies give 1 and 3 super-blocks of 1,024 in the symmetric kernel
+    for n, eps in ((1, 0.01), (65, 0.01), (1000, 0.0), (2500, 0.01), (2500, 0.0)):
         h = nbx.gen_uniform_sphere(n, seed=n)
         d = nbx.DSDeviceParticles.from_host(h)
         nbx.check(nbx.lib.nbx_ds_forces(d.ptr, 1.0, eps2(orc, eps), None))
@@ -48,6 +57,40 @@ def test_ds_forces_ragged_and_zero_softening(nbx, orc):
             continue
         e = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
         assert np.quantile(e, 0.99) <= 1e-5, (n, eps)
+
+
+def test_ds_symmetric_mixed_masses_and_kick(nbx, orc):
+    # per-pair masses on both sides; then one step, whose kick is
+    # v += (a_prev + a_new) (0.5 dt) in IEEE single rounding
+    n = 5000
+    h = nbx.gen_plummer(n, seed=8)
+    h["mass"][:] = np.random.default_rng(4).uniform(0.2, 1.8, n).astype(np.float32) / np.float32(n)
+    rng = np.random.default_rng(5)
+    x64 = [h[k].astype(np.float64) + rng.uniform(-1, 1, n) * 2.0 ** -26 * np.abs(h[k])
+           for k in ("pos_x", "pos_y", "pos_z")]
+    d = nbx.DSDeviceParticles.from_host(h, pos64=x64)
+    e2 = eps2(orc, 0.01)
+    dt = np.float32(1e-3)
+    nbx.check(nbx.lib.nbx_ds_set_kernel(1))
+    try:
+        nbx.check(nbx.lib.nbx_ds_forces(d.ptr, 1.0, e2, None))
+        nbx.synchronize()
+        got = d.acc().astype(np.float64)
+        pos = d.positions64()
+        ref = orc.direct_f64_pos64(pos[0], pos[1], pos[2], h["mass"], 1.0, e2)
+        e = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        assert np.median(e) <= 3e-7 and np.quantile(e, 0.99) <= 1e-6
+        a_prev = [d.read(f) for f in ("acc_x", "acc_y", "acc_z")]
+        v_prev = [d.read(f) for f in ("vel_x", "vel_y", "vel_z")]
+        nbx.check(nbx.lib.nbx_ds_step(d.ptr, 1.0, e2, float(dt), None))
+        nbx.synchronize()
+    finally:
+        nbx.check(nbx.lib.nbx_ds_set_kernel(-1))
+    half = np.float32(np.float32(0.5) * dt)
+    for k, c in enumerate("xyz"):
+        a_new = d.read(f"acc_{c}")
+        want = (v_prev[k] + ((a_prev[k] + a_new).astype(np.float32) * half).astype(np.float32))
+        assert np.array_equal(d.read(f"vel_{c}"), want.astype(np.float32)), c
 
 
 def test_ds_drift_bit_exact(nbx):
