@@ -76,6 +76,29 @@ def test_direct_step_tracks_oracle_run(nbx, orc):
         assert np.max(np.abs(got[k] - ref[k])) < 1e-5, k
 
 
+def test_plummer_4096_100_steps_match_cpu_path(nbx, orc):
+    # the reference's CPU-runnable case: equal-mass (1/N) Plummer sphere, eps 0.01,
+    # dt 1e-3, 100 Velocity Verlet steps, against the oracle's restatement of the
+    # reference CPU path (FP32 sequential j sums)
+    h = nbx.gen_plummer(4096, seed=42)
+    G, eps, dt, steps = 1.0, 0.01, 1e-3, 100
+    d = nbx.DeviceParticles.from_host(h)
+    nbx.direct_forces(d, G, eps)
+    for _ in range(steps):
+        nbx.direct_step(d, G, eps, dt)
+    nbx.synchronize()
+    ref = {k: v.copy() for k, v in h.items()}
+    orc.run_direct(ref, G, eps, dt, steps)
+    got = d.download()
+    for k in ("pos_x", "pos_y", "pos_z"):
+        assert np.max(np.abs(got[k] - ref[k])) < 1e-5, k
+    for k in ("vel_x", "vel_y", "vel_z"):
+        assert np.max(np.abs(got[k] - ref[k])) < 1e-4, k
+    e2 = orc.eps2_of(eps)
+    eg, er = orc.energy_f64(got, G, e2), orc.energy_f64(ref, G, e2)
+    assert abs((eg["ke"] + eg["pe"]) - (er["ke"] + er["pe"])) <= 1e-6 * abs(er["ke"] + er["pe"])
+
+
 def test_energy_matches_oracle(nbx, orc):
     h = nbx.gen_plummer(3000, seed=8)
     d = nbx.DeviceParticles.from_host(h)
