@@ -165,6 +165,41 @@ int nbx_bh_set_builder(nbx_bh* t, int mode);
 /* phases: bbox, morton, sort, paths, tree, com, export, forces (8 values, ms). */
 int nbx_bh_phase_ms(nbx_bh* t, double ms[8]);
 
+/* ------------------------------------------------------------ spatial hash */
+/* SpatialHashGrid (include/nbody/spatial_hash_grid.hpp:9-62,
+ * src/cuda/force_spatial_hash.cu:160-368): uniform cell grid for short-range
+ * forces. Cell numbering cx + cy*dims[0] + cz*dims[0]*dims[1]. */
+typedef struct nbx_grid nbx_grid;
+
+/* SpatialHashGrid(size_t max_particles, float cell_size) (:161-174). */
+int nbx_grid_create(size_t max_particles, float cell_size, nbx_grid** out);
+int nbx_grid_destroy(nbx_grid* g);
+/* SpatialHashGrid::build (:241-309): bbox padded by 0.001f, dims
+ * (int)ceilf((max-min)/cell)+1, cell of every particle, particles grouped by
+ * cell (ascending index within a cell), cell_start/cell_end of every cell.
+ * Reads the bbox back (24 bytes) to size the grid, so it waits for the stream.
+ * NBX_ERR_CAPACITY "Spatial hash grid too large: ..." above 1e8 cells (:257-260)
+ * or when count exceeds max_particles. */
+int nbx_grid_build(nbx_grid* g, const nbx_particles* d, nbx_stream s);
+/* SpatialHashGrid::computeForces(ParticleData*, cutoff, G, eps) (:311-322): acc of
+ * every particle from the j != i in its 27 neighbour cells with r^2 < cutoff^2.
+ * Needs a build on the same particle count (NBX_ERR_STATE otherwise). */
+int nbx_grid_forces(nbx_grid* g, nbx_particles* d, float cutoff, float G, float eps,
+                    nbx_stream s);
+/* getGridDims / getCellSize / getTotalCells (spatial_hash_grid.hpp:27-29), plus the
+ * padded bbox {min x,y,z, max x,y,z} of the last build. Any pointer may be NULL. */
+int nbx_grid_info(nbx_grid* g, int32_t dims[3], float* cell_size, int32_t* total_cells,
+                  float bbox[6]);
+/* copyCellDataToHost (:324-340): cell_start/cell_end hold total_cells entries,
+ * particle_cells/sorted_indices max_particles entries (entries past the built
+ * count are -1). Synchronous; any pointer may be NULL. */
+int nbx_grid_copy_cells(nbx_grid* g, int32_t* cell_start, int32_t* cell_end,
+                        int32_t* particle_cells, int32_t* sorted_indices);
+/* Force walk for cross-checks (no reference counterpart): 0 = automatic (shared
+ * scalar walk where a wave's targets lie in one row of cells), 1 = per-lane walks
+ * only. Results are identical either way. */
+int nbx_grid_set_walk(nbx_grid* g, int mode);
+
 /* ------------------------------------------------------------ precision form */
 /* Double-single positions (hi + lo FP32 pairs), Kahan-compensated FP32 pair-force
  * sums, FP64 energies. No reference counterpart (extension of the direct path);

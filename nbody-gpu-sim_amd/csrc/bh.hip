@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "bbox.hpp"
 #include "radix.hpp"
 
 namespace nbx {
@@ -136,69 +137,15 @@ __global__ __launch_bounds__(kThreads) void bbox_partial(const float* __restrict
                                                          const float* __restrict__ y,
                                                          const float* __restrict__ z, int n,
                                                          float* partial) {
-  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};  // :18-19
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-    const float v[3] = {x[i], y[i], z[i]};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = fminf(lo[a], v[a]);
-      hi[a] = fmaxf(hi[a], v[a]);
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    for (int off = 32; off > 0; off >>= 1) {
-      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
-      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
-    }
-  }
-  __shared__ float sh[6][kThreads / 64];
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-    for (int a = 0; a < 3; ++a) {
-      sh[a][w] = lo[a];
-      sh[3 + a][w] = hi[a];
-    }
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    float v = sh[threadIdx.x][0];
-    for (int q = 1; q < kThreads / 64; ++q)
-      v = threadIdx.x < 3 ? fminf(v, sh[threadIdx.x][q]) : fmaxf(v, sh[threadIdx.x][q]);
-    partial[blockIdx.x * 6 + threadIdx.x] = v;
-  }
+  bbox_partial_body<kThreads>(x, y, z, n, partial);
 }
 
 __global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int nparts, int n,
                                                        BhState* st) {
-  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-  for (int b = threadIdx.x; b < nparts; b += kThreads)
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = fminf(lo[a], partial[b * 6 + a]);
-      hi[a] = fmaxf(hi[a], partial[b * 6 + 3 + a]);
-    }
-  for (int a = 0; a < 3; ++a)
-    for (int off = 32; off > 0; off >>= 1) {
-      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
-      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
-    }
-  __shared__ float sh[6][kThreads / 64];
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-    for (int a = 0; a < 3; ++a) {
-      sh[a][w] = lo[a];
-      sh[3 + a][w] = hi[a];
-    }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int a = 0; a < 3; ++a)
-    for (int q = 1; q < kThreads / 64; ++q) {
-      sh[a][0] = fminf(sh[a][0], sh[a][q]);
-      sh[3 + a][0] = fmaxf(sh[3 + a][0], sh[3 + a][q]);
-    }
+  float lo[3], hi[3];
+  if (!bbox_final_body<kThreads>(partial, nparts, lo, hi)) return;
   float ext[3];
   for (int a = 0; a < 3; ++a) {
-    lo[a] = sh[a][0];
-    hi[a] = sh[3 + a][0];
     st->bmin[a] = lo[a];
     st->bmax[a] = hi[a];
     const float size = add_rn(sub_rn(hi[a], lo[a]), 0.001f);  // :313-315

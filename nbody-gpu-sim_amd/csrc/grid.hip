@@ -1,0 +1,568 @@
+// Spatial-hash (uniform cell grid) short-range forces: the reference's
+// SpatialHashGrid and SpatialHashCalculator (src/cuda/force_spatial_hash.cu,
+// include/nbody/spatial_hash_grid.hpp), on the GPU.
+//
+// Build (SpatialHashGrid::build, :241-309):
+//   bbox    exact per-axis min/max (the Barnes-Hut reduction, which the
+//           reference reuses, :184-188), read back once (24 bytes). The host
+//           pads it by 0.001f (:232-238), sets dims = (int)ceilf((max-min)/cell)+1
+//           in FP32 (:250-252) and refuses grids above 1e8 cells (:254-260).
+//   assign  cell of each particle: (int)floorf((p - min) / cell) per axis,
+//           clamped to the grid, linear cx + cy*dx + cz*dx*dy (:36-49).
+//   sort    stable radix sort of (cell, particle index), so sorted_indices
+//           ascend within each cell. The reference's atomic scatter (:72-81)
+//           leaves that order to the scheduler; ascending is one of its
+//           outcomes and makes every later sum deterministic.
+//   ranges  cell_start / cell_end of every cell, the exclusive scan of the
+//           per-cell counts (:287-300), from one merge of the sorted keys with
+//           the cell numbers: each thread takes a fixed slice of the merged
+//           sequence, so a sparse grid (many empty cells) and a dense one cost
+//           the same per element, and no count array or memset is needed.
+//
+// Forces (spatialHashForceKernel, :84-158). For target i in cell c, each j != i
+// in the 27 cells around c (clamped at the grid edge) with
+// (dx*dx + dy*dy) + dz*dz < cutoff^2 adds G m_j r_ij (r^2 + eps^2)^{-3/2};
+// cells in dz, dy, dx order, bodies in sorted order. The three dx cells of one
+// (dy, dz) row are adjacent in the linear numbering, so they are one
+// contiguous range of the sorted bodies.
+//   * bodies are gathered once per call into sorted order as float4
+//     (x, y, z, G*m); G*m is the reference's first product (:144).
+//   * a wave takes 64 consecutive sorted targets. When they all lie in one
+//     (cy, cz) row of cells (the common case once cells hold dozens of
+//     bodies), the wave walks the union of the lanes' ranges with
+//     wave-uniform addresses: the bodies arrive through scalar loads and
+//     reach every lane as SGPR operands, and a lane drops the bodies outside
+//     its own three cells with a mask. Otherwise every lane walks its own
+//     ranges with vector loads.
+//   * the cutoff test is evaluated as written, without FMA contraction, so
+//     the set of pairs is exactly the one the CPU restatement keeps; the kept
+//     pairs use v_rsq_f32 and FMA accumulation (tolerance-level differences).
+//   * both walks visit a target's bodies in the same order with the same
+//     arithmetic, so a target's result does not depend on which one ran.
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "bbox.hpp"
+#include "common.hpp"
+#include "radix.hpp"
+
+namespace nbx {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMergeSpan = 32;             // merged elements per thread in grid_ranges
+constexpr long long kMaxCells = 100000000;  // :257-260
+constexpr int kNB = 8;                      // bodies per scalar batch of the uniform walk
+
+struct GridGeom {
+  float lo[3];  // padded lower corner
+  float cell;
+  int dims[3];
+  int total;
+};
+
+// :36-49 and :103-108 (the force kernel repeats the arithmetic and the clamp).
+__device__ __forceinline__ int grid_axis(float p, float lo, float cell, int dim) {
+  const int c = (int)floorf(div_rn(sub_rn(p, lo), cell));
+  return max(0, min(c, dim - 1));
+}
+
+__global__ __launch_bounds__(kThreads) void grid_bbox_partial(const float* __restrict__ x,
+                                                              const float* __restrict__ y,
+                                                              const float* __restrict__ z, int n,
+                                                              float* partial) {
+  bbox_partial_body<kThreads>(x, y, z, n, partial);
+}
+
+__global__ __launch_bounds__(kThreads) void grid_bbox_final(const float* partial, int nparts,
+                                                            float* box) {
+  float lo[3], hi[3];
+  if (!bbox_final_body<kThreads>(partial, nparts, lo, hi)) return;
+  for (int a = 0; a < 3; ++a) {
+    box[a] = lo[a];
+    box[3 + a] = hi[a];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void grid_assign(const float* __restrict__ x,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ z, int n,
+                                                        GridGeom g, int32_t* __restrict__ pcell,
+                                                        uint32_t* __restrict__ keys,
+                                                        int32_t* __restrict__ idx,
+                                                        int32_t* __restrict__ n_dev) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i == 0) *n_dev = n;
+  if (i >= n) return;
+  const int cx = grid_axis(x[i], g.lo[0], g.cell, g.dims[0]);
+  const int cy = grid_axis(y[i], g.lo[1], g.cell, g.dims[1]);
+  const int cz = grid_axis(z[i], g.lo[2], g.cell, g.dims[2]);
+  const int c = cx + cy * g.dims[0] + cz * g.dims[0] * g.dims[1];
+  pcell[i] = c;
+  keys[i] = (uint32_t)c;
+  idx[i] = i;
+}
+
+// Merge of the cell numbers 0..total (total is a sentinel) with the n sorted
+// keys, cells placed before keys of equal value. Cell c then sits after
+// exactly start[c] = #{keys < c} keys, and key k sits at merged position
+// k + keys[k] + 1 (strictly increasing in k). A thread finds its first key by
+// binary search on that position, then walks kMergeSpan merged elements,
+// writing start[c] and end[c-1] (= start[c]) for every cell it passes.
+__global__ __launch_bounds__(kThreads) void grid_ranges(const uint32_t* __restrict__ keys, int n,
+                                                        int total, int32_t* __restrict__ start,
+                                                        int32_t* __restrict__ end) {
+  const long long m = (long long)total + 1 + n;
+  const long long q0 = ((long long)blockIdx.x * kThreads + threadIdx.x) * kMergeSpan;
+  if (q0 >= m) return;
+  int lo = 0, hi = n;  // first key whose merged position is >= q0
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((long long)mid + keys[mid] + 1 < q0)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  int j = lo;
+  long long c = q0 - j;  // cells before q0
+  const long long q1 = q0 + kMergeSpan < m ? q0 + kMergeSpan : m;
+  for (long long q = q0; q < q1; ++q) {
+    if (j < n && (long long)j + keys[j] + 1 == q) {
+      ++j;
+      continue;
+    }
+    if (c < total) start[c] = j;
+    if (c >= 1) end[c - 1] = j;
+    ++c;
+  }
+}
+
+// Bodies in sorted order with G*m (:144 forms G * mass[j] first).
+__global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restrict__ idx,
+                                                        const float* __restrict__ x,
+                                                        const float* __restrict__ y,
+                                                        const float* __restrict__ z,
+                                                        const float* __restrict__ m, int n,
+                                                        float G, float4* __restrict__ body) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= n) return;
+  const int i = idx[k];
+  body[k] = make_float4(x[i], y[i], z[i], mul_rn(G, m[i]));
+}
+
+// One pair of :133-149: r = x_j - x_i, d2 = (rx*rx + ry*ry) + rz*rz rounded as
+// written, kept when d2 < cutoff2 (and the caller's mask), then
+// f = (G m_j) * ((inv*inv)*inv) with inv = rsq(d2 + eps2).
+__device__ __forceinline__ float grid_d2(float rx, float ry, float rz) {
+  return add_rn(add_rn(mul_rn(rx, rx), mul_rn(ry, ry)), mul_rn(rz, rz));
+}
+
+// Walk of [ulo, uhi) with wave-uniform indices (scalar loads, SGPR operands).
+// MEMBER: lanes keep only bodies of their own range [own_lo, own_hi).
+template <bool MEMBER>
+__device__ __forceinline__ void walk_uniform(const float4* __restrict__ body, int ulo, int uhi,
+                                             int own_lo, int own_hi, int self, float xi,
+                                             float yi, float zi, float cutoff2, float eps2,
+                                             float& fx, float& fy, float& fz) {
+  int j = ulo;
+  for (; j + kNB <= uhi; j += kNB) {
+    float4 b[kNB];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) b[q] = body[j + q];
+    float rx[kNB], ry[kNB], rz[kNB], w[kNB];
+    bool keep[kNB];
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) {
+      rx[q] = sub_rn(b[q].x, xi);
+      ry[q] = sub_rn(b[q].y, yi);
+      rz[q] = sub_rn(b[q].z, zi);
+    }
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) {
+      const float d2 = grid_d2(rx[q], ry[q], rz[q]);
+      const int jj = j + q;
+      keep[q] = (d2 < cutoff2) & (jj != self) & (!MEMBER | ((jj >= own_lo) & (jj < own_hi)));
+      w[q] = add_rn(d2, eps2);
+    }
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) w[q] = __builtin_amdgcn_rsqf(w[q]);
+#pragma unroll
+    for (int q = 0; q < kNB; ++q) {
+      const float inv = w[q];
+      const float f = keep[q] ? mul_rn(b[q].w, mul_rn(mul_rn(inv, inv), inv)) : 0.0f;
+      fx = __builtin_fmaf(f, rx[q], fx);
+      fy = __builtin_fmaf(f, ry[q], fy);
+      fz = __builtin_fmaf(f, rz[q], fz);
+    }
+  }
+  for (; j < uhi; ++j) {
+    const float4 b = body[j];
+    const float rx = sub_rn(b.x, xi), ry = sub_rn(b.y, yi), rz = sub_rn(b.z, zi);
+    const float d2 = grid_d2(rx, ry, rz);
+    const bool keep = (d2 < cutoff2) & (j != self) & (!MEMBER | ((j >= own_lo) & (j < own_hi)));
+    const float inv = __builtin_amdgcn_rsqf(add_rn(d2, eps2));
+    const float f = keep ? mul_rn(b.w, mul_rn(mul_rn(inv, inv), inv)) : 0.0f;
+    fx = __builtin_fmaf(f, rx, fx);
+    fy = __builtin_fmaf(f, ry, fy);
+    fz = __builtin_fmaf(f, rz, fz);
+  }
+}
+
+// Walk of a lane's own [lo, hi) with per-lane (vector) loads.
+__device__ __forceinline__ void walk_lane(const float4* __restrict__ body, int lo, int hi,
+                                          int self, float xi, float yi, float zi, float cutoff2,
+                                          float eps2, float& fx, float& fy, float& fz) {
+  for (int j = lo; j < hi; ++j) {
+    const float4 b = body[j];
+    const float rx = sub_rn(b.x, xi), ry = sub_rn(b.y, yi), rz = sub_rn(b.z, zi);
+    const float d2 = grid_d2(rx, ry, rz);
+    const bool keep = (d2 < cutoff2) & (j != self);
+    const float inv = __builtin_amdgcn_rsqf(add_rn(d2, eps2));
+    const float f = keep ? mul_rn(b.w, mul_rn(mul_rn(inv, inv), inv)) : 0.0f;
+    fx = __builtin_fmaf(f, rx, fx);
+    fy = __builtin_fmaf(f, ry, fy);
+    fz = __builtin_fmaf(f, rz, fz);
+  }
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each
+// XCD takes one contiguous run of sorted targets, whose neighbour cells then
+// stay in that XCD's L2.
+__device__ __forceinline__ int xcd_block(int i, int nb) {
+  const int per = nb / 8, rem = nb % 8;
+  const int xcd = i % 8, slot = i / 8;
+  return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+// mode 0: uniform walk where a wave's targets share a row; 1: per-lane walks only.
+__global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict__ body,
+                                                        const int32_t* __restrict__ idx,
+                                                        const int32_t* __restrict__ cstart,
+                                                        const int32_t* __restrict__ cend,
+                                                        GridGeom g, float cutoff2, float eps2,
+                                                        int n, int mode, float* __restrict__ ax,
+                                                        float* __restrict__ ay,
+                                                        float* __restrict__ az) {
+  const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int k0 = b * kThreads + (int)(threadIdx.x & ~63u);
+  if (k0 >= n) return;  // the whole wave is past the end
+  const int k = k0 + (int)(threadIdx.x & 63u);
+  const bool active = k < n;
+  const int kc = active ? k : n - 1;  // idle lanes mirror the wave's last target
+  const float4 me = body[kc];
+  const int cx = grid_axis(me.x, g.lo[0], g.cell, g.dims[0]);
+  const int cy = grid_axis(me.y, g.lo[1], g.cell, g.dims[1]);
+  const int cz = grid_axis(me.z, g.lo[2], g.cell, g.dims[2]);
+  const int lo_x = max(cx - 1, 0), hi_x = min(cx + 1, g.dims[0] - 1);
+  const int dxy = g.dims[0] * g.dims[1];
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  const int row = cy + cz * g.dims[1];
+  const bool one_row = mode == 0 && __all(row == __builtin_amdgcn_readfirstlane(row));
+  if (one_row) {
+    const int cy0 = __builtin_amdgcn_readfirstlane(cy), cz0 = __builtin_amdgcn_readfirstlane(cz);
+    const int xmin = wave_min(cx), xmax = wave_max(cx);
+    const int ulo_x = max(xmin - 1, 0), uhi_x = min(xmax + 1, g.dims[0] - 1);
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int nz = cz0 + dz;
+      if (nz < 0 || nz >= g.dims[2]) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int ny = cy0 + dy;
+        if (ny < 0 || ny >= g.dims[1]) continue;
+        const int base = ny * g.dims[0] + nz * dxy;
+        const int ulo = cstart[base + ulo_x], uhi = cend[base + uhi_x];
+        if (xmin == xmax) {
+          walk_uniform<false>(body, ulo, uhi, 0, 0, k, me.x, me.y, me.z, cutoff2, eps2, fx, fy,
+                              fz);
+        } else {
+          const int olo = cstart[base + lo_x], ohi = cend[base + hi_x];
+          walk_uniform<true>(body, ulo, uhi, olo, ohi, k, me.x, me.y, me.z, cutoff2, eps2, fx,
+                             fy, fz);
+        }
+      }
+    }
+  } else {
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int nz = cz + dz;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int ny = cy + dy;
+        if (nz < 0 || nz >= g.dims[2] || ny < 0 || ny >= g.dims[1]) continue;
+        const int base = ny * g.dims[0] + nz * dxy;
+        walk_lane(body, cstart[base + lo_x], cend[base + hi_x], k, me.x, me.y, me.z, cutoff2,
+                  eps2, fx, fy, fz);
+      }
+    }
+  }
+  if (active) {
+    const int i = idx[k];
+    ax[i] = fx;
+    ay[i] = fy;
+    az[i] = fz;
+  }
+}
+
+int bits_for(int total) {  // key bits covering cell numbers 0 .. total-1
+  int b = 0;
+  while (b < 31 && (1LL << b) < (long long)total) ++b;
+  return b;
+}
+
+}  // namespace
+}  // namespace nbx
+
+using nbx::GridGeom;
+
+struct nbx_grid {
+  size_t max_particles = 0;
+  float cell = 0.0f;
+  int bbox_blocks = 0;
+  std::vector<void*> allocs;
+  float* bbox_part = nullptr;
+  float* box_dev = nullptr;
+  float* box_host = nullptr;  // pinned, 6 floats
+  int32_t* pcell = nullptr;
+  uint32_t* keys = nullptr;
+  int32_t* idx = nullptr;
+  int32_t* n_dev = nullptr;
+  float4* body = nullptr;
+  int32_t* cstart = nullptr;
+  int32_t* cend = nullptr;
+  size_t cell_cap = 0;
+  nbx::RadixSort sort;
+  GridGeom geom{};
+  float box[6] = {0, 0, 0, 0, 0, 0};  // padded bbox of the last build
+  int n = 0;                          // particles of the last build (0: not built)
+  int mode = 0;
+};
+
+namespace nbx {
+namespace {
+
+template <typename T>
+int galloc(nbx_grid* g, T** p, size_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(NBX_ERR_NOMEM, "spatial hash buffers: allocation of %zu bytes failed",
+                sizeof(T) * count);
+  }
+  g->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return NBX_OK;
+}
+
+void grid_release(nbx_grid* g) {
+  for (void* p : g->allocs) (void)hipFree(p);
+  g->allocs.clear();
+  if (g->cstart) (void)hipFree(g->cstart);
+  if (g->cend) (void)hipFree(g->cend);
+  g->cstart = g->cend = nullptr;
+  if (g->box_host) (void)hipHostFree(g->box_host);
+  g->box_host = nullptr;
+  g->sort.release();
+}
+
+int grid_blocks(size_t n, int cap) {
+  return (int)std::max<size_t>(1, std::min<size_t>((n + kThreads - 1) / kThreads, (size_t)cap));
+}
+
+}  // namespace
+}  // namespace nbx
+
+using namespace nbx;
+
+extern "C" {
+
+int nbx_grid_create(size_t max_particles, float cell_size, nbx_grid** out) {
+  if (!out) return fail(NBX_ERR_INVALID, "nbx_grid_create: null output");
+  *out = nullptr;
+  if (max_particles == 0) return fail(NBX_ERR_INVALID, "spatial hash grid needs max_particles > 0");
+  if (max_particles > (size_t)INT32_MAX - 1024)
+    return fail(NBX_ERR_INVALID, "spatial hash grid supports at most 2^31-1025 particles");
+  if (!(cell_size > 0.0f) || std::isinf(cell_size))
+    return fail(NBX_ERR_INVALID, "Spatial hash cell size must be positive and finite");
+  nbx_grid* g = new (std::nothrow) nbx_grid();
+  if (!g) return fail(NBX_ERR_NOMEM, "nbx_grid_create: host allocation");
+  const size_t P = max_particles;
+  g->max_particles = P;
+  g->cell = cell_size;
+  g->bbox_blocks = grid_blocks(P, 1024);
+  int rc = NBX_OK;
+  auto A = [&](auto** p, size_t c) {
+    if (rc == NBX_OK) rc = galloc(g, p, c);
+  };
+  A(&g->bbox_part, (size_t)g->bbox_blocks * 6);
+  A(&g->box_dev, 6);
+  A(&g->pcell, P);
+  A(&g->keys, P);
+  A(&g->idx, P);
+  A(&g->n_dev, 1);
+  A(&g->body, P);
+  if (rc == NBX_OK && hipHostMalloc((void**)&g->box_host, 6 * sizeof(float)) != hipSuccess) {
+    (void)hipGetLastError();
+    g->box_host = nullptr;
+    rc = fail(NBX_ERR_NOMEM, "spatial hash: pinned host allocation failed");
+  }
+  if (rc == NBX_OK) rc = g->sort.init((int)P);
+  if (rc != NBX_OK) {
+    grid_release(g);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return NBX_OK;
+}
+
+int nbx_grid_destroy(nbx_grid* g) {
+  if (!g) return NBX_OK;
+  grid_release(g);
+  delete g;
+  return NBX_OK;
+}
+
+int nbx_grid_build(nbx_grid* g, const nbx_particles* d, nbx_stream stream) {
+  if (!g) return fail(NBX_ERR_INVALID, "nbx_grid_build: null grid");
+  NBX_TRY(check_particles(d, "nbx_grid_build"));
+  if (d->count > g->max_particles)
+    return fail(NBX_ERR_CAPACITY, "spatial hash grid created for %zu particles, got %zu",
+                g->max_particles, d->count);
+  hipStream_t s = as_stream(stream);
+  const int n = (int)d->count;
+  g->n = 0;
+  // bbox (:190-239): device reduction, one 24-byte read-back
+  const int bb = std::min(g->bbox_blocks, grid_blocks(n, 1024));
+  hipLaunchKernelGGL(grid_bbox_partial, dim3(bb), dim3(kThreads), 0, s, d->pos_x, d->pos_y,
+                     d->pos_z, n, g->bbox_part);
+  hipLaunchKernelGGL(grid_bbox_final, dim3(1), dim3(kThreads), 0, s, g->bbox_part, bb,
+                     g->box_dev);
+  NBX_LAUNCHED();
+  NBX_HIP(hipMemcpyAsync(g->box_host, g->box_dev, 6 * sizeof(float), hipMemcpyDeviceToHost, s));
+  NBX_HIP(hipStreamSynchronize(s));
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = g->box_host[a] - 0.001f;  // :232-238, FP32 on the host
+    hi[a] = g->box_host[3 + a] + 0.001f;
+  }
+  // dims (:249-260)
+  GridGeom geom{};
+  long long total = 1;
+  bool too_large = false;
+  for (int a = 0; a < 3; ++a) {
+    const float r = (hi[a] - lo[a]) / g->cell;
+    if (!(r <= 1e8f)) {  // also catches non-finite positions
+      too_large = true;
+      break;
+    }
+    geom.dims[a] = (int)std::ceil(r) + 1;
+    total *= geom.dims[a];
+    if (total > kMaxCells) too_large = true;
+  }
+  if (too_large)
+    return fail(NBX_ERR_CAPACITY, "Spatial hash grid too large: reduce cell_size or bounding box");
+  for (int a = 0; a < 3; ++a) {
+    geom.lo[a] = lo[a];
+    g->box[a] = lo[a];
+    g->box[3 + a] = hi[a];
+  }
+  geom.cell = g->cell;
+  geom.total = (int)total;
+  if ((size_t)total > g->cell_cap) {  // grow-only cell arrays
+    if (g->cstart) (void)hipFree(g->cstart);
+    if (g->cend) (void)hipFree(g->cend);
+    g->cstart = g->cend = nullptr;
+    g->cell_cap = 0;
+    if (hipMalloc((void**)&g->cstart, sizeof(int32_t) * total) != hipSuccess ||
+        hipMalloc((void**)&g->cend, sizeof(int32_t) * total) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(NBX_ERR_NOMEM, "spatial hash: allocation of %lld cells failed", total);
+    }
+    g->cell_cap = (size_t)total;
+  }
+  g->geom = geom;
+  // assign (:281-285), stable sort by cell, ranges (:287-300)
+  hipLaunchKernelGGL(grid_assign, dim3(grid_blocks(n, INT32_MAX)), dim3(kThreads), 0, s, d->pos_x,
+                     d->pos_y, d->pos_z, n, geom, g->pcell, g->keys, g->idx, g->n_dev);
+  NBX_LAUNCHED();
+  NBX_TRY(g->sort.sort(g->keys, g->idx, g->n_dev, bits_for(geom.total), s));
+  const long long merged = total + 1 + n;
+  const long long threads = (merged + kMergeSpan - 1) / kMergeSpan;
+  hipLaunchKernelGGL(grid_ranges, dim3((unsigned)((threads + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, s, g->keys, n, geom.total, g->cstart, g->cend);
+  NBX_LAUNCHED();
+  g->n = n;
+  return NBX_OK;
+}
+
+int nbx_grid_forces(nbx_grid* g, nbx_particles* d, float cutoff, float G, float eps,
+                    nbx_stream stream) {
+  if (!g) return fail(NBX_ERR_INVALID, "nbx_grid_forces: null grid");
+  NBX_TRY(check_particles(d, "nbx_grid_forces"));
+  if (g->n == 0) return fail(NBX_ERR_STATE, "spatial hash forces before a successful build");
+  if ((int)d->count != g->n)
+    return fail(NBX_ERR_STATE, "spatial hash grid was built for %d particles, forces asked for %zu",
+                g->n, d->count);
+  hipStream_t s = as_stream(stream);
+  const int n = g->n;
+  const float cutoff2 = cutoff * cutoff;  // :320
+  const float eps2 = eps * eps;
+  hipLaunchKernelGGL(grid_gather, dim3(grid_blocks(n, INT32_MAX)), dim3(kThreads), 0, s, g->idx,
+                     d->pos_x, d->pos_y, d->pos_z, d->mass, n, G, g->body);
+  hipLaunchKernelGGL(grid_forces, dim3(grid_blocks(n, INT32_MAX)), dim3(kThreads), 0, s, g->body,
+                     g->idx, g->cstart, g->cend, g->geom, cutoff2, eps2, n, g->mode, d->acc_x,
+                     d->acc_y, d->acc_z);
+  NBX_LAUNCHED();
+  return NBX_OK;
+}
+
+int nbx_grid_info(nbx_grid* g, int32_t dims[3], float* cell_size, int32_t* total_cells,
+                  float bbox[6]) {
+  if (!g) return fail(NBX_ERR_INVALID, "nbx_grid_info: null grid");
+  const bool built = g->geom.total > 0;
+  for (int a = 0; a < 3; ++a)
+    if (dims) dims[a] = built ? g->geom.dims[a] : 0;
+  if (cell_size) *cell_size = g->cell;
+  if (total_cells) *total_cells = built ? g->geom.total : 0;
+  if (bbox)
+    for (int a = 0; a < 6; ++a) bbox[a] = g->box[a];
+  return NBX_OK;
+}
+
+int nbx_grid_copy_cells(nbx_grid* g, int32_t* cell_start, int32_t* cell_end,
+                        int32_t* particle_cells, int32_t* sorted_indices) {
+  if (!g) return fail(NBX_ERR_INVALID, "nbx_grid_copy_cells: null grid");
+  NBX_HIP(hipDeviceSynchronize());
+  const size_t total = g->geom.total > 0 ? (size_t)g->geom.total : 0;
+  const size_t n = (size_t)g->n, P = g->max_particles;
+  if (cell_start && total)
+    NBX_HIP(hipMemcpy(cell_start, g->cstart, total * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (cell_end && total)
+    NBX_HIP(hipMemcpy(cell_end, g->cend, total * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (particle_cells) {
+    if (n) NBX_HIP(hipMemcpy(particle_cells, g->pcell, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::fill(particle_cells + n, particle_cells + P, -1);
+  }
+  if (sorted_indices) {
+    if (n) NBX_HIP(hipMemcpy(sorted_indices, g->idx, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::fill(sorted_indices + n, sorted_indices + P, -1);
+  }
+  return NBX_OK;
+}
+
+int nbx_grid_set_walk(nbx_grid* g, int mode) {
+  if (!g) return fail(NBX_ERR_INVALID, "nbx_grid_set_walk: null grid");
+  if (mode != 0 && mode != 1) return fail(NBX_ERR_INVALID, "nbx_grid_set_walk: mode must be 0 or 1");
+  g->mode = mode;
+  return NBX_OK;
+}
+
+}  // extern "C"

@@ -109,6 +109,13 @@ _sig("nbx_bh_build_info", C.c_int, VP, I32P, I32P)
 _sig("nbx_bh_set_timing", C.c_int, VP, C.c_int)
 _sig("nbx_bh_set_builder", C.c_int, VP, C.c_int)
 _sig("nbx_bh_phase_ms", C.c_int, VP, F64P)
+_sig("nbx_grid_create", C.c_int, C.c_size_t, C.c_float, C.POINTER(VP))
+_sig("nbx_grid_destroy", C.c_int, VP)
+_sig("nbx_grid_build", C.c_int, VP, PP, VP)
+_sig("nbx_grid_forces", C.c_int, VP, PP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_grid_info", C.c_int, VP, I32P, F32P, I32P, F32P)
+_sig("nbx_grid_copy_cells", C.c_int, VP, I32P, I32P, I32P, I32P)
+_sig("nbx_grid_set_walk", C.c_int, VP, C.c_int)
 _sig("nbx_ds_alloc", C.c_int, DSP, C.c_size_t)
 _sig("nbx_ds_free", C.c_int, DSP)
 _sig("nbx_ds_forces", C.c_int, DSP, C.c_float, C.c_float, VP)
@@ -148,6 +155,8 @@ EXPORTED = [
     "nbx_verlet_drift", "nbx_energy", "nbx_bh_create", "nbx_bh_destroy", "nbx_bh_build",
     "nbx_bh_forces", "nbx_bh_node_count", "nbx_bh_copy_nodes", "nbx_bh_copy_sort",
     "nbx_bh_build_info", "nbx_bh_set_timing", "nbx_bh_set_builder", "nbx_bh_phase_ms",
+    "nbx_grid_create", "nbx_grid_destroy", "nbx_grid_build", "nbx_grid_forces", "nbx_grid_info",
+    "nbx_grid_copy_cells", "nbx_grid_set_walk",
     "nbx_ds_alloc", "nbx_ds_free", "nbx_ds_forces", "nbx_ds_set_kernel", "nbx_ds_drift",
     "nbx_ds_step",
     "nbx_ds_energy", "nbx_ens_create", "nbx_ens_destroy", "nbx_ens_particles", "nbx_ens_forces",
@@ -400,6 +409,48 @@ class BHTree:
     def __del__(self):
         try:
             lib.nbx_bh_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Grid:
+    """nbx_grid handle: SpatialHashGrid(max_particles, cell_size)."""
+
+    def __init__(self, max_particles: int, cell_size: float):
+        self.h = VP()
+        self.max_particles = max_particles
+        check(lib.nbx_grid_create(max_particles, cell_size, C.byref(self.h)))
+
+    def build(self, d: DeviceParticles, stream=None) -> None:
+        check(lib.nbx_grid_build(self.h, d.ptr, stream.h if stream is not None else None))
+
+    def forces(self, d: DeviceParticles, cutoff: float, G: float, eps: float, stream=None) -> None:
+        check(lib.nbx_grid_forces(self.h, d.ptr, cutoff, G, eps,
+                                  stream.h if stream is not None else None))
+
+    def info(self) -> dict:
+        dims, cell, total = (C.c_int32 * 3)(), C.c_float(), C.c_int32()
+        bbox = np.zeros(6, np.float32)
+        check(lib.nbx_grid_info(self.h, C.cast(dims, I32P), C.byref(cell), C.byref(total),
+                                fptr(bbox)))
+        return {"dims": np.array(list(dims), np.int32), "cell": np.float32(cell.value),
+                "total": total.value, "bbox": bbox}
+
+    def cells(self) -> dict:
+        """copyCellDataToHost: cell_start/cell_end (total) and particle_cells/sorted (max_particles)."""
+        total = self.info()["total"]
+        cs, ce = np.zeros(total, np.int32), np.zeros(total, np.int32)
+        pc, srt = np.zeros(self.max_particles, np.int32), np.zeros(self.max_particles, np.int32)
+        p = lambda a: a.ctypes.data_as(I32P)
+        check(lib.nbx_grid_copy_cells(self.h, p(cs), p(ce), p(pc), p(srt)))
+        return {"cell_start": cs, "cell_end": ce, "particle_cell": pc, "sorted": srt}
+
+    def set_walk(self, mode: int) -> None:
+        check(lib.nbx_grid_set_walk(self.h, mode))
+
+    def __del__(self):
+        try:
+            lib.nbx_grid_destroy(self.h)
         except Exception:
             pass
 

@@ -529,3 +529,221 @@ int orc_bh_full(size_t n, const float* x, const float* y, const float* z, const 
   orc_bh_forces(n, nodes, cnt, x, y, z, m, theta, G, eps, NULL, 0, ax, ay, az, nthreads);
   return 0;
 }
+
+/* ------------------------------------------------------------ spatial hash */
+
+/* force_spatial_hash.cu:15-18. */
+void orc_grid_cell_index(float x, float y, float z, float cell, int32_t out[3]) {
+  out[0] = (int32_t)floorf(x / cell);
+  out[1] = (int32_t)floorf(y / cell);
+  out[2] = (int32_t)floorf(z / cell);
+}
+
+/* force_spatial_hash.cu:20-26: wrap negative indices, then x fastest. */
+int32_t orc_grid_hash_cell(const int32_t c[3], const int32_t d[3]) {
+  const int32_t cx = ((c[0] % d[0]) + d[0]) % d[0];
+  const int32_t cy = ((c[1] % d[1]) + d[1]) % d[1];
+  const int32_t cz = ((c[2] % d[2]) + d[2]) % d[2];
+  return cx + cy * d[0] + cz * d[0] * d[1];
+}
+
+/* force_spatial_hash.cu:190-260: the Barnes-Hut bbox reduction (:185-188), the host
+ * padding (:232-238), dims (:250-252) and the 1e8-cell limit (:254-260). */
+int orc_grid_dims(size_t n, const float* x, const float* y, const float* z, float cell,
+                  float bbox[6], int32_t dims[3], int64_t* total) {
+  float lo[3], hi[3];
+  orc_bbox(n, x, y, z, lo, hi);
+  for (int a = 0; a < 3; ++a) {
+    lo[a] -= 0.001f;
+    hi[a] += 0.001f;
+    bbox[a] = lo[a];
+    bbox[3 + a] = hi[a];
+  }
+  int64_t t = 1;
+  int too_large = 0;
+  for (int a = 0; a < 3; ++a) {
+    const float r = (hi[a] - lo[a]) / cell;
+    if (!(r <= 1e8f)) { /* the int cast below would overflow; the product is > 1e8 anyway */
+      dims[a] = 0;
+      too_large = 1;
+      continue;
+    }
+    dims[a] = (int32_t)ceilf(r) + 1;
+    if (!too_large) {
+      t *= dims[a];
+      if (t > 100000000) too_large = 1;
+    }
+  }
+  *total = too_large ? -1 : t;
+  return too_large ? -1 : 0;
+}
+
+static int32_t clamp_i32(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* Cell coordinates of a position: :36-49 (assignCellsKernel) and :103-108 (the
+ * force kernel repeats the same arithmetic and clamp). */
+static void grid_coords(float px, float py, float pz, const float bbox[6], const int32_t dims[3],
+                        float cell, int32_t c[3]) {
+  c[0] = clamp_i32((int32_t)floorf((px - bbox[0]) / cell), 0, dims[0] - 1);
+  c[1] = clamp_i32((int32_t)floorf((py - bbox[1]) / cell), 0, dims[1] - 1);
+  c[2] = clamp_i32((int32_t)floorf((pz - bbox[2]) / cell), 0, dims[2] - 1);
+}
+
+/* force_spatial_hash.cu:278-308 as a counting sort, stable in the particle index. */
+void orc_grid_build(size_t n, const float* x, const float* y, const float* z, float cell,
+                    const float bbox[6], const int32_t dims[3], int32_t* particle_cell,
+                    int32_t* cell_start, int32_t* cell_end, int32_t* sorted) {
+  const size_t total = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
+  int32_t* fill = (int32_t*)calloc(total ? total : 1, sizeof(int32_t));
+  for (size_t i = 0; i < n; ++i) {
+    int32_t c[3];
+    grid_coords(x[i], y[i], z[i], bbox, dims, cell, c);
+    const int32_t id = c[0] + c[1] * dims[0] + c[2] * dims[0] * dims[1];
+    particle_cell[i] = id;
+    fill[id] += 1;
+  }
+  int32_t run = 0;
+  for (size_t c = 0; c < total; ++c) {
+    const int32_t cnt = fill[c];
+    cell_start[c] = run;
+    cell_end[c] = run + cnt;
+    fill[c] = run;
+    run += cnt;
+  }
+  for (size_t i = 0; i < n; ++i) sorted[fill[particle_cell[i]]++] = (int32_t)i;
+  free(fill);
+}
+
+void orc_grid_forces_f32(size_t n, const float* x, const float* y, const float* z,
+                         const float* m, const float bbox[6], const int32_t dims[3], float cell,
+                         const int32_t* cell_start, const int32_t* cell_end,
+                         const int32_t* sorted, float cutoff, float G, float eps,
+                         const int64_t* targets, size_t ntargets, float* ax, float* ay,
+                         float* az, int nthreads) {
+  const size_t nt = targets ? ntargets : n;
+  const float cutoff2 = cutoff * cutoff; /* :320 */
+  const float eps2 = eps * eps;
+  int th = pick_threads(nthreads);
+  (void)th;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(th)
+  for (long long t = 0; t < (long long)nt; ++t) {
+    const int32_t i = targets ? (int32_t)targets[t] : (int32_t)t;
+    const float xi = x[i], yi = y[i], zi = z[i];
+    int32_t c[3];
+    grid_coords(xi, yi, zi, bbox, dims, cell, c);
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int32_t nx = c[0] + dx, ny = c[1] + dy, nz = c[2] + dz;
+          if (nx < 0 || nx >= dims[0] || ny < 0 || ny >= dims[1] || nz < 0 || nz >= dims[2])
+            continue;
+          const int32_t id = nx + ny * dims[0] + nz * dims[0] * dims[1];
+          for (int32_t k = cell_start[id]; k < cell_end[id]; ++k) {
+            const int32_t j = sorted[k];
+            if (j == i) continue;
+            const float rx = x[j] - xi, ry = y[j] - yi, rz = z[j] - zi;
+            float d2 = rx * rx + ry * ry + rz * rz;
+            if (d2 < cutoff2) {
+              d2 += eps2;
+              const float inv = 1.0f / sqrtf(d2);
+              const float inv3 = inv * inv * inv;
+              const float f = G * m[j] * inv3;
+              sx += f * rx;
+              sy += f * ry;
+              sz += f * rz;
+            }
+          }
+        }
+    ax[t] = sx;
+    ay[t] = sy;
+    az[t] = sz;
+  }
+}
+
+/* FP64 contribution of one kept pair (shared by the grid and brute-force sums). */
+static void add_f64(double G, double eps2, float mj, double rx, double ry, double rz, double* sx,
+                    double* sy, double* sz, double* smag) {
+  const double r2 = rx * rx + ry * ry + rz * rz;
+  const double d2 = r2 + eps2;
+  const double inv3 = 1.0 / (d2 * sqrt(d2));
+  const double gm = G * (double)mj;
+  *sx += gm * inv3 * rx;
+  *sy += gm * inv3 * ry;
+  *sz += gm * inv3 * rz;
+  *smag += fabs(gm) * inv3 * sqrt(r2);
+}
+
+static int keep_pair(float xi, float yi, float zi, float xj, float yj, float zj, float cutoff2) {
+  const float rx = xj - xi, ry = yj - yi, rz = zj - zi;
+  const float d2 = rx * rx + ry * ry + rz * rz;
+  return d2 < cutoff2;
+}
+
+void orc_grid_forces_f64(size_t n, const float* x, const float* y, const float* z,
+                         const float* m, const float bbox[6], const int32_t dims[3], float cell,
+                         const int32_t* cell_start, const int32_t* cell_end,
+                         const int32_t* sorted, float cutoff, double G, double eps2,
+                         const int64_t* targets, size_t ntargets, double* ax, double* ay,
+                         double* az, double* mag, int64_t* npairs, int nthreads) {
+  const size_t nt = targets ? ntargets : n;
+  const float cutoff2 = cutoff * cutoff;
+  int th = pick_threads(nthreads);
+  (void)th;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(th)
+  for (long long t = 0; t < (long long)nt; ++t) {
+    const int32_t i = targets ? (int32_t)targets[t] : (int32_t)t;
+    const float xi = x[i], yi = y[i], zi = z[i];
+    int32_t c[3];
+    grid_coords(xi, yi, zi, bbox, dims, cell, c);
+    double sx = 0.0, sy = 0.0, sz = 0.0, smag = 0.0;
+    int64_t np = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int32_t nx = c[0] + dx, ny = c[1] + dy, nz = c[2] + dz;
+          if (nx < 0 || nx >= dims[0] || ny < 0 || ny >= dims[1] || nz < 0 || nz >= dims[2])
+            continue;
+          const int32_t id = nx + ny * dims[0] + nz * dims[0] * dims[1];
+          for (int32_t k = cell_start[id]; k < cell_end[id]; ++k) {
+            const int32_t j = sorted[k];
+            if (j == i || !keep_pair(xi, yi, zi, x[j], y[j], z[j], cutoff2)) continue;
+            add_f64(G, eps2, m[j], (double)x[j] - xi, (double)y[j] - yi, (double)z[j] - zi, &sx,
+                    &sy, &sz, &smag);
+            ++np;
+          }
+        }
+    ax[t] = sx;
+    ay[t] = sy;
+    az[t] = sz;
+    if (mag) mag[t] = smag;
+    if (npairs) npairs[t] = np;
+  }
+}
+
+void orc_cutoff_forces_f64(size_t n, const float* x, const float* y, const float* z,
+                           const float* m, float cutoff, double G, double eps2,
+                           const int64_t* targets, size_t ntargets, double* ax, double* ay,
+                           double* az, int64_t* npairs, int nthreads) {
+  const size_t nt = targets ? ntargets : n;
+  const float cutoff2 = cutoff * cutoff;
+  int th = pick_threads(nthreads);
+  (void)th;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(th)
+  for (long long t = 0; t < (long long)nt; ++t) {
+    const size_t i = targets ? (size_t)targets[t] : (size_t)t;
+    const float xi = x[i], yi = y[i], zi = z[i];
+    double sx = 0.0, sy = 0.0, sz = 0.0, smag = 0.0;
+    int64_t np = 0;
+    for (size_t j = 0; j < n; ++j) {
+      if (j == i || !keep_pair(xi, yi, zi, x[j], y[j], z[j], cutoff2)) continue;
+      add_f64(G, eps2, m[j], (double)x[j] - xi, (double)y[j] - yi, (double)z[j] - zi, &sx, &sy,
+              &sz, &smag);
+      ++np;
+    }
+    ax[t] = sx;
+    ay[t] = sy;
+    az[t] = sz;
+    if (npairs) npairs[t] = np;
+  }
+}

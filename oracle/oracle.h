@@ -26,6 +26,15 @@
  *                           invariants + mass conservation (test_barnes_hut.cpp)
  *   orc_bh_forces           theta->0 limit vs FP64 direct sum, 2% KAT of
  *                           test_spatial_hash.cpp:171-227, convergence KAT
+ *   orc_grid_cell_index     KAT of test_spatial_hash.cpp:34-47
+ *   orc_grid_hash_cell      Python modulo on every offset of a small box
+ *   orc_grid_dims           numpy float32 restatement of the sizing formulas
+ *   orc_grid_build          numpy bincount/cumsum and argsort(kind="stable");
+ *                           one-cell-per-particle property (test_spatial_hash.cpp:133-167);
+ *                           every particle inside its cell's bounds (verifyCellAssignment)
+ *   orc_grid_forces_f32/f64 brute-force cutoff sum without a grid (same pair set,
+ *                           FP64), cutoff-boundary KAT of test_spatial_hash.cpp:81-127,
+ *                           cutoff >= all distances reduces to the FP64 direct sum
  */
 #ifndef NBODY_ORACLE_H
 #define NBODY_ORACLE_H
@@ -123,6 +132,53 @@ void orc_bh_forces(size_t n, const orc_node* nodes, int32_t node_count, const fl
 int orc_bh_full(size_t n, const float* x, const float* y, const float* z, const float* m,
                 float theta, float G, float eps, size_t max_nodes, orc_node* nodes,
                 uint32_t* codes, int32_t* idx, float* ax, float* ay, float* az, int nthreads);
+
+/* ---- Spatial hash pieces (force_spatial_hash.cu) ---- */
+/* SpatialHashGrid::getCellIndex (:15-18): (int)floorf(p / cell) per axis. */
+void orc_grid_cell_index(float x, float y, float z, float cell, int32_t out[3]);
+/* SpatialHashGrid::hashCell (:20-26): each axis wrapped into [0, dims), linearised. */
+int32_t orc_grid_hash_cell(const int32_t cell[3], const int32_t dims[3]);
+/* Grid sizing of SpatialHashGrid::build (:190-260): exact bbox padded by 0.001f
+ * (:232-238), dims = (int)ceilf((max-min)/cell) + 1 (:250-252), total = product
+ * in 64 bits (:255). bbox = {min x,y,z, max x,y,z} after padding. Returns 0, or
+ * -1 when total exceeds 1e8 cells (:257-260) or a ratio is not finite. */
+int orc_grid_dims(size_t n, const float* x, const float* y, const float* z, float cell,
+                  float bbox[6], int32_t dims[3], int64_t* total);
+/* Cell contents of SpatialHashGrid::build (:278-308): particle_cell (n) by
+ * :36-49, cell_start / cell_end (total) as the exclusive scan of the counts and
+ * start + count (:291-300), sorted (n) the indices grouped by cell; within a
+ * cell ascending (the reference's atomic scatter, :72-81, leaves that order
+ * open). */
+void orc_grid_build(size_t n, const float* x, const float* y, const float* z, float cell,
+                    const float bbox[6], const int32_t dims[3], int32_t* particle_cell,
+                    int32_t* cell_start, int32_t* cell_end, int32_t* sorted);
+/* spatialHashForceKernel (:84-158) in IEEE FP32 with 1.0f/sqrtf for rsqrtf:
+ * target cell from its position with the clamp (:103-108), neighbour cells in
+ * dz, dy, dx order, bodies in `sorted` order, j == i skipped, pair kept when
+ * (dx*dx + dy*dy) + dz*dz < cutoff*cutoff (FP32, evaluated as written). targets
+ * as in orc_direct_f32. */
+void orc_grid_forces_f32(size_t n, const float* x, const float* y, const float* z,
+                         const float* m, const float bbox[6], const int32_t dims[3], float cell,
+                         const int32_t* cell_start, const int32_t* cell_end,
+                         const int32_t* sorted, float cutoff, float G, float eps,
+                         const int64_t* targets, size_t ntargets, float* ax, float* ay,
+                         float* az, int nthreads);
+/* The same pair set summed in FP64: G m_j r_ij (r^2 + eps2)^{-3/2} in double.
+ * mag (may be NULL) receives sum_j |G m_j| |r_ij| (r^2+eps2)^{-3/2} per target,
+ * the scale for rounding-error bounds; npairs (may be NULL) the kept pairs. */
+void orc_grid_forces_f64(size_t n, const float* x, const float* y, const float* z,
+                         const float* m, const float bbox[6], const int32_t dims[3], float cell,
+                         const int32_t* cell_start, const int32_t* cell_end,
+                         const int32_t* sorted, float cutoff, double G, double eps2,
+                         const int64_t* targets, size_t ntargets, double* ax, double* ay,
+                         double* az, double* mag, int64_t* npairs, int nthreads);
+/* Brute force without a grid: every j != i whose FP32 (dx*dx + dy*dy) + dz*dz
+ * is below cutoff*cutoff, summed in FP64. Equal to the grid sum whenever the
+ * 27-cell search is complete (cell >= cutoff). */
+void orc_cutoff_forces_f64(size_t n, const float* x, const float* y, const float* z,
+                           const float* m, float cutoff, double G, double eps2,
+                           const int64_t* targets, size_t ntargets, double* ax, double* ay,
+                           double* az, int64_t* npairs, int nthreads);
 
 int orc_max_threads(void);
 

@@ -62,6 +62,18 @@ _s("orc_build_tree", C.c_int, C.c_size_t, F32P, F32P, F32P, F32P, I32P, F32P, F3
    C.c_void_p, I32P, I32P, I32P, I32P)
 _s("orc_bh_forces", None, C.c_size_t, C.c_void_p, C.c_int32, F32P, F32P, F32P, F32P, C.c_float,
    C.c_float, C.c_float, I64P, C.c_size_t, F32P, F32P, F32P, C.c_int)
+_s("orc_grid_cell_index", None, C.c_float, C.c_float, C.c_float, C.c_float, I32P)
+_s("orc_grid_hash_cell", C.c_int32, I32P, I32P)
+_s("orc_grid_dims", C.c_int, C.c_size_t, F32P, F32P, F32P, C.c_float, F32P, I32P, I64P)
+_s("orc_grid_build", None, C.c_size_t, F32P, F32P, F32P, C.c_float, F32P, I32P, I32P, I32P, I32P,
+   I32P)
+_s("orc_grid_forces_f32", None, C.c_size_t, F32P, F32P, F32P, F32P, F32P, I32P, C.c_float, I32P,
+   I32P, I32P, C.c_float, C.c_float, C.c_float, I64P, C.c_size_t, F32P, F32P, F32P, C.c_int)
+_s("orc_grid_forces_f64", None, C.c_size_t, F32P, F32P, F32P, F32P, F32P, I32P, C.c_float, I32P,
+   I32P, I32P, C.c_float, C.c_double, C.c_double, I64P, C.c_size_t, F64P, F64P, F64P, F64P, I64P,
+   C.c_int)
+_s("orc_cutoff_forces_f64", None, C.c_size_t, F32P, F32P, F32P, F32P, C.c_float, C.c_double,
+   C.c_double, I64P, C.c_size_t, F64P, F64P, F64P, I64P, C.c_int)
 _s("orc_max_threads", C.c_int)
 
 
@@ -222,3 +234,87 @@ def bh_pipeline(h: dict, max_particles: int | None = None):
     tree.update({"bbox_min": lo, "bbox_max": hi, "codes": scodes, "sorted_idx": sidx,
                  "unsorted_codes": codes})
     return tree
+
+
+# ------------------------------------------------------------ spatial hash
+
+def _i32(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(I32P)
+
+
+def grid_cell_index(x, y, z, cell) -> tuple:
+    out = np.zeros(3, np.int32)
+    lib.orc_grid_cell_index(x, y, z, cell, _i32(out))
+    return tuple(int(v) for v in out)
+
+
+def grid_hash_cell(cell, dims) -> int:
+    c, d = np.asarray(cell, np.int32).copy(), np.asarray(dims, np.int32).copy()
+    return int(lib.orc_grid_hash_cell(_i32(c), _i32(d)))
+
+
+def grid_dims(x, y, z, cell):
+    """Padded bbox (6 floats), dims (3 ints) and total cells, or None if the grid
+    would exceed 1e8 cells (SpatialHashGrid::build, force_spatial_hash.cu:241-260)."""
+    x, y, z = map(f32, (x, y, z))
+    bbox, dims, total = np.zeros(6, np.float32), np.zeros(3, np.int32), C.c_int64()
+    rc = lib.orc_grid_dims(len(x), _f(x), _f(y), _f(z), cell, _f(bbox), _i32(dims),
+                           C.byref(total))
+    if rc != 0:
+        return None
+    return bbox, dims, total.value
+
+
+def grid_build(x, y, z, cell):
+    """Everything SpatialHashGrid::build leaves behind, as a dict."""
+    x, y, z = map(f32, (x, y, z))
+    sized = grid_dims(x, y, z, cell)
+    if sized is None:
+        raise RuntimeError("Spatial hash grid too large: reduce cell_size or bounding box")
+    bbox, dims, total = sized
+    n = len(x)
+    pc, srt = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    cs, ce = np.zeros(total, np.int32), np.zeros(total, np.int32)
+    lib.orc_grid_build(n, _f(x), _f(y), _f(z), cell, _f(bbox), _i32(dims), _i32(pc), _i32(cs),
+                       _i32(ce), _i32(srt))
+    return {"bbox": bbox, "dims": dims, "total": total, "cell": np.float32(cell),
+            "particle_cell": pc, "cell_start": cs, "cell_end": ce, "sorted": srt}
+
+
+def grid_forces_f32(g: dict, x, y, z, m, cutoff, G, eps, targets=None, nthreads=0):
+    x, y, z, m = map(f32, (x, y, z, m))
+    t, nt, nout = _targets(targets, len(x))
+    out = [np.zeros(nout, np.float32) for _ in range(3)]
+    lib.orc_grid_forces_f32(len(x), _f(x), _f(y), _f(z), _f(m), _f(g["bbox"]), _i32(g["dims"]),
+                            float(g["cell"]), _i32(g["cell_start"]), _i32(g["cell_end"]),
+                            _i32(g["sorted"]), cutoff, G, eps,
+                            t.ctypes.data_as(I64P) if t is not None else None, nt,
+                            *map(_f, out), nthreads)
+    return np.stack(out, axis=1)
+
+
+def grid_forces_f64(g: dict, x, y, z, m, cutoff, G, eps2, targets=None, nthreads=0):
+    """(acc (n,3) float64, sum of |contribution| per target, kept pairs per target)."""
+    x, y, z, m = map(f32, (x, y, z, m))
+    t, nt, nout = _targets(targets, len(x))
+    out = [np.zeros(nout, np.float64) for _ in range(4)]
+    np_ = np.zeros(nout, np.int64)
+    lib.orc_grid_forces_f64(len(x), _f(x), _f(y), _f(z), _f(m), _f(g["bbox"]), _i32(g["dims"]),
+                            float(g["cell"]), _i32(g["cell_start"]), _i32(g["cell_end"]),
+                            _i32(g["sorted"]), cutoff, G, eps2,
+                            t.ctypes.data_as(I64P) if t is not None else None, nt,
+                            *map(_d, out), np_.ctypes.data_as(I64P), nthreads)
+    return np.stack(out[:3], axis=1), out[3], np_
+
+
+def cutoff_forces_f64(x, y, z, m, cutoff, G, eps2, targets=None, nthreads=0):
+    """Brute-force cutoff sum (no grid): (acc (n,3) float64, kept pairs per target)."""
+    x, y, z, m = map(f32, (x, y, z, m))
+    t, nt, nout = _targets(targets, len(x))
+    out = [np.zeros(nout, np.float64) for _ in range(3)]
+    np_ = np.zeros(nout, np.int64)
+    lib.orc_cutoff_forces_f64(len(x), _f(x), _f(y), _f(z), _f(m), cutoff, G, eps2,
+                              t.ctypes.data_as(I64P) if t is not None else None, nt,
+                              *map(_d, out), np_.ctypes.data_as(I64P), nthreads)
+    return np.stack(out, axis=1), np_
