@@ -21,10 +21,17 @@ The same line carries the other two parts of the metric:
                steps advancing each system alone; per-system forces are
                bit-identical to the standalone kernel.
 
---only {direct,barnes-hut,ensemble,precision} runs one part (for profiling);
-precision compares plain FP32 with the double-single/Kahan form at N=65,536.
+It also carries spatial_hash: the short-range operator on the reference's own
+force.spatial_hash workload (1,048,576 bodies uniform in a sphere of radius 10,
+unit masses, cell = cutoff = 2, eps = 0.1), ms per computeForces call (grid
+build + forces) and the roofline of the force kernel.
+
+--only {direct,barnes-hut,ensemble,precision,spatial-hash} runs one part (for
+profiling); precision compares plain FP32 with the double-single/Kahan form at
+N=65,536.
 """
 import argparse
+import itertools
 import json
 import os
 import re
@@ -247,6 +254,105 @@ def run_bh(nbx, args):
     return out
 
 
+# ------------------------------------------------------------------ spatial hash
+
+# Algorithmic flop of the spatial-hash force kernel: every candidate pair (a body
+# of the 27 neighbour cells, itself excluded) costs the distance test, 3 sub +
+# 3 mul + 2 add = 8 flop; a kept pair adds + eps^2, rsqrt, the force factor
+# (G m) (inv^3) and the three accumulations, 12 flop more (20 in all, as the
+# direct kernel counts an interaction).
+SH_FLOP_CANDIDATE = 8.0
+SH_FLOP_KEPT = 12.0
+
+
+def grid_pair_counts(orc, h, cell, cutoff, eps, sample):
+    """Candidate pairs of one force call (exact, from the cell counts) and kept pairs
+    (pairs inside the cutoff, from the oracle on `sample` random targets)."""
+    x, y, z, m = h["pos_x"], h["pos_y"], h["pos_z"], h["mass"]
+    g = orc.grid_build(x, y, z, cell)
+    dx, dy, dz = (int(v) for v in g["dims"])
+    counts = (g["cell_end"] - g["cell_start"]).astype(np.int64).reshape(dz, dy, dx)
+    p = np.pad(counts, 1)
+    near = sum(p[1 + a:1 + a + dz, 1 + b:1 + b + dy, 1 + c:1 + c + dx]
+               for a, b, c in itertools.product((-1, 0, 1), repeat=3))
+    candidates = int((counts * (near - 1)).sum())
+    n = len(x)
+    tg = np.sort(np.random.default_rng(5).choice(n, min(sample, n), replace=False))
+    _, _, kept = orc.grid_forces_f64(g, x, y, z, m, cutoff, 1.0, eps2_of(eps), targets=tg)
+    return g, candidates, float(kept.mean()) * n, tg
+
+
+def run_spatial_hash(nbx, args):
+    n, G, eps, cell, cutoff, R = args.sh_n, 1.0, 0.1, 2.0, 2.0, 10.0
+    h = nbx.gen_uniform_sphere(n, seed=42, R=R, M=float(n), sigma=0.0)
+    d = nbx.DeviceParticles.from_host(h)
+    grid = nbx.Grid(n, cell)
+    s = nbx.Stream()
+    grid.build(d, s)
+    grid.forces(d, cutoff, G, eps, s)
+    s.synchronize()
+    K = args.sh_steps
+    e0, e1, f0, f1 = nbx.Event(), nbx.Event(), nbx.Event(), nbx.Event()
+    e0.record(s)
+    for _ in range(K):  # SpatialHashCalculator::computeForces: build, then forces
+        grid.build(d, s)
+        grid.forces(d, cutoff, G, eps, s)
+    e1.record(s)
+    s.synchronize()
+    ms_call = e0.elapsed_ms(e1) / K
+    f0.record(s)
+    for _ in range(K):
+        grid.forces(d, cutoff, G, eps, s)
+    f1.record(s)
+    s.synchronize()
+    ms_forces = f0.elapsed_ms(f1) / K
+    info = grid.info()
+    orc = oracle()
+    g, cand, kept, tg = grid_pair_counts(orc, h, cell, cutoff, eps, args.sh_check_targets)
+    flop = SH_FLOP_CANDIDATE * cand + SH_FLOP_KEPT * kept
+    tflops = flop / (ms_forces * 1e-3) / 1e12
+    acc = d.acc()
+    a64, mag, npairs = orc.grid_forces_f64(g, h["pos_x"], h["pos_y"], h["pos_z"], h["mass"],
+                                           cutoff, G, eps2_of(eps), targets=tg)
+    err = np.abs(acc[tg].astype(np.float64) - a64).max(axis=1) / np.maximum(mag, 1e-300)
+    log(f"spatial hash N={n}: {ms_call:.3f} ms/call (forces {ms_forces:.3f} ms), "
+        f"{cand / n:.0f} candidates and {kept / n:.0f} kept pairs per target, "
+        f"{tflops:.1f} TFLOP/s = {tflops / PEAK_FP32_TFLOPS:.1%} of FP32 peak, "
+        f"err/sum|terms| max {err.max():.2e}")
+    out = {"n": n, "cell": cell, "cutoff": cutoff, "eps": eps, "G": G, "calls": K,
+           "workload": f"uniform sphere R={R:g}, unit masses (the reference's force.spatial_hash "
+                       f"benchmark: spherical init, default cell, cutoff and softening)",
+           "grid_dims": info["dims"].tolist(), "total_cells": info["total"],
+           "ms_per_call": ms_call, "ms_forces": ms_forces, "ms_build": ms_call - ms_forces,
+           "candidate_pairs": cand, "kept_pairs_est": kept,
+           "roofline": {"bound": "valu-fp32", "kernel": "grid_forces (with the body gather)",
+                        "achieved": tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": tflops / PEAK_FP32_TFLOPS,
+                        "flop": f"{SH_FLOP_CANDIDATE:g} per candidate pair + "
+                                f"{SH_FLOP_KEPT:g} per kept pair"},
+           "force_error": {"targets": len(tg), "max_err_over_sum_abs_terms": float(err.max()),
+                           "median": float(np.median(err))}}
+    if not args.no_cpu_baseline:
+        threads = orc.max_threads()
+        x, y, z, m = h["pos_x"], h["pos_y"], h["pos_z"], h["mass"]
+        t0 = time.perf_counter()
+        gb = orc.grid_build(x, y, z, cell)
+        build_s = time.perf_counter() - t0
+        sample = min(n, args.sh_cpu_targets)
+        st = np.sort(np.random.default_rng(6).choice(n, sample, replace=False))
+        t0 = time.perf_counter()
+        orc.grid_forces_f32(gb, x, y, z, m, cutoff, G, eps, targets=st, nthreads=threads)
+        f_s = time.perf_counter() - t0
+        est = (build_s + f_s * n / sample) * 1e3
+        out["cpu_baseline"] = {
+            "value": est, "unit": "ms/call", "cores": threads, "kind": "port",
+            "sample": f"oracle grid build (single thread) {build_s * 1e3:.0f} ms + FP32 forces of "
+                      f"{sample} of {n} targets on {threads} threads {f_s * 1e3:.0f} ms, scaled "
+                      f"to all targets"}
+        log(f"spatial hash cpu baseline: {est:.1f} ms/call")
+    return out
+
+
 # ------------------------------------------------------------------ ensemble
 
 def run_ensemble(nbx, args):
@@ -390,7 +496,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=262144)
-    ap.add_argument("--only", choices=["direct", "barnes-hut", "ensemble", "precision"])
+    ap.add_argument("--only", choices=["direct", "barnes-hut", "ensemble", "precision",
+                                       "spatial-hash"])
     ap.add_argument("--variant", type=int, default=-1, help="force a direct tile variant")
     ap.add_argument("--check-targets", type=int, default=4096,
                     help="sample targets of the FP64 force check")
@@ -406,6 +513,10 @@ def main() -> int:
     ap.add_argument("--ens-steps", type=int, default=100)
     ap.add_argument("--ens-chunk", type=int, default=100)
     ap.add_argument("--ens-seq-steps", type=int, default=5)
+    ap.add_argument("--sh-n", type=int, default=1 << 20)
+    ap.add_argument("--sh-steps", type=int, default=10)
+    ap.add_argument("--sh-check-targets", type=int, default=2048)
+    ap.add_argument("--sh-cpu-targets", type=int, default=8192)
     ap.add_argument("--prec-n", type=int, default=65536)
     ap.add_argument("--prec-steps", type=int, default=10000)
     args = ap.parse_args()
@@ -424,6 +535,10 @@ def main() -> int:
         return 0
     if args.only == "barnes-hut":
         print(json.dumps({"barnes_hut": run_bh(nbx, args), "device": info["name"]}), flush=True)
+        return 0
+    if args.only == "spatial-hash":
+        print(json.dumps({"spatial_hash": run_spatial_hash(nbx, args), "device": info["name"]}),
+              flush=True)
         return 0
     if args.only == "ensemble":
         print(json.dumps({"ensemble": run_ensemble(nbx, args), "device": info["name"]}),
@@ -452,6 +567,7 @@ def main() -> int:
     if args.only != "direct":
         line["barnes_hut"] = run_bh(nbx, args)
         line["ensemble"] = run_ensemble(nbx, args)
+        line["spatial_hash"] = run_spatial_hash(nbx, args)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -57,7 +57,27 @@ class BarnesHutCalculator : public ForceCalculator {
   float theta_;
 };
 
-// DIRECT_N2 and BARNES_HUT; SPATIAL_HASH throws ValidationException (not built).
+// Short-range forces over a uniform cell grid. cell_size must be >= cutoff_radius
+// for the 3x3x3 neighbour search to be complete; computeForces uses
+// max(cell_size, cutoff_radius) as the grid's cell either way.
+class SpatialHashCalculator : public ForceCalculator {
+ public:
+  SpatialHashCalculator(float cell_size = 2.0f, float cutoff_radius = 2.0f);
+  ~SpatialHashCalculator();
+  void computeForces(ParticleData* d_particles) override;
+  ForceMethod getMethod() const noexcept override { return ForceMethod::SPATIAL_HASH; }
+  void setCellSize(float size) { cell_size_ = size; }
+  void setCutoffRadius(float radius) { cutoff_radius_ = radius; }
+  float getCellSize() const noexcept { return cell_size_; }
+  float getCutoffRadius() const noexcept { return cutoff_radius_; }
+  SpatialHashGrid* getGrid() noexcept { return grid_.get(); }
+
+ private:
+  std::unique_ptr<SpatialHashGrid> grid_;
+  float cell_size_;
+  float cutoff_radius_;
+};
+
 std::unique_ptr<ForceCalculator> createForceCalculator(ForceMethod method,
                                                        const SimulationConfig& config);
 

@@ -37,6 +37,8 @@ class ParticleSystem {
   void setSofteningParameter(float eps);
   void setTimeStep(float dt);
   void setBarnesHutTheta(float theta);
+  void setSpatialHashCellSize(float size);
+  void setSpatialHashCutoff(float cutoff);
 
   ForceMethod getForceMethod() const noexcept { return force_method_; }
   float getGravitationalConstant() const noexcept { return G_; }

@@ -6,17 +6,29 @@
 #include <cmath>
 #include <cstddef>
 
+// Integer 3-vector of the cell coordinates (SpatialHashGrid::getCellIndex): HIP's
+// own int3 when the including file is compiled by hipcc, a plain struct of the
+// same layout otherwise (the reference does the same with CUDA's, types.hpp:6-21).
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#else
+struct int3 {
+  int x, y, z;
+};
+inline constexpr int3 make_int3(int x, int y, int z) { return {x, y, z}; }
+#endif
+
 namespace nbody {
 
 struct ParticleData;
 class ForceCalculator;
 class Integrator;
 class BarnesHutTree;
+class SpatialHashGrid;
 class ParticleSystem;
 
 // Values 0/1/2 are part of the .nbody checkpoint format (serialization.cpp:52-55
-// of the reference); SPATIAL_HASH is kept so checkpoints round-trip, but this
-// engine builds no spatial-hash operator.
+// of the reference).
 enum class ForceMethod { DIRECT_N2, BARNES_HUT, SPATIAL_HASH };
 
 inline const char* forceMethodToString(ForceMethod method) {

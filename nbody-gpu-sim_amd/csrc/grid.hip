@@ -34,9 +34,10 @@
 //     reach every lane as SGPR operands, and a lane drops the bodies outside
 //     its own three cells with a mask. Otherwise every lane walks its own
 //     ranges with vector loads.
-//   * the cutoff test is evaluated as written, without FMA contraction, so
-//     the set of pairs is exactly the one the CPU restatement keeps; the kept
-//     pairs use v_rsq_f32 and FMA accumulation (tolerance-level differences).
+//   * r^2 is fma(rz, rz, fma(rx, rx, ry*ry)) on both sides (here and in the
+//     oracle), so the set of pairs kept by the cutoff test is exactly the
+//     oracle's; the kept pairs use v_rsq_f32 and FMA accumulation
+//     (tolerance-level differences).
 //   * both walks visit a target's bodies in the same order with the same
 //     arithmetic, so a target's result does not depend on which one ran.
 #include <algorithm>
@@ -152,61 +153,121 @@ __global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restric
   body[k] = make_float4(x[i], y[i], z[i], mul_rn(G, m[i]));
 }
 
-// One pair of :133-149: r = x_j - x_i, d2 = (rx*rx + ry*ry) + rz*rz rounded as
-// written, kept when d2 < cutoff2 (and the caller's mask), then
-// f = (G m_j) * ((inv*inv)*inv) with inv = rsq(d2 + eps2).
+// r^2 of one pair as fma(rz, rz, fma(rx, rx, ry*ry)): the fused form an LLVM-based
+// compiler gives the reference's dx*dx + dy*dy + dz*dz (:137) under its default
+// contraction. The oracle evaluates the same form, so the pairs kept by
+// r^2 < cutoff^2 are exactly the same on both sides.
 __device__ __forceinline__ float grid_d2(float rx, float ry, float rz) {
-  return add_rn(add_rn(mul_rn(rx, rx), mul_rn(ry, ry)), mul_rn(rz, rz));
+  return __builtin_fmaf(rz, rz, __builtin_fmaf(rx, rx, mul_rn(ry, ry)));
 }
 
-// Walk of [ulo, uhi) with wave-uniform indices (scalar loads, SGPR operands).
-// MEMBER: lanes keep only bodies of their own range [own_lo, own_hi).
-template <bool MEMBER>
+// v_cndmask with the keep mask in SGPRs (a plain select may be turned into a
+// branch around the force factor).
+__device__ __forceinline__ float keep_or_zero(uint64_t keep, float v) {
+  float r;
+  asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(r) : "v"(v), "s"(keep));
+  return r;
+}
+
+template <int NB>
+__device__ __forceinline__ void load_bodies(const float4* __restrict__ body, int j,
+                                            float4 (&b)[NB]) {
+#pragma unroll
+  for (int q = 0; q < NB; ++q) b[q] = body[j + q];
+}
+
+// kNB bodies (wave-uniform, in SGPRs) against each lane's target, issued stage
+// by stage over independent pairs: differences, then (once this batch's
+// operands are in) the loads of the next batch nxt_j, r^2 and the masks, the
+// rsq back to back, the force factors, the accumulations. Per pair (:133-149):
+// keep = r^2 < cutoff2 [and j != self] [and j in the lane's own range],
+// f = (G m_j) ((inv inv) inv), inv = rsq(r^2 + eps2), a += f r.
+template <bool SELF, bool MEMBER>
+__device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], int j,
+                                           const float4* __restrict__ body, int nxt_j,
+                                           float4 (&nxt)[kNB], int own_lo, int own_hi, int self,
+                                           float xi, float yi, float zi, float cutoff2,
+                                           float eps2, float& fx, float& fy, float& fz) {
+  float rx[kNB], ry[kNB], rz[kNB], w[kNB];
+  uint64_t keep[kNB];
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) {
+    rx[q] = sub_rn(b[q].x, xi);
+    ry[q] = sub_rn(b[q].y, yi);
+    rz[q] = sub_rn(b[q].z, zi);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  load_bodies<kNB>(body, nxt_j, nxt);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) {
+    const float d2 = grid_d2(rx[q], ry[q], rz[q]);
+    uint64_t k = __builtin_amdgcn_ballot_w64(d2 < cutoff2);
+    if (SELF) k &= __builtin_amdgcn_ballot_w64(j + q != self);
+    if (MEMBER)
+      k &= __builtin_amdgcn_ballot_w64(j + q >= own_lo) & __builtin_amdgcn_ballot_w64(j + q < own_hi);
+    keep[q] = k;
+    w[q] = add_rn(d2, eps2);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) w[q] = __builtin_amdgcn_rsqf(w[q]);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) {
+    const float inv = w[q];
+    w[q] = keep_or_zero(keep[q], mul_rn(b[q].w, mul_rn(mul_rn(inv, inv), inv)));
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int q = 0; q < kNB; ++q) {
+    fx = __builtin_fmaf(w[q], rx[q], fx);
+    fy = __builtin_fmaf(w[q], ry[q], fy);
+    fz = __builtin_fmaf(w[q], rz[q], fz);
+  }
+}
+
+// One pair with the same arithmetic (tails of a walk, per-lane walks).
+__device__ __forceinline__ void grid_pair(float4 b, bool keep, float xi, float yi, float zi,
+                                          float cutoff2, float eps2, float& fx, float& fy,
+                                          float& fz) {
+  const float rx = sub_rn(b.x, xi), ry = sub_rn(b.y, yi), rz = sub_rn(b.z, zi);
+  const float d2 = grid_d2(rx, ry, rz);
+  const uint64_t k = __builtin_amdgcn_ballot_w64(keep & (d2 < cutoff2));
+  const float inv = __builtin_amdgcn_rsqf(add_rn(d2, eps2));
+  const float f = keep_or_zero(k, mul_rn(b.w, mul_rn(mul_rn(inv, inv), inv)));
+  fx = __builtin_fmaf(f, rx, fx);
+  fy = __builtin_fmaf(f, ry, fy);
+  fz = __builtin_fmaf(f, rz, fz);
+}
+
+// Walk of [ulo, uhi) with wave-uniform indices (scalar loads, SGPR operands),
+// batches of kNB with the next batch's loads in flight. SELF: the range may hold
+// a lane's own body. MEMBER: lanes keep only bodies of their own [own_lo, own_hi).
+template <bool SELF, bool MEMBER>
 __device__ __forceinline__ void walk_uniform(const float4* __restrict__ body, int ulo, int uhi,
                                              int own_lo, int own_hi, int self, float xi,
                                              float yi, float zi, float cutoff2, float eps2,
                                              float& fx, float& fy, float& fz) {
   int j = ulo;
-  for (; j + kNB <= uhi; j += kNB) {
-    float4 b[kNB];
-#pragma unroll
-    for (int q = 0; q < kNB; ++q) b[q] = body[j + q];
-    float rx[kNB], ry[kNB], rz[kNB], w[kNB];
-    bool keep[kNB];
-#pragma unroll
-    for (int q = 0; q < kNB; ++q) {
-      rx[q] = sub_rn(b[q].x, xi);
-      ry[q] = sub_rn(b[q].y, yi);
-      rz[q] = sub_rn(b[q].z, zi);
-    }
-#pragma unroll
-    for (int q = 0; q < kNB; ++q) {
-      const float d2 = grid_d2(rx[q], ry[q], rz[q]);
-      const int jj = j + q;
-      keep[q] = (d2 < cutoff2) & (jj != self) & (!MEMBER | ((jj >= own_lo) & (jj < own_hi)));
-      w[q] = add_rn(d2, eps2);
-    }
-#pragma unroll
-    for (int q = 0; q < kNB; ++q) w[q] = __builtin_amdgcn_rsqf(w[q]);
-#pragma unroll
-    for (int q = 0; q < kNB; ++q) {
-      const float inv = w[q];
-      const float f = keep[q] ? mul_rn(b[q].w, mul_rn(mul_rn(inv, inv), inv)) : 0.0f;
-      fx = __builtin_fmaf(f, rx[q], fx);
-      fy = __builtin_fmaf(f, ry[q], fy);
-      fz = __builtin_fmaf(f, rz[q], fz);
+  if (j + kNB <= uhi) {
+    float4 cur[kNB], nxt[kNB];
+    load_bodies<kNB>(body, j, cur);
+    for (;;) {
+      const int jn = j + kNB;
+      const bool more = jn + kNB <= uhi;
+      grid_batch<SELF, MEMBER>(cur, j, body, more ? jn : j, nxt, own_lo, own_hi, self, xi, yi,
+                               zi, cutoff2, eps2, fx, fy, fz);
+      j = jn;
+      if (!more) break;
+      for (int q = 0; q < kNB; ++q) cur[q] = nxt[q];
     }
   }
   for (; j < uhi; ++j) {
-    const float4 b = body[j];
-    const float rx = sub_rn(b.x, xi), ry = sub_rn(b.y, yi), rz = sub_rn(b.z, zi);
-    const float d2 = grid_d2(rx, ry, rz);
-    const bool keep = (d2 < cutoff2) & (j != self) & (!MEMBER | ((j >= own_lo) & (j < own_hi)));
-    const float inv = __builtin_amdgcn_rsqf(add_rn(d2, eps2));
-    const float f = keep ? mul_rn(b.w, mul_rn(mul_rn(inv, inv), inv)) : 0.0f;
-    fx = __builtin_fmaf(f, rx, fx);
-    fy = __builtin_fmaf(f, ry, fy);
-    fz = __builtin_fmaf(f, rz, fz);
+    bool keep = true;
+    if (SELF) keep &= j != self;
+    if (MEMBER) keep &= (j >= own_lo) & (j < own_hi);
+    grid_pair(body[j], keep, xi, yi, zi, cutoff2, eps2, fx, fy, fz);
   }
 }
 
@@ -214,17 +275,8 @@ __device__ __forceinline__ void walk_uniform(const float4* __restrict__ body, in
 __device__ __forceinline__ void walk_lane(const float4* __restrict__ body, int lo, int hi,
                                           int self, float xi, float yi, float zi, float cutoff2,
                                           float eps2, float& fx, float& fy, float& fz) {
-  for (int j = lo; j < hi; ++j) {
-    const float4 b = body[j];
-    const float rx = sub_rn(b.x, xi), ry = sub_rn(b.y, yi), rz = sub_rn(b.z, zi);
-    const float d2 = grid_d2(rx, ry, rz);
-    const bool keep = (d2 < cutoff2) & (j != self);
-    const float inv = __builtin_amdgcn_rsqf(add_rn(d2, eps2));
-    const float f = keep ? mul_rn(b.w, mul_rn(mul_rn(inv, inv), inv)) : 0.0f;
-    fx = __builtin_fmaf(f, rx, fx);
-    fy = __builtin_fmaf(f, ry, fy);
-    fz = __builtin_fmaf(f, rz, fz);
-  }
+  for (int j = lo; j < hi; ++j)
+    grid_pair(body[j], j != self, xi, yi, zi, cutoff2, eps2, fx, fy, fz);
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -281,13 +333,24 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
         if (ny < 0 || ny >= g.dims[1]) continue;
         const int base = ny * g.dims[0] + nz * dxy;
         const int ulo = cstart[base + ulo_x], uhi = cend[base + uhi_x];
+        // the self test is needed only where a lane's own body lies in the range
+        // (the centre row, unless positions moved since the build)
+        const bool self_in = __any((k >= ulo) & (k < uhi));
         if (xmin == xmax) {
-          walk_uniform<false>(body, ulo, uhi, 0, 0, k, me.x, me.y, me.z, cutoff2, eps2, fx, fy,
-                              fz);
+          if (self_in)
+            walk_uniform<true, false>(body, ulo, uhi, 0, 0, k, me.x, me.y, me.z, cutoff2, eps2,
+                                      fx, fy, fz);
+          else
+            walk_uniform<false, false>(body, ulo, uhi, 0, 0, k, me.x, me.y, me.z, cutoff2, eps2,
+                                       fx, fy, fz);
         } else {
           const int olo = cstart[base + lo_x], ohi = cend[base + hi_x];
-          walk_uniform<true>(body, ulo, uhi, olo, ohi, k, me.x, me.y, me.z, cutoff2, eps2, fx,
-                             fy, fz);
+          if (self_in)
+            walk_uniform<true, true>(body, ulo, uhi, olo, ohi, k, me.x, me.y, me.z, cutoff2,
+                                     eps2, fx, fy, fz);
+          else
+            walk_uniform<false, true>(body, ulo, uhi, olo, ohi, k, me.x, me.y, me.z, cutoff2,
+                                      eps2, fx, fy, fz);
         }
       }
     }

@@ -1,10 +1,12 @@
 // Force operators over the C ABI (reference: src/cuda/force_direct.cu:100-117,
-// src/cuda/force_barnes_hut.cu:573-583, factory force_spatial_hash.cu:391-404).
+// src/cuda/force_barnes_hut.cu:573-583, src/cuda/force_spatial_hash.cu:370-404).
+#include <algorithm>
 #include <cmath>
 
 #include "nbody/barnes_hut_tree.hpp"
 #include "nbody/error_handling.hpp"
 #include "nbody/force_calculator.hpp"
+#include "nbody/spatial_hash_grid.hpp"
 #include "nbx.h"
 
 namespace nbody {
@@ -26,6 +28,21 @@ void BarnesHutCalculator::computeForces(ParticleData* d_particles) {
   tree_->computeForces(d_particles, theta_, G_, softening_eps_);
 }
 
+SpatialHashCalculator::SpatialHashCalculator(float cell_size, float cutoff_radius)
+    : cell_size_(cell_size), cutoff_radius_(cutoff_radius) {}
+SpatialHashCalculator::~SpatialHashCalculator() = default;
+
+// :376-388. The grid is remade when the effective cell changes, and also when the
+// particle count outgrows it (the reference keeps the first count's buffers).
+void SpatialHashCalculator::computeForces(ParticleData* d_particles) {
+  const float effective_cell = std::max(cell_size_, cutoff_radius_);
+  if (!grid_ || grid_->getCellSize() != effective_cell ||
+      d_particles->count > grid_->getMaxParticles())
+    grid_ = std::make_unique<SpatialHashGrid>(d_particles->count, effective_cell);
+  grid_->build(d_particles);
+  grid_->computeForces(d_particles, cutoff_radius_, G_, softening_eps_);
+}
+
 std::unique_ptr<ForceCalculator> createForceCalculator(ForceMethod method,
                                                        const SimulationConfig& config) {
   switch (method) {
@@ -34,7 +51,8 @@ std::unique_ptr<ForceCalculator> createForceCalculator(ForceMethod method,
     case ForceMethod::BARNES_HUT:
       return std::make_unique<BarnesHutCalculator>(config.barnes_hut_theta);
     case ForceMethod::SPATIAL_HASH:
-      throw ValidationException("The spatial-hash force method is not built in this engine");
+      return std::make_unique<SpatialHashCalculator>(config.spatial_hash_cell_size,
+                                                     config.spatial_hash_cutoff);
   }
   throw ValidationException("Unknown force method");
 }

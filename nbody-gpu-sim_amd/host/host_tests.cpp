@@ -22,6 +22,7 @@
 #include "nbody/particle_system.hpp"
 #include "nbody/performance_observability.hpp"
 #include "nbody/serialization.hpp"
+#include "nbody/spatial_hash_grid.hpp"
 
 using namespace nbody;
 
@@ -268,10 +269,28 @@ TEST_CPU(factory_and_spatial_hash) {
   auto b = createForceCalculator(ForceMethod::BARNES_HUT, c);
   CHECK(b->getMethod() == ForceMethod::BARNES_HUT);
   CHECK(dynamic_cast<BarnesHutCalculator*>(b.get())->getTheta() == 0.3f);
-  CHECK_THROWS(createForceCalculator(ForceMethod::SPATIAL_HASH, c), ValidationException);
+  c.spatial_hash_cell_size = 3.0f;
+  c.spatial_hash_cutoff = 1.5f;
+  auto sh = createForceCalculator(ForceMethod::SPATIAL_HASH, c);
+  CHECK(sh->getMethod() == ForceMethod::SPATIAL_HASH);
+  auto* shc = dynamic_cast<SpatialHashCalculator*>(sh.get());
+  CHECK(shc && shc->getCellSize() == 3.0f && shc->getCutoffRadius() == 1.5f);
+  CHECK(shc->getGrid() == nullptr);  // made on the first computeForces
   d->setSofteningParameter(0.2f);
   CHECK(d->getSofteningParameter() == 0.2f);
   CHECK(std::string(forceMethodToString(ForceMethod::BARNES_HUT)) == "barnes_hut");
+}
+
+// tests/test_spatial_hash.cpp:34-47 and hashCell's wrap of negative indices.
+TEST_CPU(spatial_hash_cell_index) {
+  const int3 a = SpatialHashGrid::getCellIndex(0.5f, 0.5f, 0.5f, 2.0f);
+  CHECK(a.x == 0 && a.y == 0 && a.z == 0);
+  const int3 b = SpatialHashGrid::getCellIndex(2.5f, 4.5f, 6.5f, 2.0f);
+  CHECK(b.x == 1 && b.y == 2 && b.z == 3);
+  const int3 c = SpatialHashGrid::getCellIndex(-0.5f, -2.0f, -2.5f, 2.0f);
+  CHECK(c.x == -1 && c.y == -1 && c.z == -2);
+  CHECK(SpatialHashGrid::hashCell(make_int3(-1, -1, -1), make_int3(3, 4, 5)) == 2 + 3 * 3 + 4 * 12);
+  CHECK(SpatialHashGrid::hashCell(make_int3(4, 9, 5), make_int3(3, 4, 5)) == 1 + 1 * 3 + 0);
 }
 
 TEST_CPU(benchmark_json_schema) {
@@ -444,6 +463,88 @@ TEST_GPU(barnes_hut_vs_direct) {
   CHECK(mid.first <= 1.1 * coarse.first);
 }
 
+// tests/test_spatial_hash.cpp:18-32, 49-73, 81-127, 133-167.
+TEST_GPU(spatial_hash_grid_kats) {
+  {
+    Scoped s(100);
+    UniformDistParams p;
+    p.min_bounds = Vec3(-10, -10, -10);
+    p.max_bounds = Vec3(10, 10, 10);
+    ParticleInitializer::initUniform(s.h, p);
+    s.up();
+    SpatialHashGrid grid(100, 2.0f);
+    grid.build(&s.d);
+    CHECK(grid.getTotalCells() > 0);
+    const int3 dims = grid.getGridDims();
+    CHECK(dims.x * dims.y * dims.z == grid.getTotalCells());
+    CHECK(grid.verifyCellAssignment(&s.h));
+    for (float cell : {0.6f, 1.3f, 2.0f, 4.7f, 9.9f}) {  // each particle in exactly one cell
+      SpatialHashGrid g(100, cell);
+      g.build(&s.d);
+      std::vector<int> cs, ce, pc, si;
+      g.copyCellDataToHost(cs, ce, pc, si);
+      std::vector<int> seen(100, 0);
+      for (int c = 0; c < g.getTotalCells(); ++c)
+        for (int k = cs[c]; k < ce[c]; ++k) seen[si[k]] += 1;
+      for (int v : seen) CHECK(v == 1);
+      CHECK(g.verifyCellAssignment(&s.h));
+    }
+  }
+  {
+    Scoped s(100);
+    UniformDistParams p;
+    p.min_bounds = Vec3(-5, -5, -5);
+    p.max_bounds = Vec3(5, 5, 5);
+    ParticleInitializer::initUniform(s.h, p);
+    s.up();
+    SpatialHashCalculator calc(2.0f, 2.0f);
+    calc.setGravitationalConstant(1.0f);
+    calc.setSofteningParameter(0.1f);
+    calc.computeForces(&s.d);
+    s.down();
+    for (size_t i = 0; i < 100; ++i)
+      CHECK(std::isfinite(s.h.acc_x[i]) && std::isfinite(s.h.acc_y[i]) &&
+            std::isfinite(s.h.acc_z[i]));
+    CHECK(calc.getGrid() != nullptr && calc.getGrid()->getCellSize() == 2.0f);
+  }
+  const float G = 1.0f, eps = 0.01f, cutoff = 2.0f, delta = 0.25f;
+  for (int inside = 0; inside < 2; ++inside) {
+    Scoped s(2);
+    const float r = inside ? cutoff - delta : cutoff + delta;
+    s.set(0, Vec3(0, 0, 0), Vec3(), 1.0f);
+    s.set(1, Vec3(r, 0, 0), Vec3(), 1.0f);
+    s.up();
+    SpatialHashCalculator calc(2.0f, cutoff);
+    calc.setGravitationalConstant(G);
+    calc.setSofteningParameter(eps);
+    calc.computeForces(&s.d);
+    s.down();
+    if (!inside) {
+      CHECK(s.h.acc_x[0] == 0.0f && s.h.acc_y[0] == 0.0f && s.h.acc_z[0] == 0.0f);
+      CHECK(s.h.acc_x[1] == 0.0f);
+    } else {
+      const float d2 = r * r + eps * eps;
+      const float expected = G * 1.0f * r / (d2 * std::sqrt(d2));
+      CHECK_NEAR(s.h.acc_x[0], expected, 1e-4f);
+      CHECK_NEAR(s.h.acc_x[1], -expected, 1e-4f);
+    }
+  }
+  {
+    Scoped s(2);
+    s.set(0, Vec3(-1e4f, 0, 0), Vec3(), 1.0f);
+    s.set(1, Vec3(1e4f, 1e4f, 1e4f), Vec3(), 1.0f);
+    s.up();
+    SpatialHashGrid grid(2, 0.01f);
+    bool refused = false;
+    try {
+      grid.build(&s.d);
+    } catch (const std::runtime_error& e) {
+      refused = std::string(e.what()).find("too large") != std::string::npos;
+    }
+    CHECK(refused);
+  }
+}
+
 TEST_GPU(particle_system_pause_and_methods) {
   ParticleSystem ps;
   SimulationConfig c;
@@ -461,7 +562,13 @@ TEST_GPU(particle_system_pause_and_methods) {
   ps.setBarnesHutTheta(0.4f);
   ps.update(0.01f);
   CHECK(ps.getForceMethod() == ForceMethod::BARNES_HUT);
-  CHECK_THROWS(ps.setForceMethod(ForceMethod::SPATIAL_HASH), ValidationException);
+  ps.setForceMethod(ForceMethod::SPATIAL_HASH);
+  CHECK_THROWS(ps.setSpatialHashCutoff(5.0f), ValidationException);    // > cell size 2
+  CHECK_THROWS(ps.setSpatialHashCellSize(1.0f), ValidationException);  // < cutoff 2
+  ps.setSpatialHashCellSize(4.0f);
+  ps.setSpatialHashCutoff(3.0f);
+  ps.update(0.01f);
+  CHECK(ps.getForceMethod() == ForceMethod::SPATIAL_HASH);
   SimulationState bad = ps.getState();
   bad.mass.pop_back();
   CHECK_THROWS(ps.setState(bad), ValidationException);

@@ -28,6 +28,8 @@ int main(int argc, char* argv[]) {
     config.G = opt.G;
     config.softening = opt.softening;
     config.barnes_hut_theta = opt.barnes_hut_theta;
+    config.spatial_hash_cell_size = opt.spatial_hash_cell_size;
+    config.spatial_hash_cutoff = opt.spatial_hash_cutoff;
     ParticleSystem system;
     system.initialize(config);
     if (!opt.import_path.empty()) {
@@ -59,8 +61,12 @@ int main(int argc, char* argv[]) {
     r.parameters["gravity"] = system.getGravitationalConstant();
     r.parameters["softening"] = system.getSofteningParameter();
     r.parameters["particle_count"] = (double)system.getParticleCount();
-    if (system.getForceMethod() == ForceMethod::BARNES_HUT)
+    if (system.getForceMethod() == ForceMethod::BARNES_HUT) {
       r.parameters["theta"] = config.barnes_hut_theta;
+    } else if (system.getForceMethod() == ForceMethod::SPATIAL_HASH) {
+      r.parameters["cell_size"] = config.spatial_hash_cell_size;
+      r.parameters["cutoff_radius"] = config.spatial_hash_cutoff;
+    }
     const std::vector<BenchmarkRunRecord> recs{r};
     std::cout << serializeBenchmarkRunRecords(recs) << std::endl;
     if (!opt.benchmark_output_path.empty()) writeBenchmarkRunRecords(opt.benchmark_output_path, recs);

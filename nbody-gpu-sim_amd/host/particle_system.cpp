@@ -111,8 +111,6 @@ void ParticleSystem::reset() {
 
 void ParticleSystem::setForceMethod(ForceMethod method) {
   if (force_method_ != method) {
-    if (method == ForceMethod::SPATIAL_HASH)
-      throw ValidationException("The spatial-hash force method is not built in this engine");
     force_method_ = method;
     config_.force_method = method;
     if (is_initialized_) createForceCalculator();
@@ -145,6 +143,29 @@ void ParticleSystem::setBarnesHutTheta(float theta) {
   config_.barnes_hut_theta = theta;
   if (force_method_ == ForceMethod::BARNES_HUT)
     if (auto* bh = dynamic_cast<BarnesHutCalculator*>(force_calculator_.get())) bh->setTheta(theta);
+}
+
+// :192-224 of the reference's particle_system.cpp.
+void ParticleSystem::setSpatialHashCellSize(float size) {
+  if (size <= 0 || std::isnan(size) || std::isinf(size))
+    throw ValidationException("Spatial hash cell size must be positive and finite");
+  if (size < config_.spatial_hash_cutoff)
+    throw ValidationException("Spatial hash cell size must not be smaller than the cutoff");
+  config_.spatial_hash_cell_size = size;
+  if (force_method_ == ForceMethod::SPATIAL_HASH)
+    if (auto* sh = dynamic_cast<SpatialHashCalculator*>(force_calculator_.get()))
+      sh->setCellSize(size);
+}
+
+void ParticleSystem::setSpatialHashCutoff(float cutoff) {
+  if (cutoff <= 0 || std::isnan(cutoff) || std::isinf(cutoff))
+    throw ValidationException("Spatial hash cutoff must be positive and finite");
+  if (cutoff > config_.spatial_hash_cell_size)
+    throw ValidationException("Spatial hash cutoff must not exceed the cell size");
+  config_.spatial_hash_cutoff = cutoff;
+  if (force_method_ == ForceMethod::SPATIAL_HASH)
+    if (auto* sh = dynamic_cast<SpatialHashCalculator*>(force_calculator_.get()))
+      sh->setCutoffRadius(cutoff);
 }
 
 void ParticleSystem::copyToHost(ParticleData& h) const {
@@ -190,8 +211,6 @@ void ParticleSystem::setState(const SimulationState& st) {
   c.softening = st.softening;
   c.force_method = st.force_method;
   validateSimulationConfig(c);
-  if (c.force_method == ForceMethod::SPATIAL_HASH)
-    throw ValidationException("The spatial-hash force method is not built in this engine");
   validateResourceRequirements(n);
   freeMemory();
   allocateMemory(n);

@@ -589,6 +589,14 @@ static void grid_coords(float px, float py, float pz, const float bbox[6], const
   c[2] = clamp_i32((int32_t)floorf((pz - bbox[2]) / cell), 0, dims[2] - 1);
 }
 
+/* r^2 of a pair as the reference writes it, dx*dx + dy*dy + dz*dz (:137), in the
+ * fused form an LLVM-based CUDA compiler gives it under its default contraction:
+ * fma(dz, dz, fma(dx, dx, dy*dy)). The spatial-hash kernel evaluates the same form,
+ * so both keep exactly the same pairs. */
+static float grid_d2(float rx, float ry, float rz) {
+  return fmaf(rz, rz, fmaf(rx, rx, ry * ry));
+}
+
 /* force_spatial_hash.cu:278-308 as a counting sort, stable in the particle index. */
 void orc_grid_build(size_t n, const float* x, const float* y, const float* z, float cell,
                     const float bbox[6], const int32_t dims[3], int32_t* particle_cell,
@@ -643,7 +651,7 @@ void orc_grid_forces_f32(size_t n, const float* x, const float* y, const float* 
             const int32_t j = sorted[k];
             if (j == i) continue;
             const float rx = x[j] - xi, ry = y[j] - yi, rz = z[j] - zi;
-            float d2 = rx * rx + ry * ry + rz * rz;
+            float d2 = grid_d2(rx, ry, rz);
             if (d2 < cutoff2) {
               d2 += eps2;
               const float inv = 1.0f / sqrtf(d2);
@@ -675,9 +683,7 @@ static void add_f64(double G, double eps2, float mj, double rx, double ry, doubl
 }
 
 static int keep_pair(float xi, float yi, float zi, float xj, float yj, float zj, float cutoff2) {
-  const float rx = xj - xi, ry = yj - yi, rz = zj - zi;
-  const float d2 = rx * rx + ry * ry + rz * rz;
-  return d2 < cutoff2;
+  return grid_d2(xj - xi, yj - yi, zj - zi) < cutoff2;
 }
 
 void orc_grid_forces_f64(size_t n, const float* x, const float* y, const float* z,

@@ -155,8 +155,9 @@ void orc_grid_build(size_t n, const float* x, const float* y, const float* z, fl
 /* spatialHashForceKernel (:84-158) in IEEE FP32 with 1.0f/sqrtf for rsqrtf:
  * target cell from its position with the clamp (:103-108), neighbour cells in
  * dz, dy, dx order, bodies in `sorted` order, j == i skipped, pair kept when
- * (dx*dx + dy*dy) + dz*dz < cutoff*cutoff (FP32, evaluated as written). targets
- * as in orc_direct_f32. */
+ * r^2 < cutoff*cutoff with r^2 = fmaf(dz, dz, fmaf(dx, dx, dy*dy)) in FP32 (the
+ * contracted form of the reference's dx*dx + dy*dy + dz*dz). targets as in
+ * orc_direct_f32. */
 void orc_grid_forces_f32(size_t n, const float* x, const float* y, const float* z,
                          const float* m, const float bbox[6], const int32_t dims[3], float cell,
                          const int32_t* cell_start, const int32_t* cell_end,
@@ -172,7 +173,7 @@ void orc_grid_forces_f64(size_t n, const float* x, const float* y, const float* 
                          const int32_t* sorted, float cutoff, double G, double eps2,
                          const int64_t* targets, size_t ntargets, double* ax, double* ay,
                          double* az, double* mag, int64_t* npairs, int nthreads);
-/* Brute force without a grid: every j != i whose FP32 (dx*dx + dy*dy) + dz*dz
+/* Brute force without a grid: every j != i whose FP32 r^2 (the same fused form)
  * is below cutoff*cutoff, summed in FP64. Equal to the grid sum whenever the
  * 27-cell search is complete (cell >= cutoff). */
 void orc_cutoff_forces_f64(size_t n, const float* x, const float* y, const float* z,
