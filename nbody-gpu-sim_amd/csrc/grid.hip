@@ -55,19 +55,35 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMergeSpan = 32;             // merged elements per thread in grid_ranges
 constexpr long long kMaxCells = 100000000;  // :257-260
-constexpr int kNB = 8;                      // bodies per scalar batch of the uniform walk
+constexpr int kNB = 8;                      // bodies per batch (and per culling box)
+constexpr int kSubBits = 6;                 // 4 x 4 x 4 sub-cells, Morton order
+constexpr long long kSubMaxCells = 1LL << (32 - kSubBits);  // larger grids: plain cell order
 
 struct GridGeom {
   float lo[3];  // padded lower corner
   float cell;
   int dims[3];
   int total;
+  int shift;  // sort key = cell << shift | sub-cell (shift 0: cell only)
 };
 
 // :36-49 and :103-108 (the force kernel repeats the arithmetic and the clamp).
 __device__ __forceinline__ int grid_axis(float p, float lo, float cell, int dim) {
   const int c = (int)floorf(div_rn(sub_rn(p, lo), cell));
   return max(0, min(c, dim - 1));
+}
+
+// Cell and 2-bit sub-cell coordinate on one axis: u = (p - lo) / cell as above,
+// sub = floor(4u) - 4c clamped to 0..3 (4u is exact).
+__device__ __forceinline__ int grid_axis_sub(float p, float lo, float cell, int dim, int& sub) {
+  const float u = div_rn(sub_rn(p, lo), cell);
+  const int c = max(0, min((int)floorf(u), dim - 1));
+  sub = max(0, min((int)floorf(u * 4.0f) - 4 * c, 3));
+  return c;
+}
+
+__device__ __forceinline__ uint32_t spread2(int v) {  // bits 1,0 -> 3,0
+  return (uint32_t)((v & 1) | ((v & 2) << 2));
 }
 
 __global__ __launch_bounds__(kThreads) void grid_bbox_partial(const float* __restrict__ x,
@@ -87,6 +103,10 @@ __global__ __launch_bounds__(kThreads) void grid_bbox_final(const float* partial
   }
 }
 
+// Cell of each particle (:36-49) and its sort key: the cell, then (shift 6) the
+// Morton index of its 4 x 4 x 4 sub-cell, so that a cell's bodies are ordered in
+// compact spatial groups. The reference's scatter leaves the order within a
+// cell open (:72-81); this order is one of its outcomes.
 __global__ __launch_bounds__(kThreads) void grid_assign(const float* __restrict__ x,
                                                         const float* __restrict__ y,
                                                         const float* __restrict__ z, int n,
@@ -97,23 +117,26 @@ __global__ __launch_bounds__(kThreads) void grid_assign(const float* __restrict_
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i == 0) *n_dev = n;
   if (i >= n) return;
-  const int cx = grid_axis(x[i], g.lo[0], g.cell, g.dims[0]);
-  const int cy = grid_axis(y[i], g.lo[1], g.cell, g.dims[1]);
-  const int cz = grid_axis(z[i], g.lo[2], g.cell, g.dims[2]);
+  int sx, sy, sz;
+  const int cx = grid_axis_sub(x[i], g.lo[0], g.cell, g.dims[0], sx);
+  const int cy = grid_axis_sub(y[i], g.lo[1], g.cell, g.dims[1], sy);
+  const int cz = grid_axis_sub(z[i], g.lo[2], g.cell, g.dims[2], sz);
   const int c = cx + cy * g.dims[0] + cz * g.dims[0] * g.dims[1];
+  const uint32_t sub = (spread2(sx) << 2) | (spread2(sy) << 1) | spread2(sz);
   pcell[i] = c;
-  keys[i] = (uint32_t)c;
+  keys[i] = g.shift ? ((uint32_t)c << g.shift) | sub : (uint32_t)c;
   idx[i] = i;
 }
 
-// Merge of the cell numbers 0..total (total is a sentinel) with the n sorted
-// keys, cells placed before keys of equal value. Cell c then sits after
-// exactly start[c] = #{keys < c} keys, and key k sits at merged position
-// k + keys[k] + 1 (strictly increasing in k). A thread finds its first key by
+// Merge of the cell numbers 0..total (total is a sentinel) with the cells of the
+// n sorted keys (key >> shift), cells placed before keys of equal cell. Cell c
+// then sits after exactly start[c] = #{keys of cell < c} keys, and key k sits at
+// merged position k + cell_k + 1 (strictly increasing in k). A thread finds its first key by
 // binary search on that position, then walks kMergeSpan merged elements,
 // writing start[c] and end[c-1] (= start[c]) for every cell it passes.
 __global__ __launch_bounds__(kThreads) void grid_ranges(const uint32_t* __restrict__ keys, int n,
-                                                        int total, int32_t* __restrict__ start,
+                                                        int total, int shift,
+                                                        int32_t* __restrict__ start,
                                                         int32_t* __restrict__ end) {
   const long long m = (long long)total + 1 + n;
   const long long q0 = ((long long)blockIdx.x * kThreads + threadIdx.x) * kMergeSpan;
@@ -121,7 +144,7 @@ __global__ __launch_bounds__(kThreads) void grid_ranges(const uint32_t* __restri
   int lo = 0, hi = n;  // first key whose merged position is >= q0
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if ((long long)mid + keys[mid] + 1 < q0)
+    if ((long long)mid + (keys[mid] >> shift) + 1 < q0)
       lo = mid + 1;
     else
       hi = mid;
@@ -130,7 +153,7 @@ __global__ __launch_bounds__(kThreads) void grid_ranges(const uint32_t* __restri
   long long c = q0 - j;  // cells before q0
   const long long q1 = q0 + kMergeSpan < m ? q0 + kMergeSpan : m;
   for (long long q = q0; q < q1; ++q) {
-    if (j < n && (long long)j + keys[j] + 1 == q) {
+    if (j < n && (long long)j + (keys[j] >> shift) + 1 == q) {
       ++j;
       continue;
     }
@@ -140,17 +163,41 @@ __global__ __launch_bounds__(kThreads) void grid_ranges(const uint32_t* __restri
   }
 }
 
-// Bodies in sorted order with G*m (:144 forms G * mass[j] first).
+// Bodies in sorted order with G*m (:144 forms G * mass[j] first), padded with
+// zeros to a multiple of kNB, and the bounding box of every aligned group of kNB
+// bodies (box[2g] = min, box[2g+1] = max over the group's real bodies). The
+// grid covers the padded length; the groups' lanes sit in one wave.
 __global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restrict__ idx,
                                                         const float* __restrict__ x,
                                                         const float* __restrict__ y,
                                                         const float* __restrict__ z,
                                                         const float* __restrict__ m, int n,
-                                                        float G, float4* __restrict__ body) {
+                                                        int n_pad, float G,
+                                                        float4* __restrict__ body,
+                                                        float4* __restrict__ box) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
-  if (k >= n) return;
-  const int i = idx[k];
-  body[k] = make_float4(x[i], y[i], z[i], mul_rn(G, m[i]));
+  float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (k < n) {
+    const int i = idx[k];
+    b = make_float4(x[i], y[i], z[i], mul_rn(G, m[i]));
+    lo[0] = hi[0] = b.x;
+    lo[1] = hi[1] = b.y;
+    lo[2] = hi[2] = b.z;
+  }
+#pragma unroll
+  for (int off = 1; off < kNB; off <<= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
+      hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
+    }
+  if (k >= n_pad) return;
+  body[k] = b;
+  if ((k & (kNB - 1)) == 0) {
+    box[2 * (k / kNB)] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    box[2 * (k / kNB) + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+  }
 }
 
 // r^2 of one pair as fma(rz, rz, fma(rx, rx, ry*ry)): the fused form an LLVM-based
@@ -241,33 +288,61 @@ __device__ __forceinline__ void grid_pair(float4 b, bool keep, float xi, float y
   fz = __builtin_fmaf(f, rz, fz);
 }
 
-// Walk of [ulo, uhi) with wave-uniform indices (scalar loads, SGPR operands),
-// batches of kNB with the next batch's loads in flight. SELF: the range may hold
-// a lane's own body. MEMBER: lanes keep only bodies of their own [own_lo, own_hi).
-template <bool SELF, bool MEMBER>
-__device__ __forceinline__ void walk_uniform(const float4* __restrict__ body, int ulo, int uhi,
-                                             int own_lo, int own_hi, int self, float xi,
-                                             float yi, float zi, float cutoff2, float eps2,
-                                             float& fx, float& fy, float& fz) {
-  int j = ulo;
-  if (j + kNB <= uhi) {
+// Walk of [ulo, uhi) with wave-uniform indices (scalar loads, SGPR operands) over
+// the aligned groups of kNB bodies that can hold a kept pair. Each lane tests one
+// group's box against the box of the wave's targets; a group whose per-axis gaps
+// (sx, sy, sz) give fma(sz, sz, fma(sx, sx, sy*sy)) >= cutoff2 is skipped. No pair
+// of a skipped group can be kept: every pair difference is at least the gap on
+// each axis, the gaps are rounded from the same coordinates by the same monotone
+// operations, and r^2 is the same monotone expression. The groups left are
+// evaluated in ascending order with the next one's loads in flight; a skipped
+// group's pairs would only have added exact zeros, so the sums are those of a
+// walk over every body.
+// SELF: the range may hold a lane's own body. multi: the wave's targets lie in
+// more than one cell, so each lane keeps only its own [own_lo, own_hi); edge
+// groups keep [ulo, uhi) the same way.
+template <bool SELF>
+__device__ __forceinline__ void walk_culled(const float4* __restrict__ body,
+                                            const float4* __restrict__ box, int ulo, int uhi,
+                                            bool multi, int own_lo, int own_hi, int self,
+                                            const float (&tlo)[3], const float (&thi)[3],
+                                            float xi, float yi, float zi, float cutoff2,
+                                            float eps2, float& fx, float& fy, float& fz) {
+  if (ulo >= uhi) return;
+  const int lane = (int)(threadIdx.x & 63u);
+  const int g_first = ulo / kNB, g_last = (uhi - 1) / kNB;
+  const int mlo = multi ? own_lo : ulo, mhi = multi ? own_hi : uhi;
+  for (int gt = g_first; gt <= g_last; gt += 64) {
+    const int gl = gt + lane;
+    bool pass = false;
+    if (gl <= g_last) {
+      const float4 lo = box[2 * gl], hi = box[2 * gl + 1];
+      const float sx = fmaxf(fmaxf(sub_rn(lo.x, thi[0]), sub_rn(tlo[0], hi.x)), 0.0f);
+      const float sy = fmaxf(fmaxf(sub_rn(lo.y, thi[1]), sub_rn(tlo[1], hi.y)), 0.0f);
+      const float sz = fmaxf(fmaxf(sub_rn(lo.z, thi[2]), sub_rn(tlo[2], hi.z)), 0.0f);
+      pass = grid_d2(sx, sy, sz) < cutoff2;
+    }
+    uint64_t todo = __builtin_amdgcn_ballot_w64(pass);
+    if (!todo) continue;
+    int gc = gt + (int)__builtin_ctzll(todo);
+    todo &= todo - 1;
     float4 cur[kNB], nxt[kNB];
-    load_bodies<kNB>(body, j, cur);
+    load_bodies<kNB>(body, gc * kNB, cur);
     for (;;) {
-      const int jn = j + kNB;
-      const bool more = jn + kNB <= uhi;
-      grid_batch<SELF, MEMBER>(cur, j, body, more ? jn : j, nxt, own_lo, own_hi, self, xi, yi,
-                               zi, cutoff2, eps2, fx, fy, fz);
-      j = jn;
+      const bool more = todo != 0;
+      const int gn = more ? gt + (int)__builtin_ctzll(todo) : gc;
+      const int j = gc * kNB;
+      if (multi || j < ulo || j + kNB > uhi)
+        grid_batch<SELF, true>(cur, j, body, gn * kNB, nxt, mlo, mhi, self, xi, yi, zi, cutoff2,
+                               eps2, fx, fy, fz);
+      else
+        grid_batch<SELF, false>(cur, j, body, gn * kNB, nxt, 0, 0, self, xi, yi, zi, cutoff2,
+                                eps2, fx, fy, fz);
       if (!more) break;
+      todo &= todo - 1;
+      gc = gn;
       for (int q = 0; q < kNB; ++q) cur[q] = nxt[q];
     }
-  }
-  for (; j < uhi; ++j) {
-    bool keep = true;
-    if (SELF) keep &= j != self;
-    if (MEMBER) keep &= (j >= own_lo) & (j < own_hi);
-    grid_pair(body[j], keep, xi, yi, zi, cutoff2, eps2, fx, fy, fz);
   }
 }
 
@@ -287,6 +362,14 @@ __device__ __forceinline__ int wave_max(int v) {
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
   return __builtin_amdgcn_readfirstlane(v);
 }
+__device__ __forceinline__ float wave_fmin(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float wave_fmax(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
 
 // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each
 // XCD takes one contiguous run of sorted targets, whose neighbour cells then
@@ -299,6 +382,7 @@ __device__ __forceinline__ int xcd_block(int i, int nb) {
 
 // mode 0: uniform walk where a wave's targets share a row; 1: per-lane walks only.
 __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict__ body,
+                                                        const float4* __restrict__ box,
                                                         const int32_t* __restrict__ idx,
                                                         const int32_t* __restrict__ cstart,
                                                         const int32_t* __restrict__ cend,
@@ -325,6 +409,9 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
     const int cy0 = __builtin_amdgcn_readfirstlane(cy), cz0 = __builtin_amdgcn_readfirstlane(cz);
     const int xmin = wave_min(cx), xmax = wave_max(cx);
     const int ulo_x = max(xmin - 1, 0), uhi_x = min(xmax + 1, g.dims[0] - 1);
+    const bool multi = xmin != xmax;
+    const float tlo[3] = {wave_fmin(me.x), wave_fmin(me.y), wave_fmin(me.z)};
+    const float thi[3] = {wave_fmax(me.x), wave_fmax(me.y), wave_fmax(me.z)};
     for (int dz = -1; dz <= 1; ++dz) {
       const int nz = cz0 + dz;
       if (nz < 0 || nz >= g.dims[2]) continue;
@@ -333,25 +420,15 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
         if (ny < 0 || ny >= g.dims[1]) continue;
         const int base = ny * g.dims[0] + nz * dxy;
         const int ulo = cstart[base + ulo_x], uhi = cend[base + uhi_x];
+        const int olo = multi ? cstart[base + lo_x] : 0, ohi = multi ? cend[base + hi_x] : 0;
         // the self test is needed only where a lane's own body lies in the range
         // (the centre row, unless positions moved since the build)
-        const bool self_in = __any((k >= ulo) & (k < uhi));
-        if (xmin == xmax) {
-          if (self_in)
-            walk_uniform<true, false>(body, ulo, uhi, 0, 0, k, me.x, me.y, me.z, cutoff2, eps2,
-                                      fx, fy, fz);
-          else
-            walk_uniform<false, false>(body, ulo, uhi, 0, 0, k, me.x, me.y, me.z, cutoff2, eps2,
-                                       fx, fy, fz);
-        } else {
-          const int olo = cstart[base + lo_x], ohi = cend[base + hi_x];
-          if (self_in)
-            walk_uniform<true, true>(body, ulo, uhi, olo, ohi, k, me.x, me.y, me.z, cutoff2,
-                                     eps2, fx, fy, fz);
-          else
-            walk_uniform<false, true>(body, ulo, uhi, olo, ohi, k, me.x, me.y, me.z, cutoff2,
-                                      eps2, fx, fy, fz);
-        }
+        if (__any((k >= ulo) & (k < uhi)))
+          walk_culled<true>(body, box, ulo, uhi, multi, olo, ohi, k, tlo, thi, me.x, me.y, me.z,
+                            cutoff2, eps2, fx, fy, fz);
+        else
+          walk_culled<false>(body, box, ulo, uhi, multi, olo, ohi, k, tlo, thi, me.x, me.y,
+                             me.z, cutoff2, eps2, fx, fy, fz);
       }
     }
   } else {
@@ -397,7 +474,8 @@ struct nbx_grid {
   uint32_t* keys = nullptr;
   int32_t* idx = nullptr;
   int32_t* n_dev = nullptr;
-  float4* body = nullptr;
+  float4* body = nullptr;  // sorted bodies, padded to a multiple of kNB
+  float4* group_box = nullptr;  // per group of kNB sorted bodies: min, max
   int32_t* cstart = nullptr;
   int32_t* cend = nullptr;
   size_t cell_cap = 0;
@@ -470,7 +548,9 @@ int nbx_grid_create(size_t max_particles, float cell_size, nbx_grid** out) {
   A(&g->keys, P);
   A(&g->idx, P);
   A(&g->n_dev, 1);
-  A(&g->body, P);
+  const size_t P8 = (P + kNB - 1) / kNB * kNB;
+  A(&g->body, P8);
+  A(&g->group_box, 2 * (P8 / kNB));
   if (rc == NBX_OK && hipHostMalloc((void**)&g->box_host, 6 * sizeof(float)) != hipSuccess) {
     (void)hipGetLastError();
     g->box_host = nullptr;
@@ -539,6 +619,7 @@ int nbx_grid_build(nbx_grid* g, const nbx_particles* d, nbx_stream stream) {
   }
   geom.cell = g->cell;
   geom.total = (int)total;
+  geom.shift = total <= kSubMaxCells ? kSubBits : 0;
   if ((size_t)total > g->cell_cap) {  // grow-only cell arrays
     if (g->cstart) (void)hipFree(g->cstart);
     if (g->cend) (void)hipFree(g->cend);
@@ -556,11 +637,12 @@ int nbx_grid_build(nbx_grid* g, const nbx_particles* d, nbx_stream stream) {
   hipLaunchKernelGGL(grid_assign, dim3(grid_blocks(n, INT32_MAX)), dim3(kThreads), 0, s, d->pos_x,
                      d->pos_y, d->pos_z, n, geom, g->pcell, g->keys, g->idx, g->n_dev);
   NBX_LAUNCHED();
-  NBX_TRY(g->sort.sort(g->keys, g->idx, g->n_dev, bits_for(geom.total), s));
+  NBX_TRY(g->sort.sort(g->keys, g->idx, g->n_dev, bits_for(geom.total) + geom.shift, s));
   const long long merged = total + 1 + n;
   const long long threads = (merged + kMergeSpan - 1) / kMergeSpan;
   hipLaunchKernelGGL(grid_ranges, dim3((unsigned)((threads + kThreads - 1) / kThreads)),
-                     dim3(kThreads), 0, s, g->keys, n, geom.total, g->cstart, g->cend);
+                     dim3(kThreads), 0, s, g->keys, n, geom.total, geom.shift, g->cstart,
+                     g->cend);
   NBX_LAUNCHED();
   g->n = n;
   return NBX_OK;
@@ -578,11 +660,13 @@ int nbx_grid_forces(nbx_grid* g, nbx_particles* d, float cutoff, float G, float 
   const int n = g->n;
   const float cutoff2 = cutoff * cutoff;  // :320
   const float eps2 = eps * eps;
-  hipLaunchKernelGGL(grid_gather, dim3(grid_blocks(n, INT32_MAX)), dim3(kThreads), 0, s, g->idx,
-                     d->pos_x, d->pos_y, d->pos_z, d->mass, n, G, g->body);
+  const int n_pad = (n + kNB - 1) / kNB * kNB;
+  hipLaunchKernelGGL(grid_gather, dim3(grid_blocks(n_pad, INT32_MAX)), dim3(kThreads), 0, s,
+                     g->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, n, n_pad, G, g->body,
+                     g->group_box);
   hipLaunchKernelGGL(grid_forces, dim3(grid_blocks(n, INT32_MAX)), dim3(kThreads), 0, s, g->body,
-                     g->idx, g->cstart, g->cend, g->geom, cutoff2, eps2, n, g->mode, d->acc_x,
-                     d->acc_y, d->acc_z);
+                     g->group_box, g->idx, g->cstart, g->cend, g->geom, cutoff2, eps2, n, g->mode,
+                     d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   return NBX_OK;
 }

@@ -597,29 +597,52 @@ static float grid_d2(float rx, float ry, float rz) {
   return fmaf(rz, rz, fmaf(rx, rx, ry * ry));
 }
 
-/* force_spatial_hash.cu:278-308 as a counting sort, stable in the particle index. */
+/* 2-bit sub-cell coordinate of a position in its (clamped) cell c on one axis:
+ * floor(4u) - 4c clamped to 0..3 with u = (p - lo) / cell as for the cell. */
+static int32_t grid_sub(float p, float lo, float cell, int32_t c) {
+  const float u = (p - lo) / cell;
+  const int32_t s = (int32_t)floorf(u * 4.0f) - 4 * c;
+  return s < 0 ? 0 : (s > 3 ? 3 : s);
+}
+
+static uint32_t spread2(int32_t v) { return (uint32_t)((v & 1) | ((v & 2) << 2)); }
+
+/* force_spatial_hash.cu:278-308. Cell ranges: the exclusive scan of the counts.
+ * Order within a cell (open in the reference, whose scatter is atomic): the Morton
+ * index of the position's 4 x 4 x 4 sub-cell, then the particle index; grids of
+ * more than 2^26 cells keep the particle index alone. */
 void orc_grid_build(size_t n, const float* x, const float* y, const float* z, float cell,
                     const float bbox[6], const int32_t dims[3], int32_t* particle_cell,
                     int32_t* cell_start, int32_t* cell_end, int32_t* sorted) {
   const size_t total = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
-  int32_t* fill = (int32_t*)calloc(total ? total : 1, sizeof(int32_t));
+  const int with_sub = total <= ((size_t)1 << 26);
+  int32_t* count = (int32_t*)calloc(total ? total : 1, sizeof(int32_t));
+  uint32_t* key = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
   for (size_t i = 0; i < n; ++i) {
     int32_t c[3];
     grid_coords(x[i], y[i], z[i], bbox, dims, cell, c);
     const int32_t id = c[0] + c[1] * dims[0] + c[2] * dims[0] * dims[1];
     particle_cell[i] = id;
-    fill[id] += 1;
+    count[id] += 1;
+    uint32_t k = (uint32_t)id;
+    if (with_sub) {
+      const uint32_t sub = (spread2(grid_sub(x[i], bbox[0], cell, c[0])) << 2) |
+                           (spread2(grid_sub(y[i], bbox[1], cell, c[1])) << 1) |
+                           spread2(grid_sub(z[i], bbox[2], cell, c[2]));
+      k = (k << 6) | sub;
+    }
+    key[i] = k;
+    sorted[i] = (int32_t)i;
   }
   int32_t run = 0;
   for (size_t c = 0; c < total; ++c) {
-    const int32_t cnt = fill[c];
     cell_start[c] = run;
-    cell_end[c] = run + cnt;
-    fill[c] = run;
-    run += cnt;
+    run += count[c];
+    cell_end[c] = run;
   }
-  for (size_t i = 0; i < n; ++i) sorted[fill[particle_cell[i]]++] = (int32_t)i;
-  free(fill);
+  orc_stable_sort(n, key, sorted);
+  free(key);
+  free(count);
 }
 
 void orc_grid_forces_f32(size_t n, const float* x, const float* y, const float* z,

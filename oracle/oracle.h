@@ -29,7 +29,8 @@
  *   orc_grid_cell_index     KAT of test_spatial_hash.cpp:34-47
  *   orc_grid_hash_cell      Python modulo on every offset of a small box
  *   orc_grid_dims           numpy float32 restatement of the sizing formulas
- *   orc_grid_build          numpy bincount/cumsum and argsort(kind="stable");
+ *   orc_grid_build          numpy bincount/cumsum and a numpy stable argsort of the
+ *                           (cell, sub-cell) key;
  *                           one-cell-per-particle property (test_spatial_hash.cpp:133-167);
  *                           every particle inside its cell's bounds (verifyCellAssignment)
  *   orc_grid_forces_f32/f64 brute-force cutoff sum without a grid (same pair set,
@@ -146,9 +147,11 @@ int orc_grid_dims(size_t n, const float* x, const float* y, const float* z, floa
                   float bbox[6], int32_t dims[3], int64_t* total);
 /* Cell contents of SpatialHashGrid::build (:278-308): particle_cell (n) by
  * :36-49, cell_start / cell_end (total) as the exclusive scan of the counts and
- * start + count (:291-300), sorted (n) the indices grouped by cell; within a
- * cell ascending (the reference's atomic scatter, :72-81, leaves that order
- * open). */
+ * start + count (:291-300), sorted (n) the indices grouped by cell. Within a cell
+ * (an order the reference's atomic scatter, :72-81, leaves open): ascending Morton
+ * index of the 4 x 4 x 4 sub-cell, sub = floor(4u) - 4c per axis with
+ * u = (p - min) / cell, then ascending particle index; grids above 2^26 cells
+ * order by particle index alone. */
 void orc_grid_build(size_t n, const float* x, const float* y, const float* z, float cell,
                     const float bbox[6], const int32_t dims[3], int32_t* particle_cell,
                     int32_t* cell_start, int32_t* cell_end, int32_t* sorted);

@@ -5,9 +5,10 @@ src/cuda/force_spatial_hash.cu:
   * getCellIndex            known answers of tests/test_spatial_hash.cpp:34-47
   * hashCell                Python's modulo over every offset of a small box
   * grid sizing             a numpy float32 restatement of :232-260
-  * build                   numpy bincount/cumsum and a stable argsort; the
-                            one-cell-per-particle property of :133-167; every
-                            particle inside its cell (verifyCellAssignment :342-368)
+  * build                   numpy bincount/cumsum and a stable argsort of the (cell,
+                            sub-cell) key; the one-cell-per-particle property of
+                            :133-167; every particle inside its cell
+                            (verifyCellAssignment :342-368) and inside its sub-cell
   * forces                  a brute-force cutoff sum without any grid (same pair set,
                             FP64), the cutoff-boundary known answers of :81-127, and the
                             FP64 direct sum once the cutoff exceeds every distance
@@ -37,6 +38,23 @@ def numpy_cells(x, y, z, lo, dims, cell):
     c = [np.clip(np.floor((p - lo[a]) / f(cell)).astype(np.int64), 0, dims[a] - 1)
          for a, p in enumerate((x, y, z))]
     return c[0] + c[1] * dims[0] + c[2] * dims[0] * dims[1], c
+
+
+def numpy_sort_key(x, y, z, lo, dims, cell):
+    """Order within a cell: Morton index of the 4x4x4 sub-cell (floor(4u) - 4c per axis,
+    u = (p - min) / cell in float32), for grids of at most 2^26 cells."""
+    f = np.float32
+    pc, c = numpy_cells(x, y, z, lo, dims, cell)
+    if int(np.prod(dims)) > 1 << 26:
+        return pc, None
+    s = []
+    for a, p in enumerate((x, y, z)):
+        u = (p - lo[a]) / f(cell)
+        assert u.dtype == np.float32
+        s.append(np.clip(np.floor(u * f(4)).astype(np.int64) - 4 * c[a], 0, 3))
+    spread = lambda v: (v & 1) | ((v & 2) << 2)
+    sub = (spread(s[0]) << 2) | (spread(s[1]) << 1) | spread(s[2])
+    return pc * 64 + sub, s
 
 
 def test_cell_index_known_answers(orc):
@@ -87,7 +105,8 @@ def test_build_matches_numpy(orc):
         start = np.concatenate([[0], np.cumsum(counts)[:-1]])
         np.testing.assert_array_equal(g["cell_start"], start)
         np.testing.assert_array_equal(g["cell_end"], start + counts)
-        np.testing.assert_array_equal(g["sorted"], np.argsort(pc, kind="stable"))
+        key, _ = numpy_sort_key(x, y, z, lo, g["dims"].astype(np.int64), cell)
+        np.testing.assert_array_equal(g["sorted"], np.argsort(key, kind="stable"))
 
 
 def test_each_particle_in_exactly_one_cell(orc):
@@ -103,15 +122,23 @@ def test_each_particle_in_exactly_one_cell(orc):
 
 
 def test_every_particle_inside_its_cell(orc):
-    # verifyCellAssignment (:342-368) in float32.
+    # verifyCellAssignment (:342-368) in float32, and the same for the sub-cell.
     f = np.float32
     x, y, z = cube(2000, 7.0, 7)
     for cell in (0.6, 1.5, 2.0):
         g = orc.grid_build(x, y, z, cell)
-        _, coords = numpy_cells(x, y, z, g["bbox"][:3], g["dims"].astype(np.int64), cell)
+        dims = g["dims"].astype(np.int64)
+        _, coords = numpy_cells(x, y, z, g["bbox"][:3], dims, cell)
+        _, subs = numpy_sort_key(x, y, z, g["bbox"][:3], dims, cell)
         for a, p in enumerate((x, y, z)):
             cmin = g["bbox"][a] + coords[a].astype(f) * f(cell)
             assert np.all(p >= cmin) and np.all(p <= cmin + f(cell))
+            q = f(cell) / f(4)
+            smin = cmin + subs[a].astype(f) * q
+            assert np.all(p >= smin - 1e-5 * f(cell)) and np.all(p <= smin + q + 1e-5 * f(cell))
+        # bodies of one cell are listed in ascending sub-cell order
+        key = np.asarray(numpy_sort_key(x, y, z, g["bbox"][:3], dims, cell)[0])
+        assert np.all(np.diff(key[g["sorted"]]) >= 0)
 
 
 def _cases(nbx):
