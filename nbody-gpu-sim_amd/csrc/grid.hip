@@ -9,10 +9,12 @@
 //           in FP32 (:250-252) and refuses grids above 1e8 cells (:254-260).
 //   assign  cell of each particle: (int)floorf((p - min) / cell) per axis,
 //           clamped to the grid, linear cx + cy*dx + cz*dx*dy (:36-49).
-//   sort    stable radix sort of (cell, particle index), so sorted_indices
-//           ascend within each cell. The reference's atomic scatter (:72-81)
-//           leaves that order to the scheduler; ascending is one of its
-//           outcomes and makes every later sum deterministic.
+//   sort    stable radix sort of (key, particle index), key = cell << 6 | the
+//           Morton index of the particle's 4 x 4 x 4 sub-cell (grids above 2^26
+//           cells: the cell alone). Within a cell the order is therefore
+//           spatially compact groups, then index. The reference's atomic
+//           scatter (:72-81) leaves that order to the scheduler; this is one of
+//           its outcomes, and it makes every later sum deterministic.
 //   ranges  cell_start / cell_end of every cell, the exclusive scan of the
 //           per-cell counts (:287-300), from one merge of the sorted keys with
 //           the cell numbers: each thread takes a fixed slice of the merged
@@ -20,26 +22,28 @@
 //           the same per element, and no count array or memset is needed.
 //
 // Forces (spatialHashForceKernel, :84-158). For target i in cell c, each j != i
-// in the 27 cells around c (clamped at the grid edge) with
-// (dx*dx + dy*dy) + dz*dz < cutoff^2 adds G m_j r_ij (r^2 + eps^2)^{-3/2};
-// cells in dz, dy, dx order, bodies in sorted order. The three dx cells of one
-// (dy, dz) row are adjacent in the linear numbering, so they are one
-// contiguous range of the sorted bodies.
+// in the 27 cells around c (clamped at the grid edge) with r^2 < cutoff^2
+// adds G m_j r_ij (r^2 + eps^2)^{-3/2}; cells in dz, dy, dx order, bodies in
+// sorted order. The three dx cells of one (dy, dz) row are adjacent in the
+// linear numbering, so they are one contiguous range of the sorted bodies.
 //   * bodies are gathered once per call into sorted order as float4
-//     (x, y, z, G*m); G*m is the reference's first product (:144).
+//     (x, y, z, G*m), G*m being the reference's first product (:144), with
+//     the bounding box of every aligned group of 8.
 //   * a wave takes 64 consecutive sorted targets. When they all lie in one
 //     (cy, cz) row of cells (the common case once cells hold dozens of
 //     bodies), the wave walks the union of the lanes' ranges with
-//     wave-uniform addresses: the bodies arrive through scalar loads and
-//     reach every lane as SGPR operands, and a lane drops the bodies outside
-//     its own three cells with a mask. Otherwise every lane walks its own
-//     ranges with vector loads.
+//     wave-uniform addresses, skipping the groups whose box is provably
+//     beyond the cutoff from the box of its targets (walk_culled): the bodies
+//     arrive through scalar loads and reach every lane as SGPR operands, and a
+//     lane drops the bodies outside its own three cells with a mask.
+//     Otherwise every lane walks its own ranges with vector loads.
 //   * r^2 is fma(rz, rz, fma(rx, rx, ry*ry)) on both sides (here and in the
 //     oracle), so the set of pairs kept by the cutoff test is exactly the
 //     oracle's; the kept pairs use v_rsq_f32 and FMA accumulation
 //     (tolerance-level differences).
-//   * both walks visit a target's bodies in the same order with the same
-//     arithmetic, so a target's result does not depend on which one ran.
+//   * both walks add a target's nonzero terms in the same order with the same
+//     arithmetic (skipped or masked bodies add exact zeros), so a target's
+//     result does not depend on which one ran.
 #include <algorithm>
 #include <cmath>
 #include <new>
