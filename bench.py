@@ -26,9 +26,14 @@ force.spatial_hash workload (1,048,576 bodies uniform in a sphere of radius 10,
 unit masses, cell = cutoff = 2, eps = 0.1), ms per computeForces call (grid
 build + forces) and the roofline of the force kernel.
 
---only {direct,barnes-hut,ensemble,precision,spatial-hash} runs one part (for
-profiling); precision compares plain FP32 with the double-single/Kahan form at
-N=65,536.
+cpu_path_n4096 is the reference's CPU-runnable case (4,096-body Plummer, a(0) +
+100 steps) timed on the GPU and on the oracle's restatement of the reference CPU
+path, with the final positions compared. Every CPU baseline is reported on all
+host threads and on one.
+
+--only {direct,barnes-hut,ensemble,precision,spatial-hash,cpu-path} runs one part
+(for profiling); precision compares plain FP32 with the double-single/Kahan form
+at N=65,536.
 """
 import argparse
 import itertools
@@ -62,6 +67,19 @@ def oracle():
 
 def eps2_of(eps):
     return float(np.float32(np.float32(eps) * np.float32(eps)))
+
+
+def host_info(orc) -> dict:
+    """CPU of the host the baselines ran on."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "threads_used": orc.max_threads()}
 
 
 def traffic_from_profile():
@@ -152,12 +170,22 @@ def run_direct(nbx, args):
         orc.direct_f32(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], G, eps, targets=t,
                        nthreads=threads)
         sec = time.perf_counter() - t0
+        t1 = t[:args.cpu1_targets]
+        t0 = time.perf_counter()
+        orc.direct_f32(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], G, eps, targets=t1,
+                       nthreads=1)
+        sec1 = time.perf_counter() - t0
         cpu = {"value": args.cpu_targets * (n - 1) / sec / 1e9, "unit": "Ginteractions/s",
                "cores": threads, "kind": "port",
                "sample": f"{args.cpu_targets} of {n} targets x all {n} sources: one FP32 force "
                          f"evaluation with the reference CPU pair function summed in j order "
-                         f"(oracle), {sec:.2f} s wall on {threads} threads"}
-        log(f"direct cpu baseline: {cpu['value']:.3f} Ginteractions/s")
+                         f"(oracle), {sec:.2f} s wall on {threads} threads",
+               "single_thread": {"value": len(t1) * (n - 1) / sec1 / 1e9,
+                                 "unit": "Ginteractions/s",
+                                 "sample": f"{len(t1)} targets, {sec1:.2f} s on 1 thread"},
+               "host": host_info(orc)}
+        log(f"direct cpu baseline: {cpu['value']:.3f} Ginteractions/s on {threads} threads, "
+            f"{cpu['single_thread']['value']:.3f} on 1")
     names = nbx.variant_names()
     v = args.variant if args.variant >= 0 else int(nbx.lib.nbx_direct_variant_for(n))
     sym = names[v].startswith("sym_")
@@ -182,6 +210,57 @@ def run_direct(nbx, args):
         "force_error_fp64": err,
         "cpu_baseline": cpu,
     }
+
+
+# ------------------------------------------------------------------ CPU-path config
+
+def run_cpu_path(nbx, args):
+    """The reference's CPU-runnable case: direct N^2 FP32 on a 4,096-body Plummer sphere
+    (G = M = a = 1, m = 1/N, seed 42), eps 0.01, dt 1e-3, a(0) + 100 Verlet steps, on the
+    GPU and on the oracle's restatement of the reference CPU path (all threads and 1)."""
+    n, G, eps, dt, steps = 4096, 1.0, 0.01, 1e-3, 100
+    e2 = eps2_of(eps)
+    h = nbx.gen_plummer(n, seed=42)
+    d = nbx.DeviceParticles.from_host(h)
+    s = nbx.Stream()
+    e0, e1 = nbx.Event(), nbx.Event()
+    e0.record(s)
+    nbx.check(nbx.lib.nbx_direct_forces(d.ptr, G, e2, 256, s.h))
+    for _ in range(steps):
+        nbx.check(nbx.lib.nbx_direct_step(d.ptr, G, e2, dt, s.h))
+    e1.record(s)
+    s.synchronize()
+    gpu_ms = e0.elapsed_ms(e1)
+    inter = (steps + 1) * float(n) * (n - 1)
+    got = d.download()
+    out = {"n": n, "steps": steps, "eps": eps, "dt": dt,
+           "workload": "Plummer a=1, M=1, m=1/N, seed 42; a(0) + 100 Velocity Verlet steps",
+           "gpu_ms": gpu_ms, "gpu_Ginteractions_per_s": inter / (gpu_ms * 1e-3) / 1e9}
+    orc = oracle()
+    ref = {k: v.copy() for k, v in h.items()}
+    threads = orc.max_threads()
+    t0 = time.perf_counter()
+    orc.run_direct(ref, G, eps, dt, steps, nthreads=threads)
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+    one = {k: v.copy() for k, v in h.items()}
+    t0 = time.perf_counter()
+    orc.run_direct(one, G, eps, dt, steps, nthreads=1)
+    cpu1_ms = (time.perf_counter() - t0) * 1e3
+    pos_diff = max(float(np.abs(got[k] - ref[k]).max()) for k in ("pos_x", "pos_y", "pos_z"))
+    eg, er = orc.energy_f64(got, G, e2), orc.energy_f64(ref, G, e2)
+    e_gpu, e_ref = eg["ke"] + eg["pe"], er["ke"] + er["pe"]
+    out.update({"cpu_baseline": {"value": cpu_ms, "unit": "ms/run", "cores": threads,
+                                 "kind": "port",
+                                 "sample": "the full run with the oracle's restatement of the "
+                                           "reference CPU path (pair function summed in j order, "
+                                           "Verlet as integrator.cu)",
+                                 "single_thread": {"value": cpu1_ms, "unit": "ms/run"},
+                                 "host": host_info(orc)},
+                "max_position_diff_vs_cpu": pos_diff,
+                "energy_rel_diff_vs_cpu": abs(e_gpu - e_ref) / abs(e_ref)})
+    log(f"cpu path N={n}: GPU {gpu_ms:.2f} ms, CPU {cpu_ms:.0f} ms on {threads} threads, "
+        f"{cpu1_ms:.0f} ms on 1; max |dx| {pos_diff:.2e}")
+    return out
 
 
 # ------------------------------------------------------------------ Barnes-Hut
@@ -244,13 +323,22 @@ def run_bh(nbx, args):
         orc.bh_forces(tree["nodes"], x, y, z, m, theta, G, eps, targets=tg, nthreads=threads)
         trav_s = time.perf_counter() - t0
         est = (build_s + trav_s * n / sample) * 1e3
+        tg1 = tg[:args.bh_cpu1_targets]
+        t0 = time.perf_counter()
+        orc.bh_forces(tree["nodes"], x, y, z, m, theta, G, eps, targets=tg1, nthreads=1)
+        trav1_s = time.perf_counter() - t0
+        est1 = (build_s + trav1_s * n / len(tg1)) * 1e3
         out["cpu_baseline"] = {
             "value": est, "unit": "ms/step", "cores": threads, "kind": "port",
             "sample": f"oracle build (bbox, Morton, stable sort, single-thread insertion + COM, "
                       f"as the reference host build) {build_s * 1e3:.0f} ms + traversal of "
                       f"{sample} of {n} targets on {threads} threads {trav_s * 1e3:.0f} ms, "
-                      f"scaled to all targets"}
-        log(f"barnes-hut cpu baseline: {est:.1f} ms/step")
+                      f"scaled to all targets",
+            "single_thread": {"value": est1, "unit": "ms/step",
+                              "sample": f"same build + traversal of {len(tg1)} targets on 1 "
+                                        f"thread {trav1_s * 1e3:.0f} ms, scaled"},
+            "host": host_info(orc)}
+        log(f"barnes-hut cpu baseline: {est:.1f} ms/step on {threads} threads, {est1:.1f} on 1")
     return out
 
 
@@ -344,12 +432,21 @@ def run_spatial_hash(nbx, args):
         orc.grid_forces_f32(gb, x, y, z, m, cutoff, G, eps, targets=st, nthreads=threads)
         f_s = time.perf_counter() - t0
         est = (build_s + f_s * n / sample) * 1e3
+        st1 = st[:args.sh_cpu1_targets]
+        t0 = time.perf_counter()
+        orc.grid_forces_f32(gb, x, y, z, m, cutoff, G, eps, targets=st1, nthreads=1)
+        f1_s = time.perf_counter() - t0
+        est1 = (build_s + f1_s * n / len(st1)) * 1e3
         out["cpu_baseline"] = {
             "value": est, "unit": "ms/call", "cores": threads, "kind": "port",
             "sample": f"oracle grid build (single thread) {build_s * 1e3:.0f} ms + FP32 forces of "
                       f"{sample} of {n} targets on {threads} threads {f_s * 1e3:.0f} ms, scaled "
-                      f"to all targets"}
-        log(f"spatial hash cpu baseline: {est:.1f} ms/call")
+                      f"to all targets",
+            "single_thread": {"value": est1, "unit": "ms/call",
+                              "sample": f"same build + forces of {len(st1)} targets on 1 thread "
+                                        f"{f1_s * 1e3:.0f} ms, scaled"},
+            "host": host_info(orc)}
+        log(f"spatial hash cpu baseline: {est:.1f} ms/call on {threads} threads, {est1:.1f} on 1")
     return out
 
 
@@ -413,6 +510,40 @@ def run_ensemble(nbx, args):
                        "eps=0.01, dt=1e-3"}
     log(f"ensemble: batched {batched:.3f} ms/step, sequential {sequential:.3f} ms/step, "
         f"speedup {out['speedup']:.2f}, tile {names[v]}")
+    if not args.no_cpu_baseline:
+        # one ensemble step on the oracle's restatement of the reference CPU path: the force
+        # evaluation of 8 systems spread over the size range (all targets), scaled by the
+        # step's total interaction count
+        orc = oracle()
+        threads = orc.max_threads()
+        pos = {k: h[k] for k in ("pos_x", "pos_y", "pos_z", "mass")}
+        offs = np.concatenate([[0], np.cumsum(counts)])
+
+        def timed(systems, nthreads):
+            t0 = time.perf_counter()
+            done = 0
+            for q in systems:
+                sl = slice(int(offs[q]), int(offs[q + 1]))
+                orc.direct_f32(pos["pos_x"][sl], pos["pos_y"][sl], pos["pos_z"][sl],
+                               pos["mass"][sl], G, eps, nthreads=nthreads)
+                done += int(counts[q]) * (int(counts[q]) - 1)
+            return done / (time.perf_counter() - t0)
+
+        order = np.argsort(counts)
+        sample = order[np.linspace(0, len(counts) - 1, 8).astype(int)]
+        rate = timed(sample, threads)
+        rate1 = timed(order[np.linspace(0, len(counts) // 2, 4).astype(int)], 1)
+        out["cpu_baseline"] = {
+            "value": pairs / rate * 1e3, "unit": "ms/step", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 forces of 8 systems spread over the size range on {threads} "
+                      f"threads ({rate / 1e9:.2f} Ginteractions/s), scaled to the step's "
+                      f"{pairs:.3e} interactions",
+            "single_thread": {"value": pairs / rate1 * 1e3, "unit": "ms/step",
+                              "sample": f"4 systems on 1 thread ({rate1 / 1e9:.3f} "
+                                        f"Ginteractions/s), scaled"},
+            "host": host_info(orc)}
+        log(f"ensemble cpu baseline: {out['cpu_baseline']['value']:.0f} ms/step on {threads} "
+            f"threads, {out['cpu_baseline']['single_thread']['value']:.0f} on 1")
     return out
 
 
@@ -484,9 +615,34 @@ def run_precision(nbx, args):
                      "force_err_end": force_err(a1, p1), "drift_series": drift}
         log(f"precision {name}: {res[name]['ms_per_step']:.3f} ms/step, final drift "
             f"{res[name]['rel_energy_drift_final']:.3e}, max {res[name]['rel_energy_drift_max']:.3e}")
-    return {"n": n, "steps": K, "dt": dt, "eps": eps,
-            "workload": "Plummer sphere a=1, M=1, FP32 pair forces; double_single = hi/lo "
-                        "positions, Kahan sums, FP64 renormalised drift", **res}
+    out = {"n": n, "steps": K, "dt": dt, "eps": eps,
+           "workload": "Plummer sphere a=1, M=1, FP32 pair forces; double_single = hi/lo "
+                       "positions, Kahan sums, FP64 renormalised drift", **res}
+    if not args.no_cpu_baseline:
+        # a step of the reference CPU path (the plain FP32 form; the reference has no
+        # precision form) is one force evaluation: timed on a target sample and scaled
+        threads = orc.max_threads()
+        ts = np.sort(np.random.default_rng(8).choice(n, 4096, replace=False))
+        t0 = time.perf_counter()
+        orc.direct_f32(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], G, eps, targets=ts,
+                       nthreads=threads)
+        sec = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        orc.direct_f32(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], G, eps, targets=ts[:256],
+                       nthreads=1)
+        sec1 = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": sec * n / len(ts) * 1e3, "unit": "ms/step", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle FP32 force evaluation of 4096 of {n} targets on {threads} threads "
+                      f"{sec * 1e3:.0f} ms, scaled (the Verlet update is O(N) and left out)",
+            "single_thread": {"value": sec1 * n / 256 * 1e3, "unit": "ms/step",
+                              "sample": f"256 targets on 1 thread {sec1 * 1e3:.0f} ms, scaled"},
+            "host": host_info(orc)}
+        log(f"precision cpu baseline (plain FP32 path): {out['cpu_baseline']['value']:.0f} "
+            f"ms/step on {threads} threads, {out['cpu_baseline']['single_thread']['value']:.0f} "
+            f"on 1")
+    return out
 
 
 def main() -> int:
@@ -497,7 +653,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--only", choices=["direct", "barnes-hut", "ensemble", "precision",
-                                       "spatial-hash"])
+                                       "spatial-hash", "cpu-path"])
     ap.add_argument("--variant", type=int, default=-1, help="force a direct tile variant")
     ap.add_argument("--check-targets", type=int, default=4096,
                     help="sample targets of the FP64 force check")
@@ -505,11 +661,13 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="one 256 MiB device copy first (reference launch for counter scaling)")
     ap.add_argument("--cpu-targets", type=int, default=16384)
+    ap.add_argument("--cpu1-targets", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--bh-n", type=int, default=1 << 20)
     ap.add_argument("--bh-steps", type=int, default=5)
     ap.add_argument("--bh-cpu-targets", type=int, default=65536)
+    ap.add_argument("--bh-cpu1-targets", type=int, default=8192)
     ap.add_argument("--ens-steps", type=int, default=100)
     ap.add_argument("--ens-chunk", type=int, default=100)
     ap.add_argument("--ens-seq-steps", type=int, default=5)
@@ -517,6 +675,7 @@ def main() -> int:
     ap.add_argument("--sh-steps", type=int, default=10)
     ap.add_argument("--sh-check-targets", type=int, default=2048)
     ap.add_argument("--sh-cpu-targets", type=int, default=8192)
+    ap.add_argument("--sh-cpu1-targets", type=int, default=1024)
     ap.add_argument("--prec-n", type=int, default=65536)
     ap.add_argument("--prec-steps", type=int, default=10000)
     args = ap.parse_args()
@@ -535,6 +694,10 @@ def main() -> int:
         return 0
     if args.only == "barnes-hut":
         print(json.dumps({"barnes_hut": run_bh(nbx, args), "device": info["name"]}), flush=True)
+        return 0
+    if args.only == "cpu-path":
+        print(json.dumps({"cpu_path": run_cpu_path(nbx, args), "device": info["name"]}),
+              flush=True)
         return 0
     if args.only == "spatial-hash":
         print(json.dumps({"spatial_hash": run_spatial_hash(nbx, args), "device": info["name"]}),
@@ -568,6 +731,8 @@ def main() -> int:
         line["barnes_hut"] = run_bh(nbx, args)
         line["ensemble"] = run_ensemble(nbx, args)
         line["spatial_hash"] = run_spatial_hash(nbx, args)
+        if not args.no_cpu_baseline:
+            line["cpu_path_n4096"] = run_cpu_path(nbx, args)
     print(json.dumps(line), flush=True)
     return 0
 
