@@ -615,8 +615,31 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
 // leaf flag in its sign); aux[p] holds the position after the cell's subtree
 // and the opening-test size (2h)^2 = 4 h h, rounded as the traversal's test
 // (:218-224) rounds it.
-__device__ __forceinline__ float4 trav_box(float4 b, bool leaf) {
-  return make_float4(b.x, b.y, b.z, leaf ? -b.w - 0.0f : b.w);
+// Traversal record of the node at preorder position p:
+//   rec[2p]   = (centre of mass, mass)
+//   rec[2p+1] = (cell centre, half size; negative for a leaf)
+//   aux[p]    = (p + subtree size, (2h)^2 as float bits)
+// A leaf's (2h)^2 is written as -1: size2 < d2 * theta2 then holds for every
+// theta >= 0, so the theta test never opens it, and the walk needs the leaf flag
+// only where a lane's own particle lies in the cell. A node of zero mass, which
+// the reference skips with its whole subtree (:172-173), gets its centre of mass
+// and cell centre at kFarPos, a zero half size and the same -1: no particle lies
+// in it, it is never opened, and its force term is an exact zero (G * 0 times a
+// finite inverse cube).
+constexpr float kFarPos = 1e18f;
+constexpr float kNeverOpen = -1.0f;
+__device__ __forceinline__ void write_trav(float4* rec, int2* aux, int p, int size, float4 com,
+                                           float4 box, bool leaf) {
+  if (com.w == 0.0f) {
+    rec[2 * p] = make_float4(kFarPos, kFarPos, kFarPos, 0.0f);
+    rec[2 * p + 1] = make_float4(kFarPos, kFarPos, kFarPos, leaf ? -0.0f : 0.0f);
+    aux[p] = make_int2(p + size, __float_as_int(kNeverOpen));
+    return;
+  }
+  rec[2 * p] = com;
+  rec[2 * p + 1] = make_float4(box.x, box.y, box.z, leaf ? -box.w - 0.0f : box.w);
+  aux[p] = make_int2(p + size, __float_as_int(leaf ? kNeverOpen
+                                                   : mul_rn(mul_rn(4.0f, box.w), box.w)));
 }
 
 __global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict__ st, TmpNodes T,
@@ -626,10 +649,7 @@ __global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict_
   for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
     const int pt = d == 0 ? 0 : T.pre[t];
     const bool leaf = (T.info[t] >> 11) & 1u;
-    rec[2 * pt] = T.com[t];
-    rec[2 * pt + 1] = trav_box(T.box[t], leaf);
-    aux[pt] = make_int2(pt + T.size[t],
-                        __float_as_int(mul_rn(mul_rn(4.0f, T.box[t].w), T.box[t].w)));
+    write_trav(rec, aux, pt, T.size[t], T.com[t], T.box[t], leaf);
     if (leaf) continue;
     int next = pt + 1;
     for (int o = 7; o >= 0; --o) {
@@ -848,9 +868,8 @@ __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __re
       for (int o = 0; o < 8; ++o)
         if (q.child[o] >= 0) sz += size[q.child[o]];
     size[v] = sz;
-    rec[2 * p] = make_float4(q.comx, q.comy, q.comz, q.mass);
-    rec[2 * p + 1] = make_float4(q.cx, q.cy, q.cz, q.leaf ? -q.half - 0.0f : q.half);
-    aux[p] = make_int2(p + sz, __float_as_int(mul_rn(mul_rn(4.0f, q.half), q.half)));
+    write_trav(rec, aux, p, sz, make_float4(q.comx, q.comy, q.comz, q.mass),
+               make_float4(q.cx, q.cy, q.cz, q.half), q.leaf);
   }
 }
 
@@ -910,16 +929,22 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
 #pragma unroll
   for (int q = 0; q < PPL; ++q) keep_lo[q] = keep_hi[q] = 0;
   int depth = 0;
+  int top = INT_MAX;  // position of the innermost remembered entry (INT_MAX: none)
   int v = 0;
-  int budget = nn + 1;
   const float theta2_hi = theta2 * (1.0f + 0x1p-20f), theta2_lo = theta2 * (1.0f - 0x1p-20f);
-  while (v < nn && budget-- > 0) {
-    while (depth > 0 && __builtin_amdgcn_readlane(keep_at, depth - 1) == v) {
+  // Every step moves v forward (to v + 1, or to the skip index, clamped to at
+  // least v + 1), so the walk ends after at most nn steps. The loop is bound by
+  // the issue of its scalar instructions (lane masks, control), so the common
+  // step - no lane's particle inside the cell - tests neither the leaf flag nor
+  // the mass (both are folded into the records, see write_trav).
+  while (v < nn) {
+    while (v == top) {  // leaving the subtrees remembered for position v
       --depth;
 #pragma unroll
       for (int q = 0; q < PPL; ++q)
         M[q] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)keep_hi[q], depth) << 32) |
                (uint32_t)__builtin_amdgcn_readlane((int)keep_lo[q], depth);
+      top = depth > 0 ? __builtin_amdgcn_readlane(keep_at, depth - 1) : INT_MAX;
     }
     // One 32-byte record plus (skip, (2h)^2), waited for together.
     const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 32u;
@@ -927,15 +952,10 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
     const float4 b4 = *reinterpret_cast<const float4*>(rp + 16);
     const int2 ax2 =
         *reinterpret_cast<const int2*>(reinterpret_cast<const char*>(aux) + (uint32_t)v * 8u);
-    const int after = ax2.x;
+    const int after = max(ax2.x, v + 1);
     const float size2 = __int_as_float(ax2.y);
     asm volatile("" ::"s"(c4.x), "s"(c4.y), "s"(c4.z), "s"(c4.w), "s"(b4.x), "s"(b4.y),
                  "s"(b4.z), "s"(b4.w), "s"(after), "s"(size2));
-    if (c4.w == 0.0f) {  // :172-173
-      v = after;
-      continue;
-    }
-    const bool leaf = __builtin_signbit(b4.w);
     const float h = fabsf(b4.w);
     uint64_t om[PPL];
     uint64_t any = 0;
@@ -946,39 +966,44 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
       const float far =
           fmaxf(fmaxf(fabsf(xi[q] - b4.x), fabsf(yi[q] - b4.y)), fabsf(zi[q] - b4.z));
       const uint64_t cm = __ballot(far <= h) & M[q];  // :180-182, particle inside the cell
-      float meff = c4.w, cx = c4.x, cy = c4.y, cz = c4.z;
+      float dx = sub_rn(c4.x, xi[q]), dy = sub_rn(c4.y, yi[q]), dz = sub_rn(c4.z, zi[q]);
+      float gm = mul_rn(G, c4.w);
+      uint64_t o = 0;     // lanes that open the node
       uint64_t gone = 0;  // lanes whose own leaf holds no other mass
-      if (leaf && cm) {   // :190-200, the particle's own leaf
-        if (cm & me) {
-          meff = sub_rn(c4.w, mi[q]);
-          if (meff > 0.0f) {
-            cx = div_rn(sub_rn(mul_rn(c4.x, c4.w), mul_rn(xi[q], mi[q])), meff);
-            cy = div_rn(sub_rn(mul_rn(c4.y, c4.w), mul_rn(yi[q], mi[q])), meff);
-            cz = div_rn(sub_rn(mul_rn(c4.z, c4.w), mul_rn(zi[q], mi[q])), meff);
+      if (cm) {
+        if (__builtin_signbit(b4.w)) {  // :190-200, the particle's own leaf
+          const bool own = (cm & me) != 0;
+          const float meff = own ? sub_rn(c4.w, mi[q]) : c4.w;
+          const bool fix = own && meff > 0.0f;
+          if (__ballot(fix)) {  // remove the particle from a leaf that holds others
+            const float cx = div_rn(sub_rn(mul_rn(c4.x, c4.w), mul_rn(xi[q], mi[q])), meff);
+            const float cy = div_rn(sub_rn(mul_rn(c4.y, c4.w), mul_rn(yi[q], mi[q])), meff);
+            const float cz = div_rn(sub_rn(mul_rn(c4.z, c4.w), mul_rn(zi[q], mi[q])), meff);
+            dx = fix ? sub_rn(cx, xi[q]) : dx;
+            dy = fix ? sub_rn(cy, yi[q]) : dy;
+            dz = fix ? sub_rn(cz, zi[q]) : dz;
           }
+          gm = own ? mul_rn(G, meff) : gm;
+          gone = __ballot(meff <= 0.0f) & cm;
+        } else {
+          o = cm;  // :222, an internal node holding the particle is always opened
         }
-        gone = __ballot(meff <= 0.0f) & cm;
       }
-      const float dx = sub_rn(cx, xi[q]), dy = sub_rn(cy, yi[q]), dz = sub_rn(cz, zi[q]);
       const float d2 =
           add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
-      om[q] = 0;
-      if (!leaf) {  // :218-224, open when contains || !(size2 / d2 < theta2)
-        // size2 against d2 * theta2 (1 -+ 2^-20): each product is within half
-        // an ulp, so outside that band the rounded quotient size2 / d2 is
-        // certainly below or above theta2; inside it the IEEE division decides.
-        const uint64_t hm = __ballot(size2 > d2 * theta2_hi);
-        const uint64_t lm = __ballot(size2 < d2 * theta2_lo);
-        om[q] = (cm | hm) & M[q];
-        const uint64_t am = M[q] & ~(cm | hm | lm);
-        if (am) {
-          const bool mine = (am & me) != 0;
-          om[q] |= __ballot(mine && !(div_rn(size2, d2) < theta2)) & am;
-        }
+      // :218-224, open when contains || !(size2 / d2 < theta2): size2 against
+      // d2 * theta2 (1 -+ 2^-20) decides outside a narrow band (each product is
+      // within half an ulp); inside the band the IEEE division decides.
+      o |= __ballot(size2 > d2 * theta2_hi);
+      const uint64_t am = M[q] & ~(o | __ballot(size2 < d2 * theta2_lo));
+      if (am) {
+        const bool mine = (am & me) != 0;
+        o |= __ballot(mine && !(div_rn(size2, d2) < theta2)) & am;
       }
-      const uint64_t im = M[q] & ~om[q] & ~gone;  // lanes that take the node's force
+      om[q] = o & M[q];
+      const uint64_t im = M[q] & ~o & ~gone;  // lanes that take the node's force
       const float inv = __builtin_amdgcn_rsqf(d2);
-      const float fi = mul_rn(G, meff) * (inv * inv * inv);
+      const float fi = gm * (inv * inv * inv);
       float f;  // fi on the lanes of im, else 0: one select on the lane mask
       asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(f) : "v"(fi), "s"(im));
       sx[q] = __builtin_fmaf(f, dx, sx[q]);
@@ -986,7 +1011,7 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
       sz[q] = __builtin_fmaf(f, dz, sz[q]);
       any |= om[q];
     }
-    if (any) {  // :234-240
+    if (any) {  // :234-240, descend; remember where this subtree ends
       const bool mine = lane == depth;
       keep_at = mine ? after : keep_at;
 #pragma unroll
@@ -996,6 +1021,7 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
         M[q] = om[q];
       }
       ++depth;
+      top = after;
       v = v + 1;
     } else {
       v = after;
