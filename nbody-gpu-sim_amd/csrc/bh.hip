@@ -611,14 +611,12 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
 // Traversal layout, one level per launch from the root down: cells in the
 // order the reference's stack walk visits them (:160-241: children pushed
 // 0..7, so popped 7..0), i.e. depth-first preorder with children 7 down to 0.
-// Record p holds (centre of mass, mass) and (cell centre, half size with the
-// leaf flag in its sign); aux[p] holds the position after the cell's subtree
-// and the opening-test size (2h)^2 = 4 h h, rounded as the traversal's test
-// (:218-224) rounds it.
-// Traversal record of the node at preorder position p:
-//   rec[2p]   = (centre of mass, mass)
-//   rec[2p+1] = (cell centre, half size; negative for a leaf)
-//   aux[p]    = (p + subtree size, (2h)^2 as float bits)
+// Record p (64 bytes, so that one scalar address reaches all of it):
+//   rec[4p]   = (centre of mass, mass)
+//   rec[4p+1] = (cell centre, half size; negative for a leaf)
+//   rec[4p+2] = (position after the cell's subtree as int bits, the opening-test
+//                size (2h)^2 = 4 h h rounded as the traversal's test (:218-224)
+//                rounds it, 0, 0); rec[4p+3] is padding.
 // A leaf's (2h)^2 is written as -1: size2 < d2 * theta2 then holds for every
 // theta >= 0, so the theta test never opens it, and the walk needs the leaf flag
 // only where a lane's own particle lies in the cell. A node of zero mass, which
@@ -628,28 +626,28 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
 // finite inverse cube).
 constexpr float kFarPos = 1e18f;
 constexpr float kNeverOpen = -1.0f;
-__device__ __forceinline__ void write_trav(float4* rec, int2* aux, int p, int size, float4 com,
-                                           float4 box, bool leaf) {
+__device__ __forceinline__ void write_trav(float4* rec, int p, int size, float4 com, float4 box,
+                                           bool leaf) {
   if (com.w == 0.0f) {
-    rec[2 * p] = make_float4(kFarPos, kFarPos, kFarPos, 0.0f);
-    rec[2 * p + 1] = make_float4(kFarPos, kFarPos, kFarPos, leaf ? -0.0f : 0.0f);
-    aux[p] = make_int2(p + size, __float_as_int(kNeverOpen));
+    rec[4 * p] = make_float4(kFarPos, kFarPos, kFarPos, 0.0f);
+    rec[4 * p + 1] = make_float4(kFarPos, kFarPos, kFarPos, leaf ? -0.0f : 0.0f);
+    rec[4 * p + 2] = make_float4(__int_as_float(p + size), kNeverOpen, 0.0f, 0.0f);
     return;
   }
-  rec[2 * p] = com;
-  rec[2 * p + 1] = make_float4(box.x, box.y, box.z, leaf ? -box.w - 0.0f : box.w);
-  aux[p] = make_int2(p + size, __float_as_int(leaf ? kNeverOpen
-                                                   : mul_rn(mul_rn(4.0f, box.w), box.w)));
+  rec[4 * p] = com;
+  rec[4 * p + 1] = make_float4(box.x, box.y, box.z, leaf ? -box.w - 0.0f : box.w);
+  rec[4 * p + 2] = make_float4(__int_as_float(p + size),
+                               leaf ? kNeverOpen : mul_rn(mul_rn(4.0f, box.w), box.w), 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict__ st, TmpNodes T,
-                                                      int d, float4* rec, int2* aux) {
+                                                      int d, float4* rec) {
   if (st->use_seq) return;
   const int b = st->level_start[d], e = st->level_start[d + 1];
   for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
     const int pt = d == 0 ? 0 : T.pre[t];
     const bool leaf = (T.info[t] >> 11) & 1u;
-    write_trav(rec, aux, pt, T.size[t], T.com[t], T.box[t], leaf);
+    write_trav(rec, pt, T.size[t], T.com[t], T.box[t], leaf);
     if (leaf) continue;
     int next = pt + 1;
     for (int o = 7; o >= 0; --o) {
@@ -842,8 +840,7 @@ __global__ void seq_build(BhState* st, const int32_t* __restrict__ sidx, const f
 // visiting order, then sizes the subtrees in reverse order (a node's children
 // always come after it). pre_of, size and order are scratch of max_nodes ints.
 __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __restrict__ nodes,
-                             int32_t* pre_of, int32_t* size, int32_t* order, float4* rec,
-                             int2* aux) {
+                             int32_t* pre_of, int32_t* size, int32_t* order, float4* rec) {
   if (!st->use_seq || threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nn = st->final_count;
   if (nn <= 0) return;
@@ -868,7 +865,7 @@ __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __re
       for (int o = 0; o < 8; ++o)
         if (q.child[o] >= 0) sz += size[q.child[o]];
     size[v] = sz;
-    write_trav(rec, aux, p, sz, make_float4(q.comx, q.comy, q.comz, q.mass),
+    write_trav(rec, p, sz, make_float4(q.comx, q.comy, q.comz, q.mass),
                make_float4(q.cx, q.cy, q.cz, q.half), q.leaf);
   }
 }
@@ -897,7 +894,7 @@ constexpr int kTravPPL = NBX_BH_PPL;
 
 template <int PPL>
 __global__ __launch_bounds__(kThreads) void bh_traverse(
-    const float4* __restrict__ rec, const int2* __restrict__ aux,
+    const float4* __restrict__ rec,
     const BhState* __restrict__ st, const int32_t* __restrict__ order,
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     const float* __restrict__ m, int n, float theta2, float G, float eps2, float* ax, float* ay,
@@ -947,13 +944,12 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
       top = depth > 0 ? __builtin_amdgcn_readlane(keep_at, depth - 1) : INT_MAX;
     }
     // One 32-byte record plus (skip, (2h)^2), waited for together.
-    const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 32u;
+    const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 64u;
     const float4 c4 = *reinterpret_cast<const float4*>(rp);
     const float4 b4 = *reinterpret_cast<const float4*>(rp + 16);
-    const int2 ax2 =
-        *reinterpret_cast<const int2*>(reinterpret_cast<const char*>(aux) + (uint32_t)v * 8u);
-    const int after = max(ax2.x, v + 1);
-    const float size2 = __int_as_float(ax2.y);
+    const float2 a2 = *reinterpret_cast<const float2*>(rp + 32);
+    const int after = max(__float_as_int(a2.x), v + 1);
+    const float size2 = a2.y;
     asm volatile("" ::"s"(c4.x), "s"(c4.y), "s"(c4.z), "s"(c4.w), "s"(b4.x), "s"(b4.y),
                  "s"(b4.z), "s"(b4.w), "s"(after), "s"(size2));
     const float h = fabsf(b4.w);
@@ -1057,8 +1053,7 @@ struct nbx_bh {
   int32_t* nvals = nullptr;
   int32_t* final_of = nullptr;
   nbx::RefNode* nodes = nullptr;
-  float4* trav_rec = nullptr;  // 2 records per cell, traversal order
-  int2* trav_aux = nullptr;  // per record: skip position, (2h)^2
+  float4* trav_rec = nullptr;  // 64 bytes per cell, traversal order
   nbx::RadixSort sort_particles;
   nbx::RadixSort sort_nodes;
   bool built = false;
@@ -1169,8 +1164,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->nodes, NN);
   A(&t->T.size, NN);
   A(&t->T.pre, NN);
-  A(&t->trav_rec, NN * 2);
-  A(&t->trav_aux, NN);
+  A(&t->trav_rec, NN * 4);
   if (rc == NBX_OK) rc = t->sort_particles.init((int)P);
   if (rc == NBX_OK) rc = t->sort_nodes.init((int)NN);
   for (auto& e : t->ev)
@@ -1254,10 +1248,9 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   hipLaunchKernelGGL(seq_build, dim3(1), dim3(64), 0, s, t->st, t->idx, d->pos_x, d->pos_y,
                      d->pos_z, d->mass, t->nodes, t->max_nodes);
   for (int lv = 0; lv <= kMaxDepth; ++lv)
-    hipLaunchKernelGGL(pre_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv, t->trav_rec,
-                       t->trav_aux);
+    hipLaunchKernelGGL(pre_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv, t->trav_rec);
   hipLaunchKernelGGL(seq_preorder, dim3(1), dim3(64), 0, s, t->st, t->nodes, t->T.pre, t->T.size,
-                     t->T.k, t->trav_rec, t->trav_aux);
+                     t->T.k, t->trav_rec);
   NBX_LAUNCHED();
   mark(t, 7, s);
   t->built = true;
@@ -1274,7 +1267,7 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   const int n = (int)d->count;
   mark(t, 8, s);
   hipLaunchKernelGGL(bh_traverse<kTravPPL>, dim3(div_up(n, kThreads * kTravPPL)), dim3(kThreads),
-                     0, s, t->trav_rec, t->trav_aux, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z,
+                     0, s, t->trav_rec, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z,
                      d->mass, n, theta * theta, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   mark(t, 9, s);
