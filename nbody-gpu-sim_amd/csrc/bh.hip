@@ -179,16 +179,21 @@ __global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restric
 
 // ---------------------------------------------------------------- 4: paths
 
+// Also copies each particle to its sorted position as (x, y, z, m), so that the
+// level passes read a leaf's particle from near its neighbours' (ranks of one
+// cell are close) instead of from a random original index.
 __global__ __launch_bounds__(kThreads) void path_kernel(const float* __restrict__ x,
                                                         const float* __restrict__ y,
                                                         const float* __restrict__ z,
+                                                        const float* __restrict__ m,
                                                         const int32_t* __restrict__ sidx, int n,
                                                         const BhState* __restrict__ st,
-                                                        uint4* path) {
+                                                        uint4* path, float4* sbody) {
   const int r = blockIdx.x * kThreads + threadIdx.x;
   if (r >= n) return;
   const int p = sidx[r];
   const float px = x[p], py = y[p], pz = z[p];
+  sbody[r] = make_float4(px, py, pz, m[p]);
   float cx = st->root_c[0], cy = st->root_c[1], cz = st->root_c[2], h = st->root_h;
   uint32_t w[4] = {0, 0, 0, 0};
   for (int d = 0; d < kMaxDepth; ++d) {
@@ -346,94 +351,85 @@ __global__ __launch_bounds__(kThreads) void level_count(BhState* st, Levels L,
 }
 
 // Level d, pass 2: create the children of every segment (cells at depth d+1).
+// One thread per (segment, octant) slot: creates the child cell of that octant
+// when it holds particles, and the next level's segment when it holds two or
+// more (below depth 32). Cell ids come from a block scan plus one atomic per
+// block, so every level's cells are contiguous (level_start).
 __global__ __launch_bounds__(kThreads) void level_children(BhState* st, Levels L, TmpNodes T,
                                                            const int32_t* __restrict__ sidx,
-                                                           const float* __restrict__ x,
-                                                           const float* __restrict__ y,
-                                                           const float* __restrict__ z,
-                                                           const float* __restrict__ m, int d,
-                                                           int max_nodes) {
+                                                           const float4* __restrict__ sbody,
+                                                           int d, int max_nodes) {
   const int cur = d & 1, nxt = cur ^ 1;
   const int ns = st->overflow ? 0 : st->seg_count[cur];
   const int cd = d + 1;  // child depth
+  const int nslots = ns * 8;
   __shared__ int sh_node_base, sh_seg_base, sh_deep;
-  for (int base = blockIdx.x * kThreads; base < ns; base += gridDim.x * kThreads) {
-    const int s = base + threadIdx.x;
-    const bool valid = s < ns;
-    int cnt[8];
-    int nchild = 0, nseg = 0;
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      cnt[o] = valid ? L.seg_cnt[cur][s * 8 + o] : 0;
-      nchild += cnt[o] > 0;
-      nseg += (cnt[o] >= 2 && cd < kMaxDepth);
-    }
+  for (int base = blockIdx.x * kThreads; base < nslots; base += gridDim.x * kThreads) {
+    const int u = base + threadIdx.x;  // slot: segment u / 8, octant u % 8
+    const bool valid = u < nslots;
+    const int c = valid ? L.seg_cnt[cur][u] : 0;
     int tot_child, tot_seg;
-    const int off_child = block_exclusive_scan(nchild, &tot_child);
-    const int off_seg = block_exclusive_scan(nseg, &tot_seg);
+    const int off_child = block_exclusive_scan(c > 0 ? 1 : 0, &tot_child);
+    const int off_seg = block_exclusive_scan((c >= 2 && cd < kMaxDepth) ? 1 : 0, &tot_seg);
     if (threadIdx.x == 0) {
-      sh_node_base = atomicAdd(&st->node_count, tot_child);
-      sh_seg_base = atomicAdd(&st->seg_count[nxt], tot_seg);
+      sh_node_base = tot_child ? atomicAdd(&st->node_count, tot_child) : 0;
+      sh_seg_base = tot_seg ? atomicAdd(&st->seg_count[nxt], tot_seg) : 0;
       sh_deep = 0;
     }
     __syncthreads();
-    if (valid && nchild) {
-      const int X = L.seg_node[cur][s];
-      const int t2 = L.seg_t2[cur][s];
-      const float4 pb = T.box[X];
-      const float hs = mul_rn(pb.w, 0.5f);
-      int id = sh_node_base + off_child;
-      int sid = sh_seg_base + off_seg;
-      for (int o = 0; o < 8; ++o) {
-        const int c = cnt[o];
-        if (c == 0) {
-          L.seg_next[cur][s * 8 + o] = -1;
-          continue;
-        }
-        if (id >= max_nodes) {  // past the budget: the sequential builder takes over
-          L.seg_next[cur][s * 8 + o] = -1;
-          st->overflow = 1;
-          continue;
-        }
-        const int t1 = L.seg_min[cur][s * 8 + o];
+    if (valid) {
+      const int id = sh_node_base + off_child;
+      if (c == 0) {
+        L.seg_next[cur][u] = -1;
+      } else if (id >= max_nodes) {  // past the budget: the sequential builder takes over
+        L.seg_next[cur][u] = -1;
+        st->overflow = 1;
+      } else {
+        const int s = u >> 3, o = u & 7;
+        const int X = L.seg_node[cur][s];
+        const float4 pb = T.box[X];
+        const float hs = mul_rn(pb.w, 0.5f);
+        const int t1 = L.seg_min[cur][u];
         const bool leaf = (c == 1) || (cd == kMaxDepth);
         T.t1[id] = t1;
-        T.k[id] = d == 0 ? t1 : max(t2, t1);
+        T.k[id] = d == 0 ? t1 : max(L.seg_t2[cur][s], t1);
         T.parent[id] = X;
         T.count[id] = c;
         T.info[id] = (uint32_t)cd | ((uint32_t)o << 8) | (leaf ? (1u << 11) : 0u);
         T.box[id] = make_float4(add_rn(pb.x, (o & 1) ? hs : -hs), add_rn(pb.y, (o & 2) ? hs : -hs),
                                 add_rn(pb.z, (o & 4) ? hs : -hs), hs);
-        for (int q = 0; q < 8; ++q) T.child[id * 8 + q] = -1;
+        int4* ch = reinterpret_cast<int4*>(T.child + (size_t)id * 8);
+        ch[0] = make_int4(-1, -1, -1, -1);
+        ch[1] = make_int4(-1, -1, -1, -1);
         T.child[X * 8 + o] = id;
         T.member_base[id] = -1;
         if (c == 1) {
-          const int p = sidx[t1];
-          T.pidx[id] = p;
-          T.com[id] = make_float4(x[p], y[p], z[p], m[p]);
-          L.seg_next[cur][s * 8 + o] = -1;
+          T.pidx[id] = sidx[t1];
+          T.com[id] = sbody[t1];
+          L.seg_next[cur][u] = -1;
         } else if (leaf) {  // depth-32 cell with several particles
           T.pidx[id] = sidx[t1];
           const int slot = atomicAdd(&st->multi_count, 1);
           L.multi_list[slot] = id;
           T.member_base[id] = atomicAdd(&st->member_count, c);
           T.member_fill[id] = 0;
-          L.seg_next[cur][s * 8 + o] = -(2 + id);
+          L.seg_next[cur][u] = -(2 + id);
         } else {
+          const int sid = sh_seg_base + off_seg;
           T.pidx[id] = -1;
           L.seg_node[nxt][sid] = id;
           L.seg_t1[nxt][sid] = t1;
           L.seg_t2[nxt][sid] = INT_MAX;
-          for (int q = 0; q < 8; ++q) {
-            L.seg_cnt[nxt][sid * 8 + q] = 0;
-            L.seg_min[nxt][sid * 8 + q] = INT_MAX;
-          }
-          L.seg_next[cur][s * 8 + o] = sid;
-          ++sid;
+          int4* sc = reinterpret_cast<int4*>(L.seg_cnt[nxt] + (size_t)sid * 8);
+          int4* sm = reinterpret_cast<int4*>(L.seg_min[nxt] + (size_t)sid * 8);
+          sc[0] = make_int4(0, 0, 0, 0);
+          sc[1] = make_int4(0, 0, 0, 0);
+          sm[0] = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+          sm[1] = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
+          L.seg_next[cur][u] = sid;
         }
-        ++id;
+        sh_deep = cd;  // benign race: every writer stores the same value
       }
-      sh_deep = cd;  // benign race: every writer stores the same value
     }
     __syncthreads();
     if (threadIdx.x == 0 && sh_deep) atomicMax(&st->deepest, sh_deep);
@@ -1047,6 +1043,7 @@ struct nbx_bh {
   uint32_t* codes = nullptr;
   int32_t* idx = nullptr;
   uint4* path = nullptr;
+  float4* sbody = nullptr;  // particles in sorted order: x, y, z, m
   nbx::TmpNodes T{};
   nbx::Levels L{};
   uint32_t* nkeys = nullptr;
@@ -1134,6 +1131,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->codes, P);
   A(&t->idx, P);
   A(&t->path, P);
+  A(&t->sbody, P);
   A(&t->T.t1, NN);
   A(&t->T.k, NN);
   A(&t->T.parent, NN);
@@ -1209,15 +1207,14 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   NBX_TRY(t->sort_particles.sort(t->codes, t->idx, &t->st->n, 30, s));
   mark(t, 3, s);
   hipLaunchKernelGGL(path_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, d->pos_x,
-                     d->pos_y, d->pos_z, t->idx, n, t->st, t->path);
+                     d->pos_y, d->pos_z, d->mass, t->idx, n, t->st, t->path, t->sbody);
   NBX_LAUNCHED();
   mark(t, 4, s);
   hipLaunchKernelGGL(tree_init, dim3(1), dim3(64), 0, s, t->st, t->T, t->L, t->max_nodes);
   for (int lv = 0; lv < kMaxDepth; ++lv) {
     hipLaunchKernelGGL(level_count, dim3(gl), dim3(kThreads), 0, s, t->st, t->L, t->path, lv);
-    hipLaunchKernelGGL(level_children, dim3(grid_for((size_t)n / 2 + 1)), dim3(kThreads), 0, s,
-                       t->st, t->L, t->T, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, lv,
-                       t->max_nodes);
+    hipLaunchKernelGGL(level_children, dim3(grid_for((size_t)n * 4 + 8)), dim3(kThreads), 0, s,
+                       t->st, t->L, t->T, t->idx, t->sbody, lv, t->max_nodes);
     hipLaunchKernelGGL(level_route, dim3(gl), dim3(kThreads), 0, s, t->st, t->L, t->T, t->path,
                        lv);
   }
