@@ -3,6 +3,7 @@
 
 #include "nbody/barnes_hut_tree.hpp"
 #include "nbody/error_handling.hpp"
+#include "nbody/performance_observability.hpp"
 #include "nbx.h"
 
 namespace nbody {
@@ -14,6 +15,7 @@ BarnesHutTree::BarnesHutTree(size_t max_particles) : max_particles_(max_particle
 BarnesHutTree::~BarnesHutTree() { nbx_bh_destroy(handle_); }
 
 void BarnesHutTree::build(const ParticleData* d_particles) {
+  NBODY_PROFILE_SCOPE("barnes_hut.build");
   NBODY_CHECK(nbx_bh_build(handle_, reinterpret_cast<const nbx_particles*>(d_particles), nullptr));
 }
 

@@ -6,6 +6,7 @@
 #include "nbody/barnes_hut_tree.hpp"
 #include "nbody/error_handling.hpp"
 #include "nbody/force_calculator.hpp"
+#include "nbody/performance_observability.hpp"
 #include "nbody/spatial_hash_grid.hpp"
 #include "nbx.h"
 
@@ -16,6 +17,7 @@ static nbx_particles* raw(ParticleData* d) { return reinterpret_cast<nbx_particl
 DirectForceCalculator::DirectForceCalculator(int block_size) : block_size_(block_size) {}
 
 void DirectForceCalculator::computeForces(ParticleData* d_particles) {
+  NBODY_PROFILE_SCOPE("force.direct_n2");
   NBODY_CHECK(nbx_direct_forces(raw(d_particles), G_, softening_eps2_, block_size_, nullptr));
 }
 
@@ -35,6 +37,7 @@ SpatialHashCalculator::~SpatialHashCalculator() = default;
 // :376-388. The grid is remade when the effective cell changes, and also when the
 // particle count outgrows it (the reference keeps the first count's buffers).
 void SpatialHashCalculator::computeForces(ParticleData* d_particles) {
+  NBODY_PROFILE_SCOPE("force.spatial_hash");
   const float effective_cell = std::max(cell_size_, cutoff_radius_);
   if (!grid_ || grid_->getCellSize() != effective_cell ||
       d_particles->count > grid_->getMaxParticles())

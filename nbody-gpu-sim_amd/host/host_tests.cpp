@@ -11,6 +11,7 @@
 #include <iostream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "nbody/app_cli.hpp"
@@ -300,13 +301,52 @@ TEST_CPU(benchmark_json_schema) {
   r.iterations = 3;
   r.metrics["wall_time_ms"] = 1.5;
   r.parameters["dt"] = 0.001;
-  r.phase_timings.push_back({"phase \"a\"", 2.0, 3});
+  r.phase_timings.push_back({"phase \"a\"", Milliseconds(2.0), 3});
   const std::string j = serializeBenchmarkRunRecords({r});
   CHECK(j.rfind("{\"benchmarks\":[{\"benchmark_name\":\"force.direct_n2\",", 0) == 0);
   CHECK(j.find("\"force_method\":\"direct_n2\"") != std::string::npos);
   CHECK(j.find("\"metrics\":{\"wall_time_ms\":1.5}") != std::string::npos);
   CHECK(j.find("\\\"a\\\"") != std::string::npos);
   CHECK(j.find("\"samples\":3}]}]}") != std::string::npos);
+}
+
+// tests/test_performance_observability.cpp:7-24 and :26-39.
+TEST_CPU(benchmark_record_fields) {
+  BenchmarkRunRecord r;
+  r.benchmark_name = "force.direct_n2";
+  r.force_method = ForceMethod::DIRECT_N2;
+  r.particle_count = 4096;
+  r.iterations = 12;
+  r.metrics["wall_time_ms"] = 1.25;
+  r.metrics["throughput_particles_per_second"] = 32768.0;
+  r.parameters["cuda_block_size"] = 256.0;
+  const std::string j = serializeBenchmarkRunRecord(r);
+  CHECK(j.find("\"benchmark_name\":\"force.direct_n2\"") != std::string::npos);
+  CHECK(j.find("\"particle_count\":4096") != std::string::npos);
+  CHECK(j.find("\"force_method\":\"direct_n2\"") != std::string::npos);
+  CHECK(j.find("\"wall_time_ms\":1.25") != std::string::npos);
+  CHECK(j.find("\"cuda_block_size\":256") != std::string::npos);
+}
+
+TEST_CPU(scoped_phase_profile_accumulates) {
+  PhaseProfiler profiler;
+  {
+    ScopedPhaseProfile profile(profiler, "simulation.update");
+    std::this_thread::sleep_for(std::chrono::milliseconds(1));
+  }
+  {
+    ScopedPhaseProfile profile(profiler, "simulation.update");
+  }
+  {
+    ScopedPhaseProfile profile(profiler, "force.direct_n2");
+  }
+  const auto snap = profiler.snapshot();
+  CHECK(snap.size() == 2u);
+  CHECK(snap[0].name == "simulation.update" && snap[0].samples == 2u);
+  CHECK(snap[0].total_duration.count() >= 0.5);
+  CHECK(snap[1].name == "force.direct_n2" && snap[1].samples == 1u);
+  const auto drained = profiler.consume();
+  CHECK(drained.size() == 2u && profiler.snapshot().empty());
 }
 
 TEST_CPU(generators_follow_reference_ranges) {

@@ -2,6 +2,7 @@
 #include "nbody/error_handling.hpp"
 #include "nbody/force_calculator.hpp"
 #include "nbody/integrator.hpp"
+#include "nbody/performance_observability.hpp"
 #include "nbx.h"
 
 namespace nbody {
@@ -25,6 +26,7 @@ Integrator::Integrator(int block_size) : block_size_(block_size) {}
 Integrator::~Integrator() = default;
 
 void Integrator::integrate(ParticleData* d, ForceCalculator* force_calc, float dt) {
+  NBODY_PROFILE_SCOPE("integrator.velocity_verlet");
   if (dynamic_cast<DirectForceCalculator*>(force_calc)) {
     // same four reference kernels, fused into two launches (drift; forces + kick)
     NBODY_CHECK(nbx_verlet_drift(raw(d), dt, nullptr));

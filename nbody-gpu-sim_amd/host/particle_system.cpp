@@ -7,6 +7,7 @@
 
 #include "nbody/error_handling.hpp"
 #include "nbody/particle_system.hpp"
+#include "nbody/performance_observability.hpp"
 #include "nbody/serialization.hpp"
 
 namespace nbody {
@@ -100,6 +101,7 @@ void ParticleSystem::createForceCalculator() {
 void ParticleSystem::update(float dt) {
   if (!is_initialized_ || is_paused_) return;
   validateTimeStep(dt);
+  NBODY_PROFILE_SCOPE("simulation.update");
   integrator_->integrate(&d_particles_, force_calculator_.get(), dt);
   simulation_time_ += dt;
 }

@@ -1,5 +1,5 @@
-// Benchmark JSON records (schema of the reference's
-// src/utils/performance_observability.cpp:97-148).
+// Phase profiler and benchmark JSON records (behaviour and schema of the
+// reference's src/utils/performance_observability.cpp:50-160).
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -43,7 +43,55 @@ void numberMap(std::ostringstream& out, const std::map<std::string, double>& m) 
   out << "}";
 }
 
-std::string record(const BenchmarkRunRecord& r) {
+PhaseProfiler g_profiler;
+
+}  // namespace
+
+void PhaseProfiler::record(std::string_view name, Milliseconds duration) {
+  std::lock_guard<std::mutex> lock(mutex_);
+  for (auto& p : phases_)
+    if (p.name == name) {
+      p.total_duration += duration;
+      p.samples += 1;
+      return;
+    }
+  PhaseTiming p;
+  p.name = std::string(name);
+  p.total_duration = duration;
+  p.samples = 1;
+  phases_.push_back(std::move(p));
+}
+
+std::vector<PhaseTiming> PhaseProfiler::snapshot() const {
+  std::lock_guard<std::mutex> lock(mutex_);
+  return phases_;
+}
+
+void PhaseProfiler::reset() {
+  std::lock_guard<std::mutex> lock(mutex_);
+  phases_.clear();
+}
+
+std::vector<PhaseTiming> PhaseProfiler::consume() {
+  std::lock_guard<std::mutex> lock(mutex_);
+  std::vector<PhaseTiming> out;
+  out.swap(phases_);
+  return out;
+}
+
+ScopedPhaseProfile::ScopedPhaseProfile(PhaseProfiler& profiler, std::string name)
+    : profiler_(profiler), name_(std::move(name)), start_time_(std::chrono::steady_clock::now()) {}
+
+ScopedPhaseProfile::~ScopedPhaseProfile() {
+  profiler_.record(name_, std::chrono::duration_cast<Milliseconds>(
+                              std::chrono::steady_clock::now() - start_time_));
+}
+
+PhaseProfiler& globalPhaseProfiler() { return g_profiler; }
+
+std::vector<PhaseTiming> consumeGlobalPhaseSnapshot() { return g_profiler.consume(); }
+
+std::string serializeBenchmarkRunRecord(const BenchmarkRunRecord& r) {
   std::ostringstream out;
   out.precision(17);
   out << "{\"benchmark_name\":\"" << escapeJson(r.benchmark_name) << "\",";
@@ -59,20 +107,18 @@ std::string record(const BenchmarkRunRecord& r) {
     const auto& p = r.phase_timings[i];
     if (i) out << ",";
     out << "{\"name\":\"" << escapeJson(p.name) << "\",\"total_duration_ms\":"
-        << p.total_duration_ms << ",\"samples\":" << p.samples << "}";
+        << p.total_duration.count() << ",\"samples\":" << p.samples << "}";
   }
   out << "]}";
   return out.str();
 }
-
-}  // namespace
 
 std::string serializeBenchmarkRunRecords(const std::vector<BenchmarkRunRecord>& records) {
   std::ostringstream out;
   out << "{\"benchmarks\":[";
   for (size_t i = 0; i < records.size(); ++i) {
     if (i) out << ",";
-    out << record(records[i]);
+    out << serializeBenchmarkRunRecord(records[i]);
   }
   out << "]}";
   return out.str();

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <fstream>
 
+#include "nbody/performance_observability.hpp"
 #include "nbody/serialization.hpp"
 
 namespace nbody {
@@ -22,6 +23,7 @@ SimulationState Serializer::load(const std::string& filename) {
 }
 
 void Serializer::save(std::ostream& out, const SimulationState& state) {
+  NBODY_PROFILE_SCOPE("serialization.save");
   writeHeader(out, state);
   writeFloatArray(out, state.pos_x);
   writeFloatArray(out, state.pos_y);
@@ -33,6 +35,7 @@ void Serializer::save(std::ostream& out, const SimulationState& state) {
 }
 
 SimulationState Serializer::load(std::istream& in) {
+  NBODY_PROFILE_SCOPE("serialization.load");
   const FileHeader header = readHeader(in);
   if (header.particle_count > MAX_PARTICLE_COUNT)
     throw ValidationException("Particle count (" + std::to_string(header.particle_count) +

@@ -58,3 +58,67 @@ def test_headless_benchmark_export_import(tmp_path, method):
     assert os.path.getsize(ck) == 56 + 7 * 4 * 4096
     p = run([sim, "--benchmark", "--benchmark-steps", "2", "--import", ck])
     assert p.returncode == 0 and "Imported 4096 particles" in p.stdout
+
+
+BENCH_NAMES = ["serialization.round_trip", "force.direct_n2", "force.barnes_hut",
+               "force.spatial_hash", "integration.velocity_verlet"]
+
+
+def test_benchmark_suite_cli_and_serialization(tmp_path):
+    """nbody_benchmarks: the reference's suite (benchmarks/benchmark_main.cpp)."""
+    exe = os.path.join(PKG, "nbody_benchmarks")
+    p = run([exe, "--list"])
+    assert p.returncode == 0
+    for name in BENCH_NAMES:
+        assert "  - " + name in p.stdout
+    p = run([exe, "--bogus"])
+    assert p.returncode == 1 and "Unknown benchmark argument: --bogus" in p.stderr
+    p = run([exe, "--iterations", "0"])
+    assert p.returncode == 1 and "iterations must be greater than zero" in p.stderr
+    p = run([exe, "--benchmark", "force.fmm"])
+    assert p.returncode == 1 and "Requested benchmark was not found" in p.stderr
+    out = str(tmp_path / "r.json")
+    p = run([exe, "--benchmark", "serialization.round_trip", "--particle-count", "1000",
+             "--iterations", "2", "--output", out])
+    assert p.returncode == 0, p.stderr
+    recs = json.load(open(out))["benchmarks"]
+    assert json.loads(p.stdout)["benchmarks"] == recs and len(recs) == 1
+    r = recs[0]
+    assert r["benchmark_name"] == "serialization.round_trip" and r["force_method"] == "direct_n2"
+    assert r["particle_count"] == 1000 and r["iterations"] == 2
+    # .nbody v1: 56-byte header + 7 float arrays
+    assert r["metrics"]["bytes_per_iteration"] == 56 + 7 * 4 * 1000
+    assert r["metrics"]["wall_time_ms"] > 0
+    assert r["parameters"] == {"particle_count": 1000}
+    assert [(t["name"], t["samples"]) for t in r["phase_timings"]] == \
+        [("serialization.save", 2), ("serialization.load", 2)]
+
+
+@pytest.mark.gpu
+def test_benchmark_suite_all(tmp_path):
+    exe = os.path.join(PKG, "nbody_benchmarks")
+    out = str(tmp_path / "all.json")
+    p = run([exe, "--benchmark", "all", "--particle-count", "4096", "--iterations", "3",
+             "--output", out])
+    assert p.returncode == 0, p.stderr
+    recs = {r["benchmark_name"]: r for r in json.load(open(out))["benchmarks"]}
+    assert list(recs) == BENCH_NAMES
+    for r in recs.values():
+        assert r["particle_count"] == 4096 and r["iterations"] == 3
+        assert r["metrics"]["wall_time_ms"] > 0
+    assert recs["force.direct_n2"]["parameters"] == {"particle_count": 4096,
+                                                    "cuda_block_size": 256}
+    bh = recs["force.barnes_hut"]
+    assert bh["force_method"] == "barnes_hut"
+    assert bh["parameters"] == {"particle_count": 4096, "theta": 0.5}
+    # 2 warm-up + 3 timed calls of the tree build scope
+    assert [t["samples"] for t in bh["phase_timings"] if t["name"] == "barnes_hut.build"] == [5]
+    assert bh["metrics"]["barnes_hut.build_ms"] > 0
+    assert bh["metrics"]["gpu.forces_ms"] > 0 and bh["metrics"]["gpu.sort_ms"] > 0
+    sh = recs["force.spatial_hash"]
+    assert sh["force_method"] == "spatial_hash"
+    assert set(sh["parameters"]) == {"particle_count", "cuda_block_size", "cell_size",
+                                     "cutoff_radius"}
+    vv = recs["integration.velocity_verlet"]
+    assert vv["parameters"]["dt"] == pytest.approx(0.001)
+    assert [t["samples"] for t in vv["phase_timings"] if t["name"] == "simulation.update"] == [5]
