@@ -608,12 +608,13 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
 // order the reference's stack walk visits them (:160-241: children pushed
 // 0..7, so popped 7..0), i.e. depth-first preorder with children 7 down to 0.
 // Record p (64 bytes, so that one scalar address reaches all of it):
-//   rec[4p]   = (centre of mass, mass)
+//   rec[4p]   = (centre of mass, G * mass; written by trav_prep per force call)
 //   rec[4p+1] = (cell centre, half size; negative for a leaf)
 //   rec[4p+2] = (position after the cell's subtree as int bits, the opening-test
 //                size (2h)^2 = 4 h h rounded as the traversal's test (:218-224)
-//                rounds it, 0, 0); rec[4p+3] is padding.
-// A leaf's (2h)^2 is written as -1: size2 < d2 * theta2 then holds for every
+//                rounds it, S_open, S_acc; the last two written by trav_prep)
+//   rec[4p+3] = (mass, 0, 0, 0)
+// A leaf's (2h)^2 is written as -1: its thresholds are negative for every
 // theta >= 0, so the theta test never opens it, and the walk needs the leaf flag
 // only where a lane's own particle lies in the cell. A node of zero mass, which
 // the reference skips with its whole subtree (:172-173), gets its centre of mass
@@ -628,12 +629,14 @@ __device__ __forceinline__ void write_trav(float4* rec, int p, int size, float4 
     rec[4 * p] = make_float4(kFarPos, kFarPos, kFarPos, 0.0f);
     rec[4 * p + 1] = make_float4(kFarPos, kFarPos, kFarPos, leaf ? -0.0f : 0.0f);
     rec[4 * p + 2] = make_float4(__int_as_float(p + size), kNeverOpen, 0.0f, 0.0f);
+    rec[4 * p + 3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return;
   }
   rec[4 * p] = com;
   rec[4 * p + 1] = make_float4(box.x, box.y, box.z, leaf ? -box.w - 0.0f : box.w);
   rec[4 * p + 2] = make_float4(__int_as_float(p + size),
                                leaf ? kNeverOpen : mul_rn(mul_rn(4.0f, box.w), box.w), 0.0f, 0.0f);
+  rec[4 * p + 3] = make_float4(com.w, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict__ st, TmpNodes T,
@@ -868,164 +871,299 @@ __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __re
 
 // ---------------------------------------------------------------- 7: traversal
 
-// Wave-coherent traversal over the preorder layout. A wave holds 64 * PPL
-// consecutive particles in Morton order (PPL slots of 64 per lane) and walks
-// the union of their trees in the reference's visiting order (:160-241).
-// State: the current record v and, per slot, the mask M of lanes that need it. Each lane takes exactly the
-// reference's accept/open decisions in the same IEEE arithmetic. If some lanes
-// open v, the walk descends to v + 1 (its first child in this order) with
-// those lanes, remembering (skip[v], masks); otherwise it continues at skip[v].
-// With PPL = 2 two particles per lane share the walk's scalar work, but the
-// union of 128 particles' trees grows enough that PPL = 1 is faster on MI355X
-// (1M two-Plummer traversal: 6.7 ms vs 7.0 ms; PPL = 4: 9.3 ms).
-// Arriving at a remembered position restores its mask. Positions only grow,
-// so the walk takes at most one step per record, and the remembered entries
-// are ancestors of v: at most the tree depth (<= 33), kept one per lane in
-// VGPRs. A lane's interactions are summed in the reference's order.
-// Particles per lane: a wave walks the union of 64 * kTravPPL particles' trees.
-#ifndef NBX_BH_PPL
-#define NBX_BH_PPL 1
-#endif
-constexpr int kTravPPL = NBX_BH_PPL;
-
-template <int PPL>
-__global__ __launch_bounds__(kThreads) void bh_traverse(
-    const float4* __restrict__ rec,
-    const BhState* __restrict__ st, const int32_t* __restrict__ order,
-    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-    const float* __restrict__ m, int n, float theta2, float G, float eps2, float* ax, float* ay,
-    float* az) {
-  const int lane = threadIdx.x & 63;
-  // wave w: particles [w * 64 * PPL, (w + 1) * 64 * PPL) in Morton order;
-  // slot q of lane l holds particle base + q * 64 + l
-  const int base = (blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)) * 64 * PPL;
-  int ip[PPL];
-  bool valid[PPL];
-  float xi[PPL], yi[PPL], zi[PPL], mi[PPL], sx[PPL], sy[PPL], sz[PPL];
-  uint64_t M[PPL];
-#pragma unroll
-  for (int q = 0; q < PPL; ++q) {
-    const int r = base + q * 64 + lane;
-    valid[q] = r < n;
-    ip[q] = valid[q] ? order[r] : 0;
-    xi[q] = x[ip[q]];
-    yi[q] = y[ip[q]];
-    zi[q] = z[ip[q]];
-    mi[q] = m[ip[q]];
-    sx[q] = sy[q] = sz[q] = 0.f;
-    M[q] = __ballot(valid[q]);
+// Per force call: each record's opening thresholds for this theta and its
+// G * mass. rec[4p+2].zw = (S_open, S_acc) with S = size2 * (1 / theta2),
+// S_open = S (1 - 2^-20) and S_acc = S (1 + 2^-20), each product rounded once;
+// rec[4p].w = G * mass rounded once, as the traversal's force term rounds it.
+// S_open and S_acc are within 2^-22 (relative) of size2 / theta2 (1 -+ 2^-20),
+// so d2 < S_open implies size2 / d2 > theta2 (1 + 2^-21) and d2 > S_acc implies
+// size2 / d2 < theta2 (1 - 2^-21): either way the rounded division takes the
+// same decision (:218-224). In between, the walk divides. Leaves and zero-mass
+// cells (size2 = -1) get negative thresholds and are never opened by the test;
+// theta = 0 gives S = +inf, so every cell with mass is opened.
+__global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState* __restrict__ st,
+                                                      float inv_theta2, float G) {
+  const int nn = st->final_count;
+  for (int p = blockIdx.x * kThreads + threadIdx.x; p < nn; p += gridDim.x * kThreads) {
+    const float mass = rec[4 * p + 3].x;
+    const float size2 = rec[4 * p + 2].y;
+    const float sc = mul_rn(size2, inv_theta2);
+    rec[4 * p].w = mul_rn(G, mass);
+    rec[4 * p + 2].z = mul_rn(sc, 1.0f - 0x1p-20f);
+    rec[4 * p + 2].w = mul_rn(sc, 1.0f + 0x1p-20f);
   }
+}
+
+// Wave-coherent traversal over the preorder layout. A wave holds 64
+// consecutive particles in Morton order and walks the union of their trees in
+// the reference's visiting order (:160-241). State: the current record v and
+// the mask M of lanes that need it. Each lane takes exactly the reference's
+// accept/open decisions in the same IEEE arithmetic. If some lanes open v, the
+// walk descends to v + 1 (its first child in this order) with those lanes,
+// remembering (skip[v], M); otherwise it continues at skip[v]. Arriving at a
+// remembered position restores its mask. Positions only grow (the skip is
+// clamped to at least v + 1), so the walk takes at most one step per record,
+// and the remembered entries are ancestors of v: at most the tree depth
+// (<= 33), kept one per lane in VGPRs (lane k holds entry k). A lane's
+// interactions are summed in the reference's order.
+//
+// Each step waits on one dependent record load, which mostly misses the
+// scalar cache; that latency, not instruction issue, bounds the walk. So the
+// common step is written out by hand (NBX_WALK_FAST) and loads ahead: while
+// it works on record v it already fetches both records the next step can
+// need, v + 1 (descend) and skip[v] (continue). Three SGPR sets of 12 dwords
+// rotate: the step on set X loads v + 1 into Y and skip[v] into Z, and
+// continues with Y or Z (A -> B/C, B -> C/A, C -> A/B). Pops change only the
+// mask, never v, so the next record is always one of the two. Prefetch
+// addresses are clamped to the last record, and every exit waits for the
+// loads in flight. The common step is: no lane's particle inside a leaf and
+// every lane's theta decision clear of the band where the division decides.
+// Any other step leaves the asm and runs as C code below (the own-leaf
+// correction, the IEEE division); the walk then re-enters the asm.
+//
+// Registers: s[20:31], s[32:43], s[44:55] the record sets (dwords 0..11 of a
+// record), s[56:57] M, s[58:59] the lanes inside the cell, s[60:61] the lanes
+// that open it, s[62:63] the lanes that clearly accept it, s[64:69] further
+// lane sets, s70 = v + 1, s71 = skip clamped to at least v + 1, s72/s73 load
+// offsets, s74 = nn - 1, s75 the pop lane. Ordering: the result of v_rsq is
+// first read two instructions later; lane sets pass from vector compares to
+// scalar instructions (interlocked) and from scalar instructions to v_cndmask
+// (no hazard); readlane takes its lane from a scalar instruction and writelane
+// from m0, written one instruction before.
+#define NBX_WALK_STEP(X, Y, Z, c0, c1, c2, gm, b0, b1, b2, hh, sk, so, sa, y8, y4, z8, z4) \
+  "L_next_" X "_%=:\n"                                                           \
+  "s_cmp_ge_i32 %[v], %[nn]\n"                                                   \
+  "s_cbranch_scc1 L_exit_%=\n"                                                   \
+  "L_popchk_" X "_%=:\n"                                                         \
+  "s_cmp_eq_u32 %[v], %[top]\n"                                                  \
+  "s_cbranch_scc1 L_pop_" X "_%=\n"                                              \
+  "s_waitcnt lgkmcnt(0)\n"                                                       \
+  "s_add_i32 s70, %[v], 1\n"                                                     \
+  "s_max_i32 s71, " sk ", s70\n"                                                 \
+  "s_min_i32 s72, s70, s74\n"                                                    \
+  "s_lshl_b32 s72, s72, 6\n"                                                     \
+  "s_min_i32 s73, s71, s74\n"                                                    \
+  "s_lshl_b32 s73, s73, 6\n"                                                     \
+  "s_load_dwordx8 " y8 ", %[rb], s72\n"                                          \
+  "s_load_dwordx4 " y4 ", %[rb], s72 offset:32\n"                                \
+  "s_load_dwordx8 " z8 ", %[rb], s73\n"                                          \
+  "s_load_dwordx4 " z4 ", %[rb], s73 offset:32\n"                                \
+  "v_subrev_f32 %[t0], " b0 ", %[xi]\n"                                          \
+  "v_subrev_f32 %[t1], " b1 ", %[yi]\n"                                          \
+  "v_subrev_f32 %[t2], " b2 ", %[zi]\n"                                          \
+  "v_max3_f32 %[t0], |%[t0]|, |%[t1]|, |%[t2]|\n"                                \
+  "v_cmp_le_f32_e64 s[58:59], %[t0], |" hh "|\n"                                 \
+  "v_sub_f32 %[dx], " c0 ", %[xi]\n"                                             \
+  "v_sub_f32 %[dy], " c1 ", %[yi]\n"                                             \
+  "v_sub_f32 %[dz], " c2 ", %[zi]\n"                                             \
+  "v_mul_f32 %[d2], %[dx], %[dx]\n"                                              \
+  "v_mul_f32 %[t1], %[dy], %[dy]\n"                                              \
+  "v_add_f32 %[d2], %[d2], %[t1]\n"                                              \
+  "v_mul_f32 %[t1], %[dz], %[dz]\n"                                              \
+  "v_add_f32 %[d2], %[d2], %[t1]\n"                                              \
+  "v_add_f32 %[d2], %[eps2], %[d2]\n"                                            \
+  "v_rsq_f32 %[inv], %[d2]\n"                                                    \
+  "v_cmp_gt_f32_e64 s[60:61], " so ", %[d2]\n"                                   \
+  "v_cmp_lt_f32_e64 s[62:63], " sa ", %[d2]\n"                                   \
+  "v_mul_f32 %[t1], %[inv], %[inv]\n"                                            \
+  "v_mul_f32 %[t1], %[t1], %[inv]\n"                                             \
+  "v_mul_f32 %[t1], " gm ", %[t1]\n"                                             \
+  "s_and_b64 s[58:59], s[58:59], s[56:57]\n"                                     \
+  "s_cbranch_scc1 L_inside_" X "_%=\n"                                           \
+  "L_back_" X "_%=:\n"                                                           \
+  "s_or_b64 s[64:65], s[60:61], s[62:63]\n"                                      \
+  "s_andn2_b64 s[64:65], s[56:57], s[64:65]\n"                                   \
+  "s_cbranch_scc1 L_exit_%=\n"                                                   \
+  "s_andn2_b64 s[66:67], s[56:57], s[60:61]\n"                                   \
+  "s_and_b64 s[68:69], s[56:57], s[60:61]\n"                                     \
+  "v_cndmask_b32_e64 %[t1], 0, %[t1], s[66:67]\n"                                \
+  "v_fmac_f32 %[sx], %[t1], %[dx]\n"                                             \
+  "v_fmac_f32 %[sy], %[t1], %[dy]\n"                                             \
+  "v_fmac_f32 %[sz], %[t1], %[dz]\n"                                             \
+  "s_cbranch_scc1 L_desc_" X "_%=\n"                                             \
+  "s_mov_b32 %[v], s71\n"                                                        \
+  "s_branch L_next_" Z "_%=\n"                                                   \
+  "L_desc_" X "_%=:\n"                                                           \
+  "s_mov_b32 m0, %[depth]\n"                                                     \
+  "s_add_i32 %[depth], %[depth], 1\n"                                            \
+  "v_writelane_b32 %[kat], s71, m0\n"                                            \
+  "v_writelane_b32 %[klo], s56, m0\n"                                            \
+  "v_writelane_b32 %[khi], s57, m0\n"                                            \
+  "s_mov_b32 %[top], s71\n"                                                      \
+  "s_mov_b64 s[56:57], s[68:69]\n"                                               \
+  "s_mov_b32 %[v], s70\n"                                                        \
+  "s_branch L_next_" Y "_%=\n"                                                   \
+  "L_inside_" X "_%=:\n"                                                         \
+  "s_cmp_lt_i32 " hh ", 0\n"                                                     \
+  "s_cbranch_scc1 L_exit_%=\n"                                                   \
+  "s_or_b64 s[60:61], s[60:61], s[58:59]\n"                                      \
+  "s_branch L_back_" X "_%=\n"                                                   \
+  "L_pop_" X "_%=:\n"                                                            \
+  "s_add_i32 %[depth], %[depth], -1\n"                                           \
+  "v_readlane_b32 s56, %[klo], %[depth]\n"                                       \
+  "v_readlane_b32 s57, %[khi], %[depth]\n"                                       \
+  "s_add_i32 s75, %[depth], -1\n"                                                \
+  "s_cmp_lt_i32 s75, 0\n"                                                        \
+  "s_cbranch_scc1 L_root_" X "_%=\n"                                             \
+  "v_readlane_b32 %[top], %[kat], s75\n"                                         \
+  "s_branch L_popchk_" X "_%=\n"                                                 \
+  "L_root_" X "_%=:\n"                                                           \
+  "s_brev_b32 %[top], -2\n"                                                      \
+  "s_branch L_popchk_" X "_%=\n"
+
+#define NBX_WALK_FAST                                                                       \
+  "s_mov_b64 s[56:57], %[M]\n"                                                              \
+  "s_add_i32 s74, %[nn], -1\n"                                                              \
+  "s_min_i32 s72, %[v], s74\n"                                                              \
+  "s_max_i32 s72, s72, 0\n"                                                                 \
+  "s_lshl_b32 s72, s72, 6\n"                                                                \
+  "s_load_dwordx8 s[20:27], %[rb], s72\n"                                                   \
+  "s_load_dwordx4 s[28:31], %[rb], s72 offset:32\n"                                         \
+  "s_branch L_next_A_%=\n"                                                                  \
+  NBX_WALK_STEP("A", "B", "C", "s20", "s21", "s22", "s23", "s24", "s25", "s26", "s27", "s28", \
+                "s30", "s31", "s[32:39]", "s[40:43]", "s[44:51]", "s[52:55]")               \
+  NBX_WALK_STEP("B", "C", "A", "s32", "s33", "s34", "s35", "s36", "s37", "s38", "s39", "s40", \
+                "s42", "s43", "s[44:51]", "s[52:55]", "s[20:27]", "s[28:31]")               \
+  NBX_WALK_STEP("C", "A", "B", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", \
+                "s54", "s55", "s[20:27]", "s[28:31]", "s[32:39]", "s[40:43]")               \
+  "L_exit_%=:\n"                                                                            \
+  "s_waitcnt lgkmcnt(0)\n"                                                                  \
+  "s_mov_b64 %[M], s[56:57]\n"
+
+__global__ __launch_bounds__(kThreads) void bh_traverse(
+    const float4* __restrict__ rec, const BhState* __restrict__ st,
+    const int32_t* __restrict__ order, const float* __restrict__ x, const float* __restrict__ y,
+    const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
+    float eps2, float* ax, float* ay, float* az) {
+  const int lane = threadIdx.x & 63;
+  // Workgroups are dealt to the 8 XCDs round-robin; each XCD takes one
+  // contiguous run of Morton order instead, so that its L2 holds the part of
+  // the tree its waves walk.
+  const int nb = (int)gridDim.x, per = nb / 8, rem = nb % 8;
+  const int xcd = (int)blockIdx.x % 8, slot = (int)blockIdx.x / 8;
+  const int blk = xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+  // wave w: particles [w * 64, (w + 1) * 64) in Morton order
+  const int r = (blk * (kThreads / 64) + (int)(threadIdx.x >> 6)) * 64 + lane;
+  const bool valid = r < n;
+  const int ip = valid ? order[r] : 0;
+  const float xi = x[ip], yi = y[ip], zi = z[ip], mi = m[ip];
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  uint64_t M = __ballot(valid);
   const int nn = st->final_count;
   const uint64_t me = 1ull << lane;
-  int keep_at = 0;                          // lane k: position of remembered entry k
-  uint32_t keep_lo[PPL], keep_hi[PPL];      // lane k: its masks
-#pragma unroll
-  for (int q = 0; q < PPL; ++q) keep_lo[q] = keep_hi[q] = 0;
+  int keep_at = 0;              // lane k: position of remembered entry k
+  uint32_t keep_lo = 0, keep_hi = 0;  // lane k: its mask
   int depth = 0;
   int top = INT_MAX;  // position of the innermost remembered entry (INT_MAX: none)
   int v = 0;
-  const float theta2_hi = theta2 * (1.0f + 0x1p-20f), theta2_lo = theta2 * (1.0f - 0x1p-20f);
-  // Every step moves v forward (to v + 1, or to the skip index, clamped to at
-  // least v + 1), so the walk ends after at most nn steps. The loop is bound by
-  // the issue of its scalar instructions (lane masks, control), so the common
-  // step - no lane's particle inside the cell - tests neither the leaf flag nor
-  // the mass (both are folded into the records, see write_trav).
-  while (v < nn) {
-    while (v == top) {  // leaving the subtrees remembered for position v
-      --depth;
-#pragma unroll
-      for (int q = 0; q < PPL; ++q)
-        M[q] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)keep_hi[q], depth) << 32) |
-               (uint32_t)__builtin_amdgcn_readlane((int)keep_lo[q], depth);
-      top = depth > 0 ? __builtin_amdgcn_readlane(keep_at, depth - 1) : INT_MAX;
-    }
-    // One 32-byte record plus (skip, (2h)^2), waited for together.
+  for (;;) {
+    // The compiler cannot prove the walk state uniform after a C step, nor the
+    // asm's scalar outputs: readfirstlane states it on both sides.
+    v = __builtin_amdgcn_readfirstlane(v);
+    top = __builtin_amdgcn_readfirstlane(top);
+    depth = __builtin_amdgcn_readfirstlane(depth);
+    M = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(M >> 32)) << 32) |
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)M);
+    float t0, t1, t2, dxa, dya, dza, d2a, inva;
+    asm volatile(NBX_WALK_FAST
+                 : [v] "+s"(v), [top] "+s"(top), [depth] "+s"(depth), [M] "+s"(M),
+                   [sx] "+v"(sx), [sy] "+v"(sy), [sz] "+v"(sz), [kat] "+v"(keep_at),
+                   [klo] "+v"(keep_lo), [khi] "+v"(keep_hi), [t0] "=&v"(t0), [t1] "=&v"(t1),
+                   [t2] "=&v"(t2), [dx] "=&v"(dxa), [dy] "=&v"(dya), [dz] "=&v"(dza),
+                   [d2] "=&v"(d2a), [inv] "=&v"(inva)
+                 : [rb] "s"(rec), [nn] "s"(nn), [eps2] "s"(eps2), [xi] "v"(xi), [yi] "v"(yi),
+                   [zi] "v"(zi)
+                 : "s20", "s21", "s22", "s23", "s24", "s25", "s26", "s27", "s28", "s29", "s30",
+                   "s31", "s32", "s33", "s34", "s35", "s36", "s37", "s38", "s39", "s40", "s41",
+                   "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52",
+                   "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63",
+                   "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74",
+                   "s75", "m0", "scc", "memory");
+    v = __builtin_amdgcn_readfirstlane(v);
+    top = __builtin_amdgcn_readfirstlane(top);
+    depth = __builtin_amdgcn_readfirstlane(depth);
+    M = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(M >> 32)) << 32) |
+        (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)M);
+    if (v >= nn) break;
+    // A step the asm leaves to C: record v (v != top here).
     const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 64u;
     const float4 c4 = *reinterpret_cast<const float4*>(rp);
     const float4 b4 = *reinterpret_cast<const float4*>(rp + 16);
-    const float2 a2 = *reinterpret_cast<const float2*>(rp + 32);
-    const int after = max(__float_as_int(a2.x), v + 1);
-    const float size2 = a2.y;
-    asm volatile("" ::"s"(c4.x), "s"(c4.y), "s"(c4.z), "s"(c4.w), "s"(b4.x), "s"(b4.y),
-                 "s"(b4.z), "s"(b4.w), "s"(after), "s"(size2));
+    const float4 a4 = *reinterpret_cast<const float4*>(rp + 32);
+    const float mass = reinterpret_cast<const float4*>(rp + 48)->x;
+    const int after = max(__float_as_int(a4.x), v + 1);
+    const float size2 = a4.y, s_open = a4.z, s_acc = a4.w;
     const float h = fabsf(b4.w);
-    uint64_t om[PPL];
-    uint64_t any = 0;
-#pragma unroll
-    for (int q = 0; q < PPL; ++q) {
-      // Lane sets as 64-bit masks: each per-lane test is one compare, and the
-      // sets combine with a few scalar operations.
-      const float far =
-          fmaxf(fmaxf(fabsf(xi[q] - b4.x), fabsf(yi[q] - b4.y)), fabsf(zi[q] - b4.z));
-      const uint64_t cm = __ballot(far <= h) & M[q];  // :180-182, particle inside the cell
-      float dx = sub_rn(c4.x, xi[q]), dy = sub_rn(c4.y, yi[q]), dz = sub_rn(c4.z, zi[q]);
-      float gm = mul_rn(G, c4.w);
-      uint64_t o = 0;     // lanes that open the node
-      uint64_t gone = 0;  // lanes whose own leaf holds no other mass
-      if (cm) {
-        if (__builtin_signbit(b4.w)) {  // :190-200, the particle's own leaf
-          const bool own = (cm & me) != 0;
-          const float meff = own ? sub_rn(c4.w, mi[q]) : c4.w;
-          const bool fix = own && meff > 0.0f;
-          if (__ballot(fix)) {  // remove the particle from a leaf that holds others
-            const float cx = div_rn(sub_rn(mul_rn(c4.x, c4.w), mul_rn(xi[q], mi[q])), meff);
-            const float cy = div_rn(sub_rn(mul_rn(c4.y, c4.w), mul_rn(yi[q], mi[q])), meff);
-            const float cz = div_rn(sub_rn(mul_rn(c4.z, c4.w), mul_rn(zi[q], mi[q])), meff);
-            dx = fix ? sub_rn(cx, xi[q]) : dx;
-            dy = fix ? sub_rn(cy, yi[q]) : dy;
-            dz = fix ? sub_rn(cz, zi[q]) : dz;
-          }
-          gm = own ? mul_rn(G, meff) : gm;
-          gone = __ballot(meff <= 0.0f) & cm;
-        } else {
-          o = cm;  // :222, an internal node holding the particle is always opened
+    const float far = fmaxf(fmaxf(fabsf(xi - b4.x), fabsf(yi - b4.y)), fabsf(zi - b4.z));
+    const uint64_t cm = __ballot(far <= h) & M;  // :180-182, particle inside the cell
+    float dx = sub_rn(c4.x, xi), dy = sub_rn(c4.y, yi), dz = sub_rn(c4.z, zi);
+    float gm = c4.w;
+    uint64_t o = 0;     // lanes that open the node
+    uint64_t gone = 0;  // lanes whose own leaf holds no other mass
+    if (cm) {
+      if (__builtin_signbit(b4.w)) {  // :190-200, the particle's own leaf
+        const bool own = (cm & me) != 0;
+        const float meff = own ? sub_rn(mass, mi) : mass;
+        const bool fix = own && meff > 0.0f;
+        if (__ballot(fix)) {  // remove the particle from a leaf that holds others
+          const float cx = div_rn(sub_rn(mul_rn(c4.x, mass), mul_rn(xi, mi)), meff);
+          const float cy = div_rn(sub_rn(mul_rn(c4.y, mass), mul_rn(yi, mi)), meff);
+          const float cz = div_rn(sub_rn(mul_rn(c4.z, mass), mul_rn(zi, mi)), meff);
+          dx = fix ? sub_rn(cx, xi) : dx;
+          dy = fix ? sub_rn(cy, yi) : dy;
+          dz = fix ? sub_rn(cz, zi) : dz;
         }
+        gm = own ? mul_rn(G, meff) : gm;
+        gone = __ballot(meff <= 0.0f) & cm;
+      } else {
+        o = cm;  // :222, an internal node holding the particle is always opened
       }
-      const float d2 =
-          add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
-      // :218-224, open when contains || !(size2 / d2 < theta2): size2 against
-      // d2 * theta2 (1 -+ 2^-20) decides outside a narrow band (each product is
-      // within half an ulp); inside the band the IEEE division decides.
-      o |= __ballot(size2 > d2 * theta2_hi);
-      const uint64_t am = M[q] & ~(o | __ballot(size2 < d2 * theta2_lo));
-      if (am) {
-        const bool mine = (am & me) != 0;
-        o |= __ballot(mine && !(div_rn(size2, d2) < theta2)) & am;
-      }
-      om[q] = o & M[q];
-      const uint64_t im = M[q] & ~o & ~gone;  // lanes that take the node's force
-      const float inv = __builtin_amdgcn_rsqf(d2);
-      const float fi = gm * (inv * inv * inv);
-      float f;  // fi on the lanes of im, else 0: one select on the lane mask
-      asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(f) : "v"(fi), "s"(im));
-      sx[q] = __builtin_fmaf(f, dx, sx[q]);
-      sy[q] = __builtin_fmaf(f, dy, sy[q]);
-      sz[q] = __builtin_fmaf(f, dz, sz[q]);
-      any |= om[q];
     }
-    if (any) {  // :234-240, descend; remember where this subtree ends
-      const bool mine = lane == depth;
-      keep_at = mine ? after : keep_at;
-#pragma unroll
-      for (int q = 0; q < PPL; ++q) {
-        keep_lo[q] = mine ? (uint32_t)M[q] : keep_lo[q];
-        keep_hi[q] = mine ? (uint32_t)(M[q] >> 32) : keep_hi[q];
-        M[q] = om[q];
-      }
+    const float d2 = add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
+    // :218-224, open when contains || !(size2 / d2 < theta2): the thresholds
+    // decide outside a narrow band (trav_prep); inside it the IEEE division.
+    o |= __ballot(d2 < s_open);
+    const uint64_t am = M & ~(o | __ballot(d2 > s_acc));
+    if (am) {
+      const bool mine = (am & me) != 0;
+      o |= __ballot(mine && !(div_rn(size2, d2) < theta2)) & am;
+    }
+    const uint64_t om = o & M;
+    const uint64_t im = M & ~o & ~gone;  // lanes that take the node's force
+    const float inv = __builtin_amdgcn_rsqf(d2);
+    const float fi = gm * (inv * inv * inv);
+    const float f = ((im >> lane) & 1) ? fi : 0.0f;
+    sx = __builtin_fmaf(f, dx, sx);
+    sy = __builtin_fmaf(f, dy, sy);
+    sz = __builtin_fmaf(f, dz, sz);
+    if (om) {  // :234-240, descend; remember where this subtree ends
+      // writelane takes its lane from m0 (one scalar operand per vector
+      // instruction); the s_nop separates the m0 write from its first read.
+      asm volatile(
+          "s_mov_b32 m0, %[d]\n"
+          "s_nop 0\n"
+          "v_writelane_b32 %[kat], %[aft], m0\n"
+          "v_writelane_b32 %[klo], %[mlo], m0\n"
+          "v_writelane_b32 %[khi], %[mhi], m0\n"
+          : [kat] "+v"(keep_at), [klo] "+v"(keep_lo), [khi] "+v"(keep_hi)
+          : [aft] "s"(__builtin_amdgcn_readfirstlane(after)),
+            [mlo] "s"(__builtin_amdgcn_readfirstlane((int)(uint32_t)M)),
+            [mhi] "s"(__builtin_amdgcn_readfirstlane((int)(uint32_t)(M >> 32))),
+            [d] "s"(__builtin_amdgcn_readfirstlane(depth))
+          : "m0");
       ++depth;
       top = after;
+      M = om;
       v = v + 1;
     } else {
       v = after;
     }
   }
-#pragma unroll
-  for (int q = 0; q < PPL; ++q)
-    if (valid[q]) {
-      ax[ip[q]] = sx[q];
-      ay[ip[q]] = sy[q];
-      az[ip[q]] = sz[q];
-    }
+  if (valid) {
+    ax[ip] = sx;
+    ay[ip] = sy;
+    az[ip] = sz;
+  }
 }
 
 }  // namespace
@@ -1263,9 +1401,13 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   hipStream_t s = as_stream(stream);
   const int n = (int)d->count;
   mark(t, 8, s);
-  hipLaunchKernelGGL(bh_traverse<kTravPPL>, dim3(div_up(n, kThreads * kTravPPL)), dim3(kThreads),
+  const float theta2 = theta * theta;
+  hipLaunchKernelGGL(trav_prep, dim3(grid_for((size_t)t->max_nodes, 2048)), dim3(kThreads), 0, s,
+                     t->trav_rec, t->st, 1.0f / theta2, G);
+  NBX_LAUNCHED();
+  hipLaunchKernelGGL(bh_traverse, dim3(div_up(n, kThreads)), dim3(kThreads),
                      0, s, t->trav_rec, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z,
-                     d->mass, n, theta * theta, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
+                     d->mass, n, theta2, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   mark(t, 9, s);
   return NBX_OK;

@@ -121,6 +121,99 @@ def test_forces_degenerate_inputs(nbx, orc):
     assert np.all(d.acc() == 0)
 
 
+def theta_on_boundary(h, nodes, i, eps, rng):
+    """A theta with fl(theta * theta) == fl(size2 / d2) for an internal cell that
+    particle i's walk must visit (a sibling of one of i's ancestors): the
+    decision size2 / d2 < theta2 (force_barnes_hut.cu:218-224) is then false by
+    equality, and true for the next float theta. Values are computed in float32
+    in the traversal's operation order."""
+    f = np.float32
+    x = np.array([h["pos_x"][i], h["pos_y"][i], h["pos_z"][i]], f)
+    eps2 = f(eps) * f(eps)
+    cand, v = [], 0
+    while not nodes["is_leaf"][v]:
+        nxt = None
+        for c in nodes["children"][v]:
+            if c < 0:
+                continue
+            if np.all(np.abs(x - nodes["center"][c]) <= nodes["half_size"][c]):
+                nxt = c if nxt is None else nxt
+            elif not nodes["is_leaf"][c] and nodes["total_mass"][c] > 0:
+                cand.append(int(c))
+        if nxt is None:
+            break
+        v = nxt
+    rng.shuffle(cand)
+    for c in cand:
+        dd = (nodes["com"][c] - x).astype(f)
+        d2 = f(f(f(dd[0] * dd[0]) + f(dd[1] * dd[1])) + f(dd[2] * dd[2])) + eps2
+        hh = nodes["half_size"][c]
+        r = f(f(f(4.0) * hh) * hh / d2)
+        if not 0.04 < r < 1.0:
+            continue
+        t = f(np.sqrt(r))
+        for _ in range(4):
+            t = np.nextafter(t, f(0))
+        for _ in range(9):
+            if f(t * t) == r:
+                return t, c
+            t = np.nextafter(t, f(1))
+    return None, None
+
+
+def test_theta_decisions_on_the_boundary(nbx, orc):
+    # A decision exactly on the boundary goes through the division in the walk
+    # (the thresholds S_open/S_acc leave it undecided); one float up, the
+    # opposite decision. Cases are kept where the flip moves the oracle's force
+    # on particle i by > 1e-3 relative, 50x the tolerance of the comparison.
+    h = nbx.gen_plummer(3000, seed=21)
+    ref = orc.bh_pipeline(h)["nodes"]
+    rng = np.random.default_rng(5)
+    found = 0
+    for i in rng.choice(3000, 400, replace=False):
+        th, c = theta_on_boundary(h, ref, int(i), 0.01, rng)
+        if th is None:
+            continue
+        thetas = (float(th), float(np.nextafter(th, np.float32(1))))
+        want = [orc.bh_forces(ref, h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], theta, 1.0,
+                              0.01, targets=np.array([i])).astype(np.float64)[0]
+                for theta in thetas]
+        if np.linalg.norm(want[0] - want[1]) < 1e-3 * np.linalg.norm(want[0]):
+            continue
+        for theta, a in zip(thetas, want):
+            d, t = gpu_build(nbx, h)
+            t.forces(d, theta, 1.0, 0.01)
+            nbx.synchronize()
+            g = d.acc()[i].astype(np.float64)
+            assert np.linalg.norm(g - a) <= 2e-5 * np.linalg.norm(a), (i, c, theta)
+        found += 1
+        if found == 3:
+            break
+    assert found == 3
+
+
+def test_force_calls_reuse_one_build(nbx, orc):
+    # Per-call thresholds and G * m are rewritten on every force call: the same
+    # build gives the same accelerations as a fresh one for any (theta, G) order.
+    h = nbx.gen_plummer(5000, seed=11)
+    d, t = gpu_build(nbx, h)
+
+    def acc(tree, theta, G):
+        tree.forces(d, theta, G, 0.01)
+        nbx.synchronize()
+        return d.acc().copy()
+
+    a1 = acc(t, 0.5, 1.0)
+    a2 = acc(t, 1.0, 1.0)
+    a3 = acc(t, 0.5, 1.0)
+    a4 = acc(t, 0.5, 2.0)
+    assert np.array_equal(a1.view(np.uint32), a3.view(np.uint32))
+    _, t2 = gpu_build(nbx, h)
+    assert np.array_equal(acc(t2, 1.0, 1.0).view(np.uint32), a2.view(np.uint32))
+    # G = 2 scales every rounded product by a power of two: exactly twice
+    assert np.array_equal(a4.view(np.uint32), (2 * a1).view(np.uint32))
+
+
 def test_theta_zero_equals_direct(nbx, orc):
     h = nbx.gen_plummer(2000, seed=8)
     d, t = gpu_build(nbx, h)
