@@ -623,30 +623,48 @@ __global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict_
 // finite inverse cube).
 constexpr float kFarPos = 1e18f;
 constexpr float kNeverOpen = -1.0f;
+
+// The per-call fields of a record for one (theta, G): G * mass and the opening
+// thresholds S_open/S_acc (their bounds are stated at trav_prep). The layout
+// passes write them for the (theta, G) of the last force call, so that a force
+// call with the same values needs no trav_prep pass.
+struct TravParams {
+  float inv_theta2, G;
+  int valid;
+};
+__device__ __forceinline__ void trav_fields(float size2, float mass, float inv_theta2, float G,
+                                            float* gm, float* s_open, float* s_acc) {
+  const float sc = mul_rn(size2, inv_theta2);
+  *gm = mul_rn(G, mass);
+  *s_open = mul_rn(sc, 1.0f - 0x1p-20f);
+  *s_acc = mul_rn(sc, 1.0f + 0x1p-20f);
+}
+
 __device__ __forceinline__ void write_trav(float4* rec, int p, int size, float4 com, float4 box,
-                                           bool leaf) {
-  if (com.w == 0.0f) {
-    rec[4 * p] = make_float4(kFarPos, kFarPos, kFarPos, 0.0f);
+                                           bool leaf, TravParams tp) {
+  const bool empty = com.w == 0.0f;
+  const float size2 = (leaf || empty) ? kNeverOpen : mul_rn(mul_rn(4.0f, box.w), box.w);
+  float gm = com.w, s_open = 0.0f, s_acc = 0.0f;
+  if (tp.valid) trav_fields(size2, com.w, tp.inv_theta2, tp.G, &gm, &s_open, &s_acc);
+  if (empty) {
+    rec[4 * p] = make_float4(kFarPos, kFarPos, kFarPos, gm);
     rec[4 * p + 1] = make_float4(kFarPos, kFarPos, kFarPos, leaf ? -0.0f : 0.0f);
-    rec[4 * p + 2] = make_float4(__int_as_float(p + size), kNeverOpen, 0.0f, 0.0f);
-    rec[4 * p + 3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    return;
+  } else {
+    rec[4 * p] = make_float4(com.x, com.y, com.z, gm);
+    rec[4 * p + 1] = make_float4(box.x, box.y, box.z, leaf ? -box.w - 0.0f : box.w);
   }
-  rec[4 * p] = com;
-  rec[4 * p + 1] = make_float4(box.x, box.y, box.z, leaf ? -box.w - 0.0f : box.w);
-  rec[4 * p + 2] = make_float4(__int_as_float(p + size),
-                               leaf ? kNeverOpen : mul_rn(mul_rn(4.0f, box.w), box.w), 0.0f, 0.0f);
+  rec[4 * p + 2] = make_float4(__int_as_float(p + size), size2, s_open, s_acc);
   rec[4 * p + 3] = make_float4(com.w, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict__ st, TmpNodes T,
-                                                      int d, float4* rec) {
+                                                      int d, float4* rec, TravParams tp) {
   if (st->use_seq) return;
   const int b = st->level_start[d], e = st->level_start[d + 1];
   for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
     const int pt = d == 0 ? 0 : T.pre[t];
     const bool leaf = (T.info[t] >> 11) & 1u;
-    write_trav(rec, pt, T.size[t], T.com[t], T.box[t], leaf);
+    write_trav(rec, pt, T.size[t], T.com[t], T.box[t], leaf, tp);
     if (leaf) continue;
     int next = pt + 1;
     for (int o = 7; o >= 0; --o) {
@@ -839,7 +857,8 @@ __global__ void seq_build(BhState* st, const int32_t* __restrict__ sidx, const f
 // visiting order, then sizes the subtrees in reverse order (a node's children
 // always come after it). pre_of, size and order are scratch of max_nodes ints.
 __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __restrict__ nodes,
-                             int32_t* pre_of, int32_t* size, int32_t* order, float4* rec) {
+                             int32_t* pre_of, int32_t* size, int32_t* order, float4* rec,
+                             TravParams tp) {
   if (!st->use_seq || threadIdx.x != 0 || blockIdx.x != 0) return;
   const int nn = st->final_count;
   if (nn <= 0) return;
@@ -865,7 +884,7 @@ __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __re
         if (q.child[o] >= 0) sz += size[q.child[o]];
     size[v] = sz;
     write_trav(rec, p, sz, make_float4(q.comx, q.comy, q.comz, q.mass),
-               make_float4(q.cx, q.cy, q.cz, q.half), q.leaf);
+               make_float4(q.cx, q.cy, q.cz, q.half), q.leaf, tp);
   }
 }
 
@@ -885,12 +904,11 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
                                                       float inv_theta2, float G) {
   const int nn = st->final_count;
   for (int p = blockIdx.x * kThreads + threadIdx.x; p < nn; p += gridDim.x * kThreads) {
-    const float mass = rec[4 * p + 3].x;
-    const float size2 = rec[4 * p + 2].y;
-    const float sc = mul_rn(size2, inv_theta2);
-    rec[4 * p].w = mul_rn(G, mass);
-    rec[4 * p + 2].z = mul_rn(sc, 1.0f - 0x1p-20f);
-    rec[4 * p + 2].w = mul_rn(sc, 1.0f + 0x1p-20f);
+    float gm, s_open, s_acc;
+    trav_fields(rec[4 * p + 2].y, rec[4 * p + 3].x, inv_theta2, G, &gm, &s_open, &s_acc);
+    rec[4 * p].w = gm;
+    rec[4 * p + 2].z = s_open;
+    rec[4 * p + 2].w = s_acc;
   }
 }
 
@@ -1193,6 +1211,9 @@ struct nbx_bh {
   nbx::RadixSort sort_nodes;
   bool built = false;
   size_t built_n = 0;
+  // (1 / theta^2, G) the traversal records hold; set by the last trav_prep
+  nbx::TravParams prep{0.0f, 0.0f, 0};
+  bool nodes_pending = false;  // the reference node array is emitted on demand
   int force_seq = 0;
   bool timing = false;
   hipEvent_t ev[10] = {};
@@ -1378,14 +1399,14 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
     hipLaunchKernelGGL(com_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv);
   NBX_LAUNCHED();
   mark(t, 6, s);
-  hipLaunchKernelGGL(emit_nodes, dim3(gn), dim3(kThreads), 0, s, t->st, t->T, t->final_of,
-                     t->nodes);
+  t->nodes_pending = true;  // emit_nodes runs when the node array is asked for
   hipLaunchKernelGGL(seq_build, dim3(1), dim3(64), 0, s, t->st, t->idx, d->pos_x, d->pos_y,
                      d->pos_z, d->mass, t->nodes, t->max_nodes);
   for (int lv = 0; lv <= kMaxDepth; ++lv)
-    hipLaunchKernelGGL(pre_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv, t->trav_rec);
+    hipLaunchKernelGGL(pre_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv, t->trav_rec,
+                       t->prep);
   hipLaunchKernelGGL(seq_preorder, dim3(1), dim3(64), 0, s, t->st, t->nodes, t->T.pre, t->T.size,
-                     t->T.k, t->trav_rec);
+                     t->T.k, t->trav_rec, t->prep);
   NBX_LAUNCHED();
   mark(t, 7, s);
   t->built = true;
@@ -1402,9 +1423,13 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   const int n = (int)d->count;
   mark(t, 8, s);
   const float theta2 = theta * theta;
-  hipLaunchKernelGGL(trav_prep, dim3(grid_for((size_t)t->max_nodes, 2048)), dim3(kThreads), 0, s,
-                     t->trav_rec, t->st, 1.0f / theta2, G);
-  NBX_LAUNCHED();
+  const nbx::TravParams want{1.0f / theta2, G, 1};
+  if (!t->prep.valid || std::memcmp(&want, &t->prep, sizeof(want)) != 0) {
+    hipLaunchKernelGGL(trav_prep, dim3(grid_for((size_t)t->max_nodes, 2048)), dim3(kThreads), 0,
+                       s, t->trav_rec, t->st, want.inv_theta2, want.G);
+    NBX_LAUNCHED();
+    t->prep = want;
+  }
   hipLaunchKernelGGL(bh_traverse, dim3(div_up(n, kThreads)), dim3(kThreads),
                      0, s, t->trav_rec, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z,
                      d->mass, n, theta2, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
@@ -1434,6 +1459,13 @@ int nbx_bh_copy_nodes(nbx_bh* t, void* host_nodes, size_t capacity_nodes) {
   NBX_TRY(nbx_bh_node_count(t, &cnt, nullptr));
   if ((size_t)cnt > capacity_nodes)
     return fail(NBX_ERR_CAPACITY, "%d nodes do not fit in %zu", cnt, capacity_nodes);
+  if (t->nodes_pending) {
+    hipLaunchKernelGGL(emit_nodes, dim3(grid_for((size_t)t->max_nodes)), dim3(kThreads), 0, 0,
+                       t->st, t->T, t->final_of, t->nodes);
+    NBX_LAUNCHED();
+    NBX_HIP(hipDeviceSynchronize());
+    t->nodes_pending = false;
+  }
   if (cnt) NBX_HIP(hipMemcpy(host_nodes, t->nodes, sizeof(RefNode) * cnt, hipMemcpyDeviceToHost));
   return NBX_OK;
 }

@@ -193,8 +193,9 @@ def test_theta_decisions_on_the_boundary(nbx, orc):
 
 
 def test_force_calls_reuse_one_build(nbx, orc):
-    # Per-call thresholds and G * m are rewritten on every force call: the same
-    # build gives the same accelerations as a fresh one for any (theta, G) order.
+    # Per-call thresholds and G * m: rewritten when (theta, G) changes, written by
+    # the build for the last (theta, G). Any order of calls and rebuilds gives
+    # the accelerations of a fresh build.
     h = nbx.gen_plummer(5000, seed=11)
     d, t = gpu_build(nbx, h)
 
@@ -212,6 +213,11 @@ def test_force_calls_reuse_one_build(nbx, orc):
     assert np.array_equal(acc(t2, 1.0, 1.0).view(np.uint32), a2.view(np.uint32))
     # G = 2 scales every rounded product by a power of two: exactly twice
     assert np.array_equal(a4.view(np.uint32), (2 * a1).view(np.uint32))
+    # a rebuild writes the fields for the last (theta, G) itself
+    t.build(d)
+    assert np.array_equal(acc(t, 0.5, 2.0).view(np.uint32), a4.view(np.uint32))
+    t.build(d)
+    assert np.array_equal(acc(t, 0.5, 1.0).view(np.uint32), a1.view(np.uint32))
 
 
 def test_theta_zero_equals_direct(nbx, orc):
