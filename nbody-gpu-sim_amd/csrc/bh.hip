@@ -1,7 +1,7 @@
 // Barnes-Hut pipeline on the GPU, bit-exact with the reference's host build
-// (src/cuda/force_barnes_hut.cu). Every phase runs on the device and the
-// element counts of the level-synchronous build live in device memory, so a
-// build issues no host round trip.
+// (src/cuda/force_barnes_hut.cu). Every phase runs on the device and every
+// count the build needs lives in device memory, so a build issues no host
+// round trip.
 //
 //  1 bbox       exact per-axis min/max (:78-122, :266-306), then the Morton
 //               reciprocals (:313-320) and the root cell (:418-422) on device
@@ -9,17 +9,21 @@
 //  3 sort       stable LSD radix sort of (code, index)        (:324-328)
 //  4 paths      each particle's octant digit at depths 0..31, walking the
 //               float centre chain c +- h/2 exactly as allocateLeaf does (:375-379)
-//  5 tree       level-synchronous PR-octree build over the paths. The
-//               reference inserts particles one by one in sorted order; its
-//               node numbering follows from two ranks per cell: t1 = first
-//               inserted particle, t2 = second. A non-root cell X with parent
-//               P is created at insertion k(X) = max(t2(P), t1(X)); within one
-//               insertion the displaced particle's chain comes first by depth,
-//               then the new particle's leaf (t1 == k). Sorting the cells by
-//               (k, t1 == k, depth) reproduces the reference indices exactly.
+//  5 tree       the PR-octree from the particles' order by octant path: a
+//               second sort (by path), adjacent common prefixes, one scan. The
+//               cells come out numbered in the traversal's preorder, with
+//               boxes, counts and subtree ends (see section 5)
 //  6 com        bottom-up, children in octant order, IEEE float ops (:512-538);
 //               depth-32 cells holding several particles merge them in
 //               insertion order (:355-365, :490-507)
+//  numbering    on demand (nbx_bh_copy_nodes), the reference's node indices.
+//               The reference inserts particles one by one in sorted order; its
+//               numbering follows from two ranks per cell: t1 = first inserted
+//               particle, t2 = second. A non-root cell X with parent P is
+//               created at insertion k(X) = max(t2(P), t1(X)); within one
+//               insertion the displaced particle's chain comes first by depth,
+//               then the new particle's leaf (t1 == k). Sorting the cells by
+//               (k, t1 == k, depth) reproduces the reference indices exactly.
 //  7 traverse   theta-criterion traversal with the reference's acceptance test,
 //               self-exclusion rule and DFS order (:142-249), one shared stack
 //               per wave (lanes carry a 64-bit "still needs this subtree" mask),
@@ -66,19 +70,17 @@ struct BhState {
   float root_c[3];
   float root_h;
   int32_t n;
-  int32_t node_count;  // parallel builder: cells allocated
-  int32_t seg_count[2];
-  int32_t elem_count[2];
-  int32_t level_start[kMaxDepth + 2];
-  int32_t deepest;
-  int32_t multi_count;
-  int32_t member_count;
+  int32_t node_count;  // parallel builder: cells
+  int32_t deepest;      // deepest leaf
+  int32_t multi_count;  // depth-32 leaves holding several particles
+  int32_t run_count;    // runs of positions sharing the sorted path digits
+  int32_t span_depth;   // deepest cell that straddles a block of positions
   int32_t use_seq;
   int32_t final_count;  // node_count_ of the reference
   int32_t max_depth;    // max_depth_ of the reference
   int32_t budget_merges;
   int32_t depth_merges;
-  int32_t overflow;  // parallel builder ran past the node budget
+  int32_t overflow;  // a run too long to order in one workgroup
   int32_t sort_n;    // cells to number (0 when the sequential builder runs)
 };
 
@@ -102,11 +104,13 @@ __device__ __forceinline__ int octant_of(float px, float py, float pz, float cx,
   return (px >= cx ? 1 : 0) | (py >= cy ? 2 : 0) | (pz >= cz ? 4 : 0);
 }
 
-__device__ __forceinline__ int digit_at(const uint4* path, int r, int d) {
-  const uint4 p = path[r];
-  const int w = d / 10;
-  const uint32_t word = w == 0 ? p.x : (w == 1 ? p.y : (w == 2 ? p.z : p.w));
-  return (int)((word >> (3 * (d % 10))) & 7u);
+// Sort key of ten digits stored 3 bits each from bit 0: digit i becomes 7 - digit
+// at bits 3 (9 - i), so the first digit is the most significant.
+__device__ __forceinline__ uint32_t order_key10(uint32_t w) {
+  uint32_t k = 0;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) k |= ((~w >> (3 * i)) & 7u) << (3 * (9 - i));
+  return k;
 }
 
 // Exclusive block scan of one int per thread (kThreads threads); returns the
@@ -158,6 +162,11 @@ __global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int
   if (ext[2] > big) big = ext[2];
   st->root_h = add_rn(mul_rn(big, 0.5f), 0.001f);  // :422
   st->n = n;
+  st->deepest = 0;
+  st->multi_count = 0;
+  st->run_count = 0;
+  st->span_depth = 0;
+  st->overflow = 0;
 }
 
 // ---------------------------------------------------------------- 2: morton
@@ -179,16 +188,17 @@ __global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restric
 
 // ---------------------------------------------------------------- 4: paths
 
-// Also copies each particle to its sorted position as (x, y, z, m), so that the
-// level passes read a leaf's particle from near its neighbours' (ranks of one
-// cell are close) instead of from a random original index.
+// Also copies each particle to its rank as (x, y, z, m), and writes the keys of
+// the path sort: digits 10..19 (sorted first) and digits 0..9 (see section 5).
 __global__ __launch_bounds__(kThreads) void path_kernel(const float* __restrict__ x,
                                                         const float* __restrict__ y,
                                                         const float* __restrict__ z,
                                                         const float* __restrict__ m,
                                                         const int32_t* __restrict__ sidx, int n,
                                                         const BhState* __restrict__ st,
-                                                        uint4* path, float4* sbody) {
+                                                        uint4* path, float4* sbody,
+                                                        uint32_t* key, uint32_t* key_hi,
+                                                        int32_t* rank) {
   const int r = blockIdx.x * kThreads + threadIdx.x;
   if (r >= n) return;
   const int p = sidx[r];
@@ -206,407 +216,14 @@ __global__ __launch_bounds__(kThreads) void path_kernel(const float* __restrict_
     h = hs;
   }
   path[r] = make_uint4(w[0], w[1], w[2], w[3]);
+  key[r] = order_key10(w[1]);
+  key_hi[r] = order_key10(w[0]);
+  rank[r] = r;
 }
 
-// ---------------------------------------------------------------- 5: tree
-
-// Cells ("tmp nodes") of the parallel builder, indexed in allocation order.
-struct TmpNodes {
-  int32_t* t1;      // first inserted rank
-  int32_t* k;       // insertion step that creates the cell
-  int32_t* parent;
-  int32_t* count;   // particles in the cell
-  int32_t* pidx;    // leaf: particle index
-  uint32_t* info;   // depth | octant << 8 | leaf << 11
-  int32_t* child;   // [8] per cell
-  float4* box;      // centre, half size
-  float4* com;      // centre of mass, total mass
-  int32_t* member_base;  // depth-32 multi leaves: slot range start (else -1)
-  int32_t* member_fill;
-  int32_t* size;  // cells in the subtree (the cell included)
-  int32_t* pre;   // position in the traversal order
-};
-
-struct Levels {
-  int32_t* seg_node[2];
-  int32_t* seg_t1[2];
-  int32_t* seg_t2[2];
-  int32_t* seg_cnt[2];   // [8] per segment
-  int32_t* seg_min[2];   // [8] per segment
-  int32_t* seg_next[2];  // [8] per segment: next-level segment, -1, or -(2+tmp leaf)
-  int32_t* elem_rank[2];
-  int32_t* elem_seg[2];
-  int32_t* multi_list;   // tmp ids of depth-32 multi leaves
-  int32_t* members;      // their member ranks
-};
-
-__global__ void tree_init(BhState* st, TmpNodes T, Levels L, int max_nodes) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int n = st->n;
-  T.t1[0] = 0;
-  T.k[0] = -1;
-  T.parent[0] = -1;
-  T.count[0] = n;
-  T.pidx[0] = -1;
-  T.info[0] = 0;
-  for (int o = 0; o < 8; ++o) T.child[o] = -1;
-  T.box[0] = make_float4(st->root_c[0], st->root_c[1], st->root_c[2], st->root_h);
-  T.com[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-  T.member_base[0] = -1;
-  st->node_count = 1;
-  st->seg_count[0] = 1;
-  st->seg_count[1] = 0;
-  st->elem_count[0] = n;
-  st->elem_count[1] = 0;
-  st->level_start[0] = 0;
-  st->deepest = 0;
-  st->multi_count = 0;
-  st->member_count = 0;
-  st->use_seq = 0;
-  st->overflow = 0;
-  st->sort_n = 0;
-  L.seg_node[0][0] = 0;
-  L.seg_t1[0][0] = 0;
-  L.seg_t2[0][0] = INT_MAX;
-  for (int o = 0; o < 8; ++o) {
-    L.seg_cnt[0][o] = 0;
-    L.seg_min[0][o] = INT_MAX;
-  }
-  (void)max_nodes;
-}
-
-// Level d, pass 1: count each segment's particles per octant, first rank per
-// octant, and the segment's second-smallest rank. Device-scope atomics are
-// expensive on this part, so each workgroup takes a contiguous run of
-// elements (spatially coherent: few distinct cells), aggregates in an LDS hash
-// table, and flushes one global atomic per distinct key.
-constexpr int kHash = 2048;   // (segment, octant) slots per workgroup
-constexpr int kHashSeg = 1024;  // segment slots (second-smallest rank)
-
-__device__ __forceinline__ int lds_probe(int* keys, int mask, int key) {
-  unsigned h = ((unsigned)key * 2654435761u) >> 7;
-  for (int p = 0; p < 16; ++p) {
-    const int slot = (int)((h + (unsigned)p) & (unsigned)mask);
-    const int prev = atomicCAS(&keys[slot], -1, key);
-    if (prev == -1 || prev == key) return slot;
-  }
-  return -1;
-}
-
-__global__ __launch_bounds__(kThreads) void level_count(BhState* st, Levels L,
-                                                        const uint4* __restrict__ path, int d) {
-  const int cur = d & 1, nxt = cur ^ 1;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->level_start[d + 1] = st->node_count;
-    st->seg_count[nxt] = 0;
-    st->elem_count[nxt] = 0;
-  }
-  const int ns = st->overflow ? 0 : st->seg_count[cur];
-  const int ne = ns ? st->elem_count[cur] : 0;
-  const int chunk = (ne + (int)gridDim.x - 1) / (int)gridDim.x;
-  const int begin = (int)blockIdx.x * chunk;
-  const int end = min(ne, begin + chunk);
-  if (begin >= end) return;
-  __shared__ int hk[kHash], hc[kHash], hm[kHash];
-  __shared__ int sk[kHashSeg], s2[kHashSeg];
-  for (int k = threadIdx.x; k < kHash; k += kThreads) {
-    hk[k] = -1;
-    hc[k] = 0;
-    hm[k] = INT_MAX;
-  }
-  for (int k = threadIdx.x; k < kHashSeg; k += kThreads) {
-    sk[k] = -1;
-    s2[k] = INT_MAX;
-  }
-  __syncthreads();
-  for (int e = begin + threadIdx.x; e < end; e += kThreads) {
-    const int r = d == 0 ? e : L.elem_rank[cur][e];
-    const int s = d == 0 ? 0 : L.elem_seg[cur][e];
-    const int o = digit_at(path, r, d);
-    const int key = s * 8 + o;
-    const int slot = lds_probe(hk, kHash - 1, key);
-    if (slot >= 0) {
-      atomicAdd(&hc[slot], 1);
-      atomicMin(&hm[slot], r);
-    } else {
-      atomicAdd(&L.seg_cnt[cur][key], 1);
-      atomicMin(&L.seg_min[cur][key], r);
-    }
-    if (r != L.seg_t1[cur][s]) {
-      const int q = lds_probe(sk, kHashSeg - 1, s);
-      if (q >= 0) atomicMin(&s2[q], r);
-      else atomicMin(&L.seg_t2[cur][s], r);
-    }
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < kHash; k += kThreads) {
-    const int key = hk[k];
-    if (key >= 0) {
-      atomicAdd(&L.seg_cnt[cur][key], hc[k]);
-      atomicMin(&L.seg_min[cur][key], hm[k]);
-    }
-  }
-  for (int k = threadIdx.x; k < kHashSeg; k += kThreads)
-    if (sk[k] >= 0 && s2[k] != INT_MAX) atomicMin(&L.seg_t2[cur][sk[k]], s2[k]);
-}
-
-// Level d, pass 2: create the children of every segment (cells at depth d+1).
-// One thread per (segment, octant) slot: creates the child cell of that octant
-// when it holds particles, and the next level's segment when it holds two or
-// more (below depth 32). Cell ids come from a block scan plus one atomic per
-// block, so every level's cells are contiguous (level_start).
-__global__ __launch_bounds__(kThreads) void level_children(BhState* st, Levels L, TmpNodes T,
-                                                           const int32_t* __restrict__ sidx,
-                                                           const float4* __restrict__ sbody,
-                                                           int d, int max_nodes) {
-  const int cur = d & 1, nxt = cur ^ 1;
-  const int ns = st->overflow ? 0 : st->seg_count[cur];
-  const int cd = d + 1;  // child depth
-  const int nslots = ns * 8;
-  __shared__ int sh_node_base, sh_seg_base, sh_deep;
-  for (int base = blockIdx.x * kThreads; base < nslots; base += gridDim.x * kThreads) {
-    const int u = base + threadIdx.x;  // slot: segment u / 8, octant u % 8
-    const bool valid = u < nslots;
-    const int c = valid ? L.seg_cnt[cur][u] : 0;
-    int tot_child, tot_seg;
-    const int off_child = block_exclusive_scan(c > 0 ? 1 : 0, &tot_child);
-    const int off_seg = block_exclusive_scan((c >= 2 && cd < kMaxDepth) ? 1 : 0, &tot_seg);
-    if (threadIdx.x == 0) {
-      sh_node_base = tot_child ? atomicAdd(&st->node_count, tot_child) : 0;
-      sh_seg_base = tot_seg ? atomicAdd(&st->seg_count[nxt], tot_seg) : 0;
-      sh_deep = 0;
-    }
-    __syncthreads();
-    if (valid) {
-      const int id = sh_node_base + off_child;
-      if (c == 0) {
-        L.seg_next[cur][u] = -1;
-      } else if (id >= max_nodes) {  // past the budget: the sequential builder takes over
-        L.seg_next[cur][u] = -1;
-        st->overflow = 1;
-      } else {
-        const int s = u >> 3, o = u & 7;
-        const int X = L.seg_node[cur][s];
-        const float4 pb = T.box[X];
-        const float hs = mul_rn(pb.w, 0.5f);
-        const int t1 = L.seg_min[cur][u];
-        const bool leaf = (c == 1) || (cd == kMaxDepth);
-        T.t1[id] = t1;
-        T.k[id] = d == 0 ? t1 : max(L.seg_t2[cur][s], t1);
-        T.parent[id] = X;
-        T.count[id] = c;
-        T.info[id] = (uint32_t)cd | ((uint32_t)o << 8) | (leaf ? (1u << 11) : 0u);
-        T.box[id] = make_float4(add_rn(pb.x, (o & 1) ? hs : -hs), add_rn(pb.y, (o & 2) ? hs : -hs),
-                                add_rn(pb.z, (o & 4) ? hs : -hs), hs);
-        int4* ch = reinterpret_cast<int4*>(T.child + (size_t)id * 8);
-        ch[0] = make_int4(-1, -1, -1, -1);
-        ch[1] = make_int4(-1, -1, -1, -1);
-        T.child[X * 8 + o] = id;
-        T.member_base[id] = -1;
-        if (c == 1) {
-          T.pidx[id] = sidx[t1];
-          T.com[id] = sbody[t1];
-          L.seg_next[cur][u] = -1;
-        } else if (leaf) {  // depth-32 cell with several particles
-          T.pidx[id] = sidx[t1];
-          const int slot = atomicAdd(&st->multi_count, 1);
-          L.multi_list[slot] = id;
-          T.member_base[id] = atomicAdd(&st->member_count, c);
-          T.member_fill[id] = 0;
-          L.seg_next[cur][u] = -(2 + id);
-        } else {
-          const int sid = sh_seg_base + off_seg;
-          T.pidx[id] = -1;
-          L.seg_node[nxt][sid] = id;
-          L.seg_t1[nxt][sid] = t1;
-          L.seg_t2[nxt][sid] = INT_MAX;
-          int4* sc = reinterpret_cast<int4*>(L.seg_cnt[nxt] + (size_t)sid * 8);
-          int4* sm = reinterpret_cast<int4*>(L.seg_min[nxt] + (size_t)sid * 8);
-          sc[0] = make_int4(0, 0, 0, 0);
-          sc[1] = make_int4(0, 0, 0, 0);
-          sm[0] = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
-          sm[1] = make_int4(INT_MAX, INT_MAX, INT_MAX, INT_MAX);
-          L.seg_next[cur][u] = sid;
-        }
-        sh_deep = cd;  // benign race: every writer stores the same value
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && sh_deep) atomicMax(&st->deepest, sh_deep);
-    __syncthreads();
-  }
-}
-
-// Level d, pass 3: move each active particle to its child segment (or to its
-// depth-32 leaf's member list). One global atomic per workgroup reserves the
-// output range of its contiguous chunk.
-__global__ __launch_bounds__(kThreads) void level_route(BhState* st, Levels L, TmpNodes T,
-                                                        const uint4* __restrict__ path, int d) {
-  const int cur = d & 1, nxt = cur ^ 1;
-  const int ne = st->overflow ? 0 : st->elem_count[cur];
-  const int chunk = (ne + (int)gridDim.x - 1) / (int)gridDim.x;
-  const int begin = (int)blockIdx.x * chunk;
-  const int end = min(ne, begin + chunk);
-  if (begin >= end) return;
-  __shared__ int sh_base;
-  // pass A: how many of this chunk continue to the next level
-  int mine = 0;
-  for (int e = begin + threadIdx.x; e < end; e += kThreads) {
-    const int r = d == 0 ? e : L.elem_rank[cur][e];
-    const int s = d == 0 ? 0 : L.elem_seg[cur][e];
-    mine += L.seg_next[cur][s * 8 + digit_at(path, r, d)] >= 0;
-  }
-  int total;
-  int off = block_exclusive_scan(mine, &total);
-  if (threadIdx.x == 0) sh_base = total ? atomicAdd(&st->elem_count[nxt], total) : 0;
-  __syncthreads();
-  off += sh_base;
-  // pass B: write them (each thread revisits its own elements in the same order)
-  for (int e = begin + threadIdx.x; e < end; e += kThreads) {
-    const int r = d == 0 ? e : L.elem_rank[cur][e];
-    const int s = d == 0 ? 0 : L.elem_seg[cur][e];
-    const int t = L.seg_next[cur][s * 8 + digit_at(path, r, d)];
-    if (t >= 0) {
-      L.elem_rank[nxt][off] = r;
-      L.elem_seg[nxt][off] = t;
-      ++off;
-    } else if (t <= -2) {
-      const int leaf = -(t + 2);
-      const int slot = T.member_base[leaf] + atomicAdd(&T.member_fill[leaf], 1);
-      L.members[slot] = r;
-    }
-  }
-}
-
-__global__ void tree_levels_done(BhState* st, int max_nodes) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  st->level_start[kMaxDepth + 1] = st->node_count;
-  st->use_seq = (st->overflow || st->node_count + kMaxDepth + 1 >= max_nodes) ? 1 : 0;  // :449
-  st->sort_n = st->use_seq ? 0 : st->node_count;
-  st->final_count = st->node_count;
-  st->max_depth = st->multi_count > 0 ? kMaxDepth : st->deepest - 1;
-  st->budget_merges = 0;
-  st->depth_merges = 0;
-}
-
-// Depth-32 cells with several particles: merge them in insertion (rank) order.
-__global__ __launch_bounds__(64) void multi_leaf_merge(BhState* st, Levels L, TmpNodes T,
-                                                       const int32_t* __restrict__ sidx,
-                                                       const float* __restrict__ x,
-                                                       const float* __restrict__ y,
-                                                       const float* __restrict__ z,
-                                                       const float* __restrict__ m) {
-  if (st->use_seq) return;
-  const int nm = st->multi_count;
-  for (int q = blockIdx.x * 64 + threadIdx.x; q < nm; q += gridDim.x * 64) {
-    const int id = L.multi_list[q];
-    int32_t* mem = L.members + T.member_base[id];
-    const int c = T.count[id];
-    // heap sort of the member ranks (ascending)
-    for (int start = c / 2 - 1; start >= 0; --start) {
-      int root = start;
-      for (;;) {
-        int child = 2 * root + 1;
-        if (child >= c) break;
-        if (child + 1 < c && mem[child + 1] > mem[child]) ++child;
-        if (mem[root] >= mem[child]) break;
-        const int tmp = mem[root];
-        mem[root] = mem[child];
-        mem[child] = tmp;
-        root = child;
-      }
-    }
-    for (int end = c - 1; end > 0; --end) {
-      const int tmp = mem[0];
-      mem[0] = mem[end];
-      mem[end] = tmp;
-      int root = 0;
-      for (;;) {
-        int child = 2 * root + 1;
-        if (child >= end) break;
-        if (child + 1 < end && mem[child + 1] > mem[child]) ++child;
-        if (mem[root] >= mem[child]) break;
-        const int t2 = mem[root];
-        mem[root] = mem[child];
-        mem[child] = t2;
-        root = child;
-      }
-    }
-    int p = sidx[mem[0]];
-    float cx = x[p], cy = y[p], cz = z[p], tm = m[p];
-    for (int k = 1; k < c; ++k) {  // :355-365
-      p = sidx[mem[k]];
-      const float pm = m[p];
-      const float tot = add_rn(tm, pm);
-      cx = div_rn(add_rn(mul_rn(cx, tm), mul_rn(x[p], pm)), tot);
-      cy = div_rn(add_rn(mul_rn(cy, tm), mul_rn(y[p], pm)), tot);
-      cz = div_rn(add_rn(mul_rn(cz, tm), mul_rn(z[p], pm)), tot);
-      tm = tot;
-    }
-    T.com[id] = make_float4(cx, cy, cz, tm);
-  }
-}
-
-// Reference node-numbering key of every cell (root first).
-__global__ __launch_bounds__(kThreads) void node_keys(const BhState* __restrict__ st, TmpNodes T,
-                                                      uint32_t* keys, int32_t* vals) {
-  const int nn = st->sort_n;
-  for (int t = blockIdx.x * kThreads + threadIdx.x; t < nn; t += gridDim.x * kThreads) {
-    uint32_t key = 0;
-    if (t > 0) {
-      const uint32_t depth = T.info[t] & 0xFFu;
-      const uint32_t flag = T.t1[t] == T.k[t] ? 1u : 0u;
-      key = ((uint32_t)(T.k[t] + 1) << 7) | (flag << 6) | depth;
-    }
-    keys[t] = key;
-    vals[t] = t;
-  }
-}
-
-__global__ __launch_bounds__(kThreads) void invert_perm(const BhState* __restrict__ st,
-                                                        const int32_t* __restrict__ order,
-                                                        int32_t* final_of) {
-  const int nn = st->sort_n;
-  for (int f = blockIdx.x * kThreads + threadIdx.x; f < nn; f += gridDim.x * kThreads)
-    final_of[order[f]] = f;
-}
-
-// COM of internal cells at depth d from their children (:514-538).
-__global__ __launch_bounds__(kThreads) void com_level(const BhState* __restrict__ st, TmpNodes T,
-                                                      int d) {
-  if (st->use_seq) return;
-  const int b = st->level_start[d], e = st->level_start[d + 1];
-  for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
-    if (T.info[t] & (1u << 11)) {
-      T.size[t] = 1;
-      continue;
-    }
-    float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-    int size = 1;
-    for (int o = 0; o < 8; ++o) {
-      const int c = T.child[t * 8 + o];
-      if (c < 0) continue;
-      const float4 q = T.com[c];
-      tm = add_rn(tm, q.w);
-      sx = add_rn(sx, mul_rn(q.x, q.w));
-      sy = add_rn(sy, mul_rn(q.y, q.w));
-      sz = add_rn(sz, mul_rn(q.z, q.w));
-      size += T.size[c];
-    }
-    float4 r = make_float4(0.f, 0.f, 0.f, tm);
-    if (tm > 0.0f) {
-      r.x = div_rn(sx, tm);
-      r.y = div_rn(sy, tm);
-      r.z = div_rn(sz, tm);
-    }
-    T.com[t] = r;
-    T.size[t] = size;
-  }
-}
-
-// Traversal layout, one level per launch from the root down: cells in the
-// order the reference's stack walk visits them (:160-241: children pushed
-// 0..7, so popped 7..0), i.e. depth-first preorder with children 7 down to 0.
+// Traversal layout: cells in the order the reference's stack walk visits them
+// (:160-241: children pushed 0..7, so popped 7..0), i.e. depth-first preorder
+// with children 7 down to 0.
 // Record p (64 bytes, so that one scalar address reaches all of it):
 //   rec[4p]   = (centre of mass, G * mass; written by trav_prep per force call)
 //   rec[4p+1] = (cell centre, half size; negative for a leaf)
@@ -657,23 +274,553 @@ __device__ __forceinline__ void write_trav(float4* rec, int p, int size, float4 
   rec[4 * p + 3] = make_float4(com.w, 0.0f, 0.0f, 0.0f);
 }
 
-__global__ __launch_bounds__(kThreads) void pre_level(const BhState* __restrict__ st, TmpNodes T,
-                                                      int d, float4* rec, TravParams tp) {
+// ---------------------------------------------------------------- 5: tree
+
+// Cells of the PR-octree from the particles' order by octant path.
+//
+// Sort the particles by their paths, comparing octants as 7 - o (ascending
+// order then lists the children of a cell 7 down to 0, the order in which the
+// reference's stack walk visits them, :160-241), ties by insertion rank. Every
+// cell is then a run of consecutive positions. With lcp[j] the number of
+// leading digits position j shares with position j - 1 (lcp[0] = lcp[n] = 0):
+//   - position j is the first particle of exactly the cells at depths
+//     lcp[j] + 1 .. L[j], L[j] = min(32, max(lcp[j], lcp[j + 1]) + 1), the
+//     deepest being its leaf (a depth-32 leaf holds every particle with the
+//     same 32 digits); the root, depth 0, is cell 0;
+//   - numbering these cells position by position, parent before child, gives
+//     the depth-first preorder of the tree with children 7..0:
+//     pre(j, d) = first_pre[j] + d - lcp[j] - 1, first_pre = 1 + the exclusive
+//     scan of L[j] - lcp[j];
+//   - a cell whose run ends before position e has its subtree end (the
+//     traversal's skip) at first_pre[e], and holds e - j particles.
+// So the cells, their boxes and the whole traversal layout come from one sort,
+// one scan and one pass per position. The centres of mass follow bottom-up:
+// each workgroup does the cells inside its block of positions level by level,
+// then one workgroup does the few cells that straddle block boundaries.
+//
+// The path sort is a stable LSD radix sort over digits 0..19 (two 30-bit
+// keys). Particles whose first 20 digits agree, rare outside exact or
+// near-coincident duplicates, form runs that one workgroup per run orders by
+// the remaining 12 digits and rank. A run longer than kRefineMax sends the
+// build to the exact sequential builder.
+
+// Cells ("tmp nodes"), indexed by preorder position.
+struct TmpNodes {
+  int32_t* t1;      // first inserted rank in the cell
+  int32_t* t2;      // second inserted rank (INT_MAX if none)
+  int32_t* k;       // insertion step that creates the cell (written by its parent)
+  int32_t* count;   // particles in the cell
+  int32_t* pidx;    // leaf: particle index of its first particle; else -1
+  uint32_t* info;   // depth | octant << 8 | leaf << 11
+  int32_t* child;   // [8] per internal cell, -1 where absent
+  float4* box;      // centre, half size
+  float4* com;      // centre of mass, total mass
+  int32_t* skip;    // preorder position after the cell's subtree
+  int32_t* pre;     // sequential builder: scratch
+  int32_t* size;    // sequential builder: scratch
+};
+
+// Per-position arrays of the path-order build.
+struct PathOrder {
+  uint32_t* key;      // sort keys (in place)
+  uint32_t* key_hi;   // digits 0..9 key of every rank
+  int32_t* rank;      // ranks in path order (in place)
+  uint4* spath;       // paths in path order
+  uint8_t* lcp;       // n + 1 entries
+  int32_t* first_pre; // n + 1 entries
+  int32_t* bsum;      // per block of kPosBlock positions: cells, then their offsets
+  int32_t* span;      // per block: [kMaxDepth + 1] straddling cells by depth, or -1
+  int32_t* runs;      // starts of runs sharing 20 digits
+};
+
+constexpr int kPosBlock = 2048;  // positions per workgroup in the cell passes
+constexpr int kPosRounds = kPosBlock / kThreads;
+constexpr int kRefineMax = 1024;
+constexpr int kSortDigits = 20;
+
+// Leading octant digits shared by two paths (0..32).
+__device__ __forceinline__ int path_lcp(uint4 a, uint4 b) {
+  uint32_t x = a.x ^ b.x;
+  if (x) return __builtin_ctz(x) / 3;
+  x = a.y ^ b.y;
+  if (x) return 10 + __builtin_ctz(x) / 3;
+  x = a.z ^ b.z;
+  if (x) return 20 + __builtin_ctz(x) / 3;
+  x = a.w ^ b.w;
+  if (x) return 30 + __builtin_ctz(x) / 3;
+  return kMaxDepth;
+}
+
+__device__ __forceinline__ int path_digit(uint4 p, int d) {
+  const uint32_t w = d < 10 ? p.x : (d < 20 ? p.y : (d < 30 ? p.z : p.w));
+  return (int)((w >> (3 * (d % 10))) & 7u);
+}
+
+// Cells started by position j (see above); lcp_next = lcp[j + 1] (0 past the end).
+__device__ __forceinline__ int leaf_depth(int l0, int l1) { return min(kMaxDepth, max(l0, l1) + 1); }
+
+__global__ __launch_bounds__(kThreads) void path_gather_key(const int32_t* __restrict__ n_dev,
+                                                            PathOrder O) {
+  const int n = *n_dev;
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n) O.key[i] = O.key_hi[O.rank[i]];
+}
+
+// lcp of every position with its predecessor, the paths in path order, and
+// the starts of runs that share their first kSortDigits digits.
+__global__ __launch_bounds__(kThreads) void path_lcp_kernel(BhState* st,
+                                                            const uint4* __restrict__ path,
+                                                            PathOrder O) {
+  const int n = st->n;
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j > n) return;
+  if (j == n) {
+    O.lcp[n] = 0;
+    return;
+  }
+  const uint4 a = path[O.rank[j]];
+  O.spath[j] = a;
+  int l = 0;
+  if (j > 0) {
+    const uint4 b = path[O.rank[j - 1]];
+    l = path_lcp(a, b);
+    if (l >= kSortDigits && (j == 1 || path_lcp(b, path[O.rank[j - 2]]) < kSortDigits))
+      O.runs[atomicAdd(&st->run_count, 1)] = j - 1;
+  }
+  O.lcp[j] = (uint8_t)l;
+}
+
+// Orders each run of positions sharing kSortDigits digits by its remaining
+// digits, then rank (the order a full stable path sort gives), and redoes the
+// run's lcp values. Bitonic sort in LDS of (digits 20..31 key << 25 | rank).
+__global__ __launch_bounds__(kThreads) void path_refine(BhState* st,
+                                                        const uint4* __restrict__ path,
+                                                        PathOrder O) {
+  __shared__ unsigned long long key[kRefineMax];
+  __shared__ int sh_end;
+  const int nr = st->run_count;
+  const int n = st->n;
+  for (int g = blockIdx.x; g < nr; g += gridDim.x) {
+    const int s = O.runs[g];
+    if (threadIdx.x == 0) sh_end = INT_MAX;
+    __syncthreads();
+    for (int base = s + 1;; base += kThreads) {
+      const int x = base + (int)threadIdx.x;
+      if (x >= n || O.lcp[x] < kSortDigits) atomicMin(&sh_end, min(x, n));
+      __syncthreads();
+      if (sh_end != INT_MAX) break;
+      __syncthreads();
+    }
+    const int m = sh_end - s;
+    if (m > kRefineMax) {
+      if (threadIdx.x == 0) st->overflow = 1;
+      __syncthreads();
+      continue;
+    }
+    int p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    for (int i = threadIdx.x; i < p2; i += kThreads) {
+      unsigned long long k = ~0ull;
+      if (i < m) {
+        const int r = O.rank[s + i];
+        const uint4 p = path[r];
+        const uint32_t lo = ((~p.w & 7u) << 3) | ((~p.w >> 3) & 7u);
+        k = ((((unsigned long long)order_key10(p.z) << 6) | lo) << 25) | (unsigned)r;
+      }
+      key[i] = k;
+    }
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1)
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int i = threadIdx.x; i < p2; i += kThreads) {
+          const int q = i ^ stride;
+          if (q > i) {
+            const bool up = (i & size) == 0;
+            const unsigned long long u = key[i], v = key[q];
+            if ((u > v) == up) {
+              key[i] = v;
+              key[q] = u;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    for (int i = threadIdx.x; i < m; i += kThreads) {
+      const int r = (int)(key[i] & 0x1FFFFFFull);
+      O.rank[s + i] = r;
+      O.spath[s + i] = path[r];
+    }
+    __syncthreads();
+    for (int i = 1 + (int)threadIdx.x; i < m; i += kThreads)
+      O.lcp[s + i] = (uint8_t)path_lcp(O.spath[s + i - 1], O.spath[s + i]);
+    __syncthreads();
+  }
+}
+
+// Cells per block of positions; the deepest leaf and the depth-32 leaves that
+// hold several particles.
+__global__ __launch_bounds__(kThreads) void cell_count(BhState* st, PathOrder O) {
+  __shared__ int sh_sum[kThreads / 64];
+  const int n = st->n;
+  const int a = blockIdx.x * kPosBlock;
+  int sum = 0, deep = 0, multi = 0;
+  for (int i = 0; i < kPosRounds; ++i) {
+    const int j = a + i * kThreads + (int)threadIdx.x;
+    if (j >= n) break;
+    const int l0 = O.lcp[j], l1 = O.lcp[j + 1];
+    const int L = leaf_depth(l0, l1);
+    sum += L - l0;
+    deep = max(deep, L);
+    multi += (l0 < kMaxDepth && l1 == kMaxDepth) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sum += __shfl_xor(sum, off, 64);
+    deep = max(deep, __shfl_xor(deep, off, 64));
+    multi += __shfl_xor(multi, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh_sum[threadIdx.x >> 6] = sum;
+    if (deep) atomicMax(&st->deepest, deep);
+    if (multi) atomicAdd(&st->multi_count, multi);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kThreads / 64; ++w) t += sh_sum[w];
+    O.bsum[blockIdx.x] = t;
+  }
+}
+
+// One workgroup: block offsets, the cell count, the builder choice, the root.
+__global__ __launch_bounds__(1024) void cell_offsets(BhState* st, PathOrder O, TmpNodes T,
+                                                     int max_nodes) {
+  __shared__ int wsum[16];
+  __shared__ int sh_carry;
+  const int n = st->n;
+  const int nb = (n + kPosBlock - 1) / kPosBlock;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) sh_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    const int i = base + (int)threadIdx.x;
+    const int v = i < nb ? O.bsum[i] : 0;
+    int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int before = sh_carry, tot = 0;
+    for (int q = 0; q < 16; ++q) {
+      if (q < w) before += wsum[q];
+      tot += wsum[q];
+    }
+    if (i < nb) O.bsum[i] = before + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 0) sh_carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const int cells = 1 + sh_carry;
+  const int seq = (st->overflow || cells + kMaxDepth + 1 >= max_nodes) ? 1 : 0;  // :449
+  st->node_count = cells;
+  st->final_count = cells;
+  st->use_seq = seq;
+  st->sort_n = seq ? 0 : cells;
+  st->max_depth = st->multi_count > 0 ? kMaxDepth : st->deepest - 1;
+  st->budget_merges = 0;
+  st->depth_merges = 0;
+  O.first_pre[n] = cells;
+  if (seq) return;
+  T.box[0] = make_float4(st->root_c[0], st->root_c[1], st->root_c[2], st->root_h);
+  T.info[0] = 0;
+  T.count[0] = n;
+  T.pidx[0] = -1;
+  T.skip[0] = cells;
+}
+
+// first_pre[j] for every position.
+__global__ __launch_bounds__(kThreads) void cell_first_pre(const BhState* __restrict__ st,
+                                                           PathOrder O) {
   if (st->use_seq) return;
-  const int b = st->level_start[d], e = st->level_start[d + 1];
-  for (int t = b + blockIdx.x * kThreads + threadIdx.x; t < e; t += gridDim.x * kThreads) {
-    const int pt = d == 0 ? 0 : T.pre[t];
-    const bool leaf = (T.info[t] >> 11) & 1u;
-    write_trav(rec, pt, T.size[t], T.com[t], T.box[t], leaf, tp);
-    if (leaf) continue;
-    int next = pt + 1;
-    for (int o = 7; o >= 0; --o) {
-      const int c = T.child[t * 8 + o];
-      if (c < 0) continue;
-      T.pre[c] = next;
-      next += T.size[c];
+  const int n = st->n;
+  const int a = blockIdx.x * kPosBlock;
+  int base = 1 + O.bsum[blockIdx.x];
+  for (int i = 0; i < kPosRounds; ++i) {
+    const int j = a + i * kThreads + (int)threadIdx.x;
+    int c = 0;
+    if (j < n) {
+      const int l0 = O.lcp[j];
+      c = leaf_depth(l0, O.lcp[j + 1]) - l0;
+    }
+    int tot;
+    const int off = block_exclusive_scan(c, &tot);
+    if (j < n) O.first_pre[j] = base + off;
+    base += tot;
+  }
+}
+
+// Every cell a position starts: box (the reference's centre chain, :375-379),
+// depth, octant, particle count, skip; leaves completely (their particle, or
+// the depth-32 merge of :355-365 and :490-507 in insertion order) with their
+// traversal record.
+__global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict__ st, PathOrder O,
+                                                      TmpNodes T, const int32_t* __restrict__ sidx,
+                                                      const float4* __restrict__ sbody,
+                                                      float4* rec, TravParams tp) {
+  if (st->use_seq) return;
+  const int n = st->n;
+  const int j = blockIdx.x * kThreads + threadIdx.x;
+  if (j >= n) return;
+  const int l0 = O.lcp[j], l1 = O.lcp[j + 1];
+  const int L = leaf_depth(l0, l1);
+  if (L <= l0) return;  // not the first particle of any cell (depth-32 duplicate)
+  const int pre0 = O.first_pre[j] - l0 - 1;  // pre(j, d) = pre0 + d
+  const uint4 pj = O.spath[j];
+  float cx = st->root_c[0], cy = st->root_c[1], cz = st->root_c[2], h = st->root_h;
+  for (int d = 1; d <= L; ++d) {
+    const int o = path_digit(pj, d - 1);
+    const float hs = mul_rn(h, 0.5f);  // :375
+    cx = add_rn(cx, (o & 1) ? hs : -hs);
+    cy = add_rn(cy, (o & 2) ? hs : -hs);
+    cz = add_rn(cz, (o & 4) ? hs : -hs);
+    h = hs;
+    if (d <= l0) continue;
+    const int p = pre0 + d;
+    const bool leaf = d == L;
+    const float4 box = make_float4(cx, cy, cz, h);
+    T.box[p] = box;
+    T.info[p] = (uint32_t)d | ((uint32_t)o << 8) | (leaf ? (1u << 11) : 0u);
+    if (!leaf) {
+      T.pidx[p] = -1;
+      continue;
+    }
+    const int r = O.rank[j];
+    int e = j + 1;
+    float4 q = sbody[r];
+    int t2 = INT_MAX;
+    if (l1 == kMaxDepth) {  // depth-32 cell with several particles, ranks ascending
+      t2 = O.rank[j + 1];
+      while (e < n && O.lcp[e] == kMaxDepth) {
+        const float4 u = sbody[O.rank[e]];
+        const float tot = add_rn(q.w, u.w);
+        q.x = div_rn(add_rn(mul_rn(q.x, q.w), mul_rn(u.x, u.w)), tot);
+        q.y = div_rn(add_rn(mul_rn(q.y, q.w), mul_rn(u.y, u.w)), tot);
+        q.z = div_rn(add_rn(mul_rn(q.z, q.w), mul_rn(u.z, u.w)), tot);
+        q.w = tot;
+        ++e;
+      }
+    }
+    T.com[p] = q;
+    T.pidx[p] = sidx[r];
+    T.t1[p] = r;
+    T.t2[p] = t2;
+    T.count[p] = e - j;
+    T.skip[p] = p + 1;
+    write_trav(rec, p, 1, q, box, true, tp);
+  }
+  // Ends of the internal cells, innermost first: the run of (j, d) extends the
+  // run of (j, d + 1) while positions keep d leading digits (gallop, then bisect).
+  int e = j + 1;
+  while (e < n && O.lcp[e] == kMaxDepth) ++e;
+  for (int d = L - 1; d > l0; --d) {
+    if (e < n && O.lcp[e] >= d) {
+      int lo = e, step = 1, hi = e + 1;
+      while (hi < n && path_lcp(pj, O.spath[hi]) >= d) {
+        lo = hi;
+        step <<= 1;
+        hi = lo + step;
+      }
+      if (hi > n) hi = n;
+      while (hi - lo > 1) {  // in(lo), !in(hi) (or hi == n)
+        const int mid = lo + ((hi - lo) >> 1);
+        if (path_lcp(pj, O.spath[mid]) >= d) lo = mid;
+        else hi = mid;
+      }
+      e = hi;
+    }
+    const int p = pre0 + d;
+    T.count[p] = e - j;
+    T.skip[p] = O.first_pre[e];
+  }
+}
+
+// Children of every internal cell: first child p + 1, each next child at the
+// previous one's skip (children 7..0), stored by octant.
+__global__ __launch_bounds__(kThreads) void cell_link(const BhState* __restrict__ st, TmpNodes T) {
+  if (st->use_seq) return;
+  const int nn = st->final_count;
+  for (int p = blockIdx.x * kThreads + threadIdx.x; p < nn; p += gridDim.x * kThreads) {
+    if (T.info[p] & (1u << 11)) continue;
+    int c[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    const int end = T.skip[p];
+    // at most 8 children, and skips only grow: the bounds keep the walk finite
+    for (int q = p + 1, it = 0; q < end && it < 8; ++it) {
+      c[(T.info[q] >> 8) & 7u] = q;
+      const int next = T.skip[q];
+      q = next > q ? next : end;
+    }
+    int4* out = reinterpret_cast<int4*>(T.child + (size_t)p * 8);
+    out[0] = make_int4(c[0], c[1], c[2], c[3]);
+    out[1] = make_int4(c[4], c[5], c[6], c[7]);
+  }
+}
+
+// Internal cell p from its children, in octant order (:514-538): mass, centre
+// of mass, the two first ranks, the children's creation steps, the record.
+__device__ void cell_reduce(int p, TmpNodes T, float4* rec, TravParams tp) {
+  const int4* cp = reinterpret_cast<const int4*>(T.child + (size_t)p * 8);
+  const int4 c0 = cp[0], c1 = cp[1];
+  const int c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+  float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+  int r1 = INT_MAX, r2 = INT_MAX;
+  for (int o = 0; o < 8; ++o) {
+    if (c[o] < 0) continue;
+    const float4 q = T.com[c[o]];
+    tm = add_rn(tm, q.w);
+    sx = add_rn(sx, mul_rn(q.x, q.w));
+    sy = add_rn(sy, mul_rn(q.y, q.w));
+    sz = add_rn(sz, mul_rn(q.z, q.w));
+    const int a = T.t1[c[o]], b = T.t2[c[o]];
+    if (a < r1) {
+      r2 = min(r1, b);
+      r1 = a;
+    } else if (a < r2) {
+      r2 = a;
     }
   }
+  float4 com = make_float4(0.f, 0.f, 0.f, tm);
+  if (tm > 0.0f) {
+    com.x = div_rn(sx, tm);
+    com.y = div_rn(sy, tm);
+    com.z = div_rn(sz, tm);
+  }
+  T.com[p] = com;
+  T.t1[p] = r1;
+  T.t2[p] = r2;
+  for (int o = 0; o < 8; ++o)
+    if (c[o] >= 0) {
+      const int t1c = T.t1[c[o]];
+      T.k[c[o]] = p == 0 ? t1c : max(r2, t1c);  // root's children: created by their first particle
+    }
+  write_trav(rec, p, T.skip[p] - p, com, T.box[p], false, tp);
+}
+
+// Bottom-up over the internal cells whose runs lie inside this block of
+// positions, deepest level first. The cells that contain the block's last
+// position and the next one (at most one per depth) are listed for cell_span.
+__global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O, TmpNodes T,
+                                                       float4* rec, TravParams tp) {
+  __shared__ int8_t smin[kPosBlock + 1];  // smin[x - a] = min(lcp[x..b]), -1 past n
+  __shared__ int seg[kThreads];
+  __shared__ int sh_top, sh_span;
+  if (st->use_seq) return;
+  const int n = st->n;
+  const int a = blockIdx.x * kPosBlock;
+  const int b = min(a + kPosBlock, n);  // the block's positions are [a, b)
+  int* span = O.span + (size_t)blockIdx.x * (kMaxDepth + 1);
+  if (threadIdx.x <= kMaxDepth) span[threadIdx.x] = -1;
+  if (threadIdx.x == 0) {
+    sh_top = 0;
+    sh_span = -1;
+  }
+  // suffix minima of lcp over (a, b], with lcp[b] = -1 when b == n
+  {
+    const int t = threadIdx.x;
+    int v[kPosRounds];
+    int m = 127;
+    for (int i = kPosRounds - 1; i >= 0; --i) {
+      const int x = a + 1 + t * kPosRounds + i;
+      int l = 127;
+      if (x <= b) l = x < n ? (int)O.lcp[x] : -1;
+      m = min(m, l);
+      v[i] = m;
+    }
+    seg[t] = m;
+    __syncthreads();
+    int after = 127;
+    for (int q = t + 1; q < kThreads; ++q) after = min(after, seg[q]);
+    for (int i = 0; i < kPosRounds; ++i) smin[1 + t * kPosRounds + i] = (int8_t)min(v[i], after);
+  }
+  __syncthreads();
+  int l0v[kPosRounds], Lv[kPosRounds], pv[kPosRounds], mv[kPosRounds];
+  int top = 0, spantop = -1;
+  for (int i = 0; i < kPosRounds; ++i) {
+    const int j = a + i * kThreads + (int)threadIdx.x;
+    l0v[i] = 0;
+    Lv[i] = 0;
+    pv[i] = 0;
+    mv[i] = -1;
+    if (j >= b) continue;
+    const int l0 = O.lcp[j];
+    const int L = leaf_depth(l0, O.lcp[j + 1]);
+    l0v[i] = l0;
+    Lv[i] = L;
+    pv[i] = O.first_pre[j] - l0 - 1;
+    mv[i] = smin[j + 1 - a];
+    top = max(top, L - 1);
+    for (int d = l0 + 1; d < L && d <= mv[i]; ++d) {  // internal cells containing position b
+      span[d] = pv[i] + d;
+      spantop = max(spantop, d);
+    }
+  }
+  if (top) atomicMax(&sh_top, top);
+  if (spantop >= 0) atomicMax(&sh_span, spantop);
+  __syncthreads();
+  for (int d = sh_top; d >= 1; --d) {
+    for (int i = 0; i < kPosRounds; ++i)
+      if (l0v[i] < d && d < Lv[i] && mv[i] < d) cell_reduce(pv[i] + d, T, rec, tp);
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (b == n) cell_reduce(0, T, rec, tp);
+    else span[0] = 0;
+  }
+  if (threadIdx.x == 0) {
+    const int sd = (blockIdx.x == 0 && b < n) ? max(sh_span, 0) : sh_span;
+    if (sd >= 0) atomicMax(&st->span_depth, sd);
+  }
+}
+
+// One workgroup: the straddling cells, deepest first. Each one's children are
+// inside some block (done by cell_local) or straddling one level deeper.
+__global__ __launch_bounds__(1024) void cell_span(const BhState* __restrict__ st, PathOrder O,
+                                                  TmpNodes T, float4* rec, TravParams tp) {
+  if (st->use_seq) return;
+  const int nb = (st->n + kPosBlock - 1) / kPosBlock;
+  if (nb <= 1) return;
+  for (int d = st->span_depth; d >= 0; --d) {
+    for (int k = threadIdx.x; k < nb; k += 1024) {
+      const int p = O.span[(size_t)k * (kMaxDepth + 1) + d];
+      if (p >= 0) cell_reduce(p, T, rec, tp);
+    }
+    __syncthreads();
+  }
+}
+
+// Reference node-numbering key of every cell (root first).
+__global__ __launch_bounds__(kThreads) void node_keys(const BhState* __restrict__ st, TmpNodes T,
+                                                      uint32_t* keys, int32_t* vals) {
+  const int nn = st->sort_n;
+  for (int t = blockIdx.x * kThreads + threadIdx.x; t < nn; t += gridDim.x * kThreads) {
+    uint32_t key = 0;
+    if (t > 0) {
+      const uint32_t depth = T.info[t] & 0xFFu;
+      const uint32_t flag = T.t1[t] == T.k[t] ? 1u : 0u;
+      key = ((uint32_t)(T.k[t] + 1) << 7) | (flag << 6) | depth;
+    }
+    keys[t] = key;
+    vals[t] = t;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void invert_perm(const BhState* __restrict__ st,
+                                                        const int32_t* __restrict__ order,
+                                                        int32_t* final_of) {
+  const int nn = st->sort_n;
+  for (int f = blockIdx.x * kThreads + threadIdx.x; f < nn; f += gridDim.x * kThreads)
+    final_of[order[f]] = f;
 }
 
 __global__ __launch_bounds__(kThreads) void emit_nodes(const BhState* __restrict__ st, TmpNodes T,
@@ -691,11 +838,11 @@ __global__ __launch_bounds__(kThreads) void emit_nodes(const BhState* __restrict
     nd.comy = c.y;
     nd.comz = c.z;
     nd.mass = c.w;
+    const bool leaf = (T.info[t] >> 11) & 1u;
     for (int o = 0; o < 8; ++o) {
-      const int ch = T.child[t * 8 + o];
+      const int ch = leaf ? -1 : T.child[t * 8 + o];
       nd.child[o] = ch >= 0 ? final_of[ch] : -1;
     }
-    const bool leaf = (T.info[t] >> 11) & 1u;
     nd.pidx = leaf ? T.pidx[t] : -1;
     nd.leaf = leaf ? 1 : 0;
     nd.pad[0] = nd.pad[1] = nd.pad[2] = 0;
@@ -1201,7 +1348,8 @@ struct nbx_bh {
   uint4* path = nullptr;
   float4* sbody = nullptr;  // particles in sorted order: x, y, z, m
   nbx::TmpNodes T{};
-  nbx::Levels L{};
+  nbx::PathOrder O{};
+  int pos_blocks = 0;  // blocks of kPosBlock positions at max_particles
   uint32_t* nkeys = nullptr;
   int32_t* nvals = nullptr;
   int32_t* final_of = nullptr;
@@ -1292,35 +1440,31 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->path, P);
   A(&t->sbody, P);
   A(&t->T.t1, NN);
+  A(&t->T.t2, NN);
   A(&t->T.k, NN);
-  A(&t->T.parent, NN);
   A(&t->T.count, NN);
   A(&t->T.pidx, NN);
   A(&t->T.info, NN);
   A(&t->T.child, NN * 8);
   A(&t->T.box, NN);
   A(&t->T.com, NN);
-  A(&t->T.member_base, NN);
-  A(&t->T.member_fill, NN);
-  const size_t segs = P / 2 + 1;
-  for (int b = 0; b < 2; ++b) {
-    A(&t->L.seg_node[b], segs);
-    A(&t->L.seg_t1[b], segs);
-    A(&t->L.seg_t2[b], segs);
-    A(&t->L.seg_cnt[b], segs * 8);
-    A(&t->L.seg_min[b], segs * 8);
-    A(&t->L.seg_next[b], segs * 8);
-    A(&t->L.elem_rank[b], P);
-    A(&t->L.elem_seg[b], P);
-  }
-  A(&t->L.multi_list, P);
-  A(&t->L.members, P);
+  A(&t->T.skip, NN);
+  A(&t->T.size, NN);
+  A(&t->T.pre, NN);
+  t->pos_blocks = (int)((P + kPosBlock - 1) / kPosBlock);
+  A(&t->O.key, P);
+  A(&t->O.key_hi, P);
+  A(&t->O.rank, P);
+  A(&t->O.spath, P);
+  A(&t->O.lcp, P + 1);
+  A(&t->O.first_pre, P + 1);
+  A(&t->O.bsum, (size_t)t->pos_blocks);
+  A(&t->O.span, (size_t)t->pos_blocks * (kMaxDepth + 1));
+  A(&t->O.runs, P);
   A(&t->nkeys, NN);
   A(&t->nvals, NN);
   A(&t->final_of, NN);
   A(&t->nodes, NN);
-  A(&t->T.size, NN);
-  A(&t->T.pre, NN);
   A(&t->trav_rec, NN * 4);
   if (rc == NBX_OK) rc = t->sort_particles.init((int)P);
   if (rc == NBX_OK) rc = t->sort_nodes.init((int)NN);
@@ -1350,8 +1494,6 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
                 d->count);
   hipStream_t s = as_stream(stream);
   const int n = (int)d->count;
-  const int g = grid_for(n);
-  const int gl = grid_for(n, 1024);  // level passes: contiguous chunks per workgroup
   mark(t, 0, s);
   const int bb = std::min(t->bbox_blocks, grid_for(n, 1024));
   hipLaunchKernelGGL(bbox_partial, dim3(bb), dim3(kThreads), 0, s, d->pos_x, d->pos_y, d->pos_z,
@@ -1366,18 +1508,22 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   NBX_TRY(t->sort_particles.sort(t->codes, t->idx, &t->st->n, 30, s));
   mark(t, 3, s);
   hipLaunchKernelGGL(path_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, d->pos_x,
-                     d->pos_y, d->pos_z, d->mass, t->idx, n, t->st, t->path, t->sbody);
+                     d->pos_y, d->pos_z, d->mass, t->idx, n, t->st, t->path, t->sbody, t->O.key,
+                     t->O.key_hi, t->O.rank);
   NBX_LAUNCHED();
   mark(t, 4, s);
-  hipLaunchKernelGGL(tree_init, dim3(1), dim3(64), 0, s, t->st, t->T, t->L, t->max_nodes);
-  for (int lv = 0; lv < kMaxDepth; ++lv) {
-    hipLaunchKernelGGL(level_count, dim3(gl), dim3(kThreads), 0, s, t->st, t->L, t->path, lv);
-    hipLaunchKernelGGL(level_children, dim3(grid_for((size_t)n * 4 + 8)), dim3(kThreads), 0, s,
-                       t->st, t->L, t->T, t->idx, t->sbody, lv, t->max_nodes);
-    hipLaunchKernelGGL(level_route, dim3(gl), dim3(kThreads), 0, s, t->st, t->L, t->T, t->path,
-                       lv);
-  }
-  hipLaunchKernelGGL(tree_levels_done, dim3(1), dim3(64), 0, s, t->st, t->max_nodes);
+  // path order: stable sort by digits 10..19, then by digits 0..9
+  NBX_TRY(t->sort_particles.sort(t->O.key, t->O.rank, &t->st->n, 30, s));
+  hipLaunchKernelGGL(path_gather_key, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, &t->st->n,
+                     t->O);
+  NBX_TRY(t->sort_particles.sort(t->O.key, t->O.rank, &t->st->n, 30, s));
+  hipLaunchKernelGGL(path_lcp_kernel, dim3(div_up((size_t)n + 1, kThreads)), dim3(kThreads), 0, s,
+                     t->st, t->path, t->O);
+  hipLaunchKernelGGL(path_refine, dim3(grid_for((size_t)n * 64, 1024)), dim3(kThreads), 0, s,
+                     t->st, t->path, t->O);
+  const int pb = (int)div_up(n, kPosBlock);
+  hipLaunchKernelGGL(cell_count, dim3(pb), dim3(kThreads), 0, s, t->st, t->O);
+  hipLaunchKernelGGL(cell_offsets, dim3(1), dim3(1024), 0, s, t->st, t->O, t->T, t->max_nodes);
   NBX_LAUNCHED();
   if (t->force_seq) {
     // diagnostic: route through the exact sequential builder
@@ -1386,25 +1532,22 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
     NBX_HIP(hipMemcpyAsync(&t->st->sort_n, &zero, sizeof(int), hipMemcpyHostToDevice, s));
     NBX_HIP(hipStreamSynchronize(s));
   }
-  mark(t, 5, s);
-  const int gn = grid_for((size_t)t->max_nodes);
-  hipLaunchKernelGGL(multi_leaf_merge, dim3(grid_for(n / 2 + 1, 256)), dim3(64), 0, s, t->st,
-                     t->L, t->T, t->idx, d->pos_x, d->pos_y, d->pos_z, d->mass);
-  hipLaunchKernelGGL(node_keys, dim3(gn), dim3(kThreads), 0, s, t->st, t->T, t->nkeys, t->nvals);
+  hipLaunchKernelGGL(cell_first_pre, dim3(pb), dim3(kThreads), 0, s, t->st, t->O);
+  hipLaunchKernelGGL(cell_emit, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->st, t->O,
+                     t->T, t->idx, t->sbody, t->trav_rec, t->prep);
+  hipLaunchKernelGGL(cell_link, dim3(grid_for((size_t)t->max_nodes)), dim3(kThreads), 0, s, t->st,
+                     t->T);
   NBX_LAUNCHED();
-  NBX_TRY(t->sort_nodes.sort(t->nkeys, t->nvals, &t->st->sort_n, key_bits_for(t->max_particles),
-                             s));
-  hipLaunchKernelGGL(invert_perm, dim3(gn), dim3(kThreads), 0, s, t->st, t->nvals, t->final_of);
-  for (int lv = kMaxDepth; lv >= 0; --lv)
-    hipLaunchKernelGGL(com_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv);
+  mark(t, 5, s);
+  hipLaunchKernelGGL(cell_local, dim3(pb), dim3(kThreads), 0, s, t->st, t->O, t->T, t->trav_rec,
+                     t->prep);
+  hipLaunchKernelGGL(cell_span, dim3(1), dim3(1024), 0, s, t->st, t->O, t->T, t->trav_rec,
+                     t->prep);
   NBX_LAUNCHED();
   mark(t, 6, s);
-  t->nodes_pending = true;  // emit_nodes runs when the node array is asked for
+  t->nodes_pending = true;  // the reference node array is numbered and emitted on demand
   hipLaunchKernelGGL(seq_build, dim3(1), dim3(64), 0, s, t->st, t->idx, d->pos_x, d->pos_y,
                      d->pos_z, d->mass, t->nodes, t->max_nodes);
-  for (int lv = 0; lv <= kMaxDepth; ++lv)
-    hipLaunchKernelGGL(pre_level, dim3(g), dim3(kThreads), 0, s, t->st, t->T, lv, t->trav_rec,
-                       t->prep);
   hipLaunchKernelGGL(seq_preorder, dim3(1), dim3(64), 0, s, t->st, t->nodes, t->T.pre, t->T.size,
                      t->T.k, t->trav_rec, t->prep);
   NBX_LAUNCHED();
@@ -1460,8 +1603,14 @@ int nbx_bh_copy_nodes(nbx_bh* t, void* host_nodes, size_t capacity_nodes) {
   if ((size_t)cnt > capacity_nodes)
     return fail(NBX_ERR_CAPACITY, "%d nodes do not fit in %zu", cnt, capacity_nodes);
   if (t->nodes_pending) {
-    hipLaunchKernelGGL(emit_nodes, dim3(grid_for((size_t)t->max_nodes)), dim3(kThreads), 0, 0,
-                       t->st, t->T, t->final_of, t->nodes);
+    const int gn = grid_for((size_t)t->max_nodes);
+    hipLaunchKernelGGL(node_keys, dim3(gn), dim3(kThreads), 0, 0, t->st, t->T, t->nkeys, t->nvals);
+    NBX_LAUNCHED();
+    NBX_TRY(t->sort_nodes.sort(t->nkeys, t->nvals, &t->st->sort_n,
+                               key_bits_for(t->max_particles), 0));
+    hipLaunchKernelGGL(invert_perm, dim3(gn), dim3(kThreads), 0, 0, t->st, t->nvals, t->final_of);
+    hipLaunchKernelGGL(emit_nodes, dim3(gn), dim3(kThreads), 0, 0, t->st, t->T, t->final_of,
+                       t->nodes);
     NBX_LAUNCHED();
     NBX_HIP(hipDeviceSynchronize());
     t->nodes_pending = false;

@@ -191,3 +191,117 @@ def same_nodes(a, b):
             bad = np.nonzero(np.any((x != y).reshape(len(a), -1), axis=1))[0]
             return False, f"field {f} differs at nodes {bad[:8].tolist()}"
     return True, ""
+
+
+def _lcp(a, b):
+    k = 0
+    while k < MAX_DEPTH and a[k] == b[k]:
+        k += 1
+    return k
+
+
+def path_order_build(pos, mass, sorted_idx, bmin, bmax, max_nodes):
+    """The GPU's construction from the particles' order by octant path.
+
+    Particles sorted by path (octants compared as 7 - o, ties by rank);
+    lcp[j] = digits shared with position j - 1; position j starts the cells at
+    depths lcp[j] + 1 .. L[j] = min(32, max(lcp[j], lcp[j + 1]) + 1), numbered
+    in preorder as first_pre[j] + d - lcp[j] - 1 with first_pre = 1 + the
+    exclusive scan of L - lcp; a cell ending before position e has skip
+    first_pre[e]. Children come from skip chains, centres of mass bottom-up in
+    octant order, indices from (k, t1 == k, depth). Returns (nodes array in the
+    reference numbering, max depth, each node's preorder position), or None when
+    the node budget could fire (the sequential builder's case)."""
+    n = len(sorted_idx)
+    P = paths(pos, sorted_idx, bmin, bmax)
+    R = sorted(range(n), key=lambda r: (tuple(7 - d for d in P[r]), r))
+    sp = [P[r] for r in R]
+    lcp = [0] * (n + 1)
+    for j in range(1, n):
+        lcp[j] = _lcp(sp[j - 1], sp[j])
+    L = [min(MAX_DEPTH, max(lcp[j], lcp[j + 1]) + 1) for j in range(n)]
+    first = [1] * (n + 1)
+    for j in range(n):
+        first[j + 1] = first[j] + L[j] - lcp[j]
+    C = first[n]
+    if C + MAX_DEPTH + 1 >= max_nodes:
+        return None
+    c0, h0 = root_cell(bmin, bmax)
+    depth, octn, leaf, skip = [0] * C, [0] * C, [False] * C, [0] * C
+    box, run = [None] * C, [None] * C
+    skip[0], box[0], run[0] = C, (c0, h0), (0, n)
+    for j in range(n):
+        cc, hh = list(c0), h0
+        for d in range(1, L[j] + 1):
+            cc, hh = child_center(cc, hh, sp[j][d - 1])
+            if d <= lcp[j]:
+                continue
+            p = first[j] + d - lcp[j] - 1
+            e = j + 1
+            while e < n and _lcp(sp[j], sp[e]) >= d:
+                e += 1
+            depth[p], octn[p], leaf[p] = d, sp[j][d - 1], d == L[j]
+            skip[p], box[p], run[p] = first[e], (cc, hh), (j, e)
+    kids = [[] for _ in range(C)]
+    for p in range(C):
+        q = p + 1
+        while not leaf[p] and q < skip[p]:
+            kids[p].append(q)
+            q = skip[q]
+    com, tm, cnt = [None] * C, [F(0)] * C, [0] * C
+    t1, t2, pidx = [0] * C, [0] * C, [-1] * C
+    none = 1 << 40
+    for p in reversed(range(C)):
+        if leaf[p]:
+            mem = [R[x] for x in range(*run[p])]  # ranks ascending: insertion order
+            q = sorted_idx[mem[0]]
+            c, m = [F(v) for v in pos[q]], F(mass[q])
+            for r in mem[1:]:
+                q2 = sorted_idx[r]
+                pm = F(mass[q2])
+                tot = F(m + pm)
+                c = [F(F(F(c[a] * m) + F(F(pos[q2][a]) * pm)) / tot) for a in range(3)]
+                m = tot
+            com[p], tm[p], cnt[p], pidx[p] = c, m, len(mem), sorted_idx[mem[0]]
+            t1[p], t2[p] = mem[0], (mem[1] if len(mem) > 1 else none)
+        else:
+            m, s, k, ranks = F(0), [F(0)] * 3, 0, []
+            for q in sorted(kids[p], key=lambda q: octn[q]):
+                m = F(m + tm[q])
+                s = [F(s[a] + F(com[q][a] * tm[q])) for a in range(3)]
+                k += cnt[q]
+                ranks += [t1[q], t2[q]]
+            ranks.sort()
+            com[p] = [F(s[a] / m) for a in range(3)] if m > 0 else [F(0)] * 3
+            tm[p], cnt[p], t1[p], t2[p] = m, k, ranks[0], ranks[1]
+    key = [0] * C
+    for p in range(C):
+        for q in kids[p]:
+            kk = t1[q] if p == 0 else max(t2[p], t1[q])
+            key[q] = ((kk + 1) << 7) | ((1 if t1[q] == kk else 0) << 6) | depth[q]
+    order = sorted(range(C), key=lambda p: (key[p], p))
+    fin = [0] * C
+    for f, p in enumerate(order):
+        fin[p] = f
+    out = []
+    for p in order:
+        nd = new_node()
+        ch = [-1] * 8
+        for q in kids[p]:
+            ch[octn[q]] = fin[q]
+        nd.update(center=box[p][0], half_size=box[p][1], com=com[p], total_mass=tm[p],
+                  children=ch, particle_index=pidx[p], is_leaf=leaf[p], particle_count=cnt[p])
+        out.append(nd)
+    multi = any(leaf[p] and cnt[p] > 1 for p in range(C))
+    pre_of = [0] * C  # preorder position of every node, by reference index
+    for p in range(C):
+        pre_of[fin[p]] = p
+    return out, (MAX_DEPTH if multi else max(L) - 1), pre_of
+
+
+def shared_prefix_runs(pos, sorted_idx, bmin, bmax, digits):
+    """Sizes of the groups of particles whose paths share their first `digits` digits."""
+    groups = {}
+    for p in paths(pos, sorted_idx, bmin, bmax):
+        groups[p[:digits]] = groups.get(p[:digits], 0) + 1
+    return sorted(groups.values(), reverse=True)

@@ -76,6 +76,41 @@ def test_two_plummer_65536_exact(nbx, orc):
     check_exact(nbx, orc, nbx.gen_two_plummer(65536, seed=42))
 
 
+def clustered(nbx, n_bg, n_cl, side, seed):
+    """A Plummer sphere whose last n_cl bodies sit in a cube of the given side."""
+    h = nbx.gen_plummer(n_bg + n_cl, seed=seed)
+    rng = np.random.default_rng(seed)
+    for a, k in enumerate(("pos_x", "pos_y", "pos_z")):
+        off = rng.uniform(0, side, n_cl).astype(np.float32)
+        h[k][n_bg:] = (np.float32((0.3, -0.2, 0.1)[a]) + off).astype(np.float32)
+    return h
+
+
+def prefix_runs(h, ref, digits=20):
+    pos = np.stack([h["pos_x"], h["pos_y"], h["pos_z"]], 1)
+    return M.shared_prefix_runs(pos, ref["sorted_idx"], ref["bbox_min"], ref["bbox_max"], digits)
+
+
+def test_runs_sharing_sorted_digits(nbx, orc):
+    # the path sort orders by the first 20 octant digits; bodies closer than that
+    # resolves (here 192 and 108 of them, leaves down to depth 27) are ordered
+    # by their remaining digits in one workgroup per run
+    h = clustered(nbx, 5000, 300, 4e-5, seed=7)
+    _, t, ref = check_exact(nbx, orc, h)
+    runs = prefix_runs(h, ref)
+    assert 2 <= runs[0] <= 1024 and ref["max_depth"] > 20, (runs[:4], ref["max_depth"])
+    assert t.build_info()["builder"] == "parallel"
+
+
+def test_run_longer_than_a_workgroup_uses_sequential_builder(nbx, orc):
+    # 1,500 distinct bodies share their first 20 digits: more than one workgroup
+    # orders, so the exact sequential builder takes the build; the result is the same
+    h = clustered(nbx, 3000, 1500, 1e-5, seed=8)
+    _, t, ref = check_exact(nbx, orc, h)
+    assert prefix_runs(h, ref)[0] > 1024 and ref["node_count"] + 33 < 2 * len(h["mass"])
+    assert t.build_info()["builder"] == "sequential"
+
+
 def force_check(nbx, orc, h, theta, G, eps, targets=None):
     d, t = gpu_build(nbx, h)
     t.forces(d, theta, G, eps)

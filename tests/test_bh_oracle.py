@@ -126,6 +126,32 @@ def test_numbering_rule_reproduces_insertion_order(orc, nbx, kind, n):
         assert nodes[i]["particle_count"] == len(members[pre])
 
 
+@pytest.mark.parametrize("kind,n", [c for c in CASES if c[0] not in ("coincident", "collinear")])
+def test_path_order_construction_reproduces_the_tree(orc, nbx, kind, n):
+    # the construction the GPU builder uses (cells from adjacent common prefixes
+    # of the path-sorted particles, preorder numbering from one scan) gives the
+    # oracle's node array field for field, and its cell order is the preorder
+    # with children 7..0 that the reference's stack walk visits
+    h = tree_inputs(nbx, kind, n)
+    t = orc.bh_pipeline(h)
+    pos = np.stack([h["pos_x"], h["pos_y"], h["pos_z"]], 1)
+    r = M.path_order_build(pos, h["mass"], t["sorted_idx"], t["bbox_min"], t["bbox_max"], 2 * n)
+    if r is None:
+        assert t["node_count"] + 33 >= 2 * n
+        pytest.skip("node budget could fire: the sequential builder handles this input")
+    nodes, depth, pre_of = r
+    ok, why = M.same_nodes(t["nodes"], M.nodes_to_array(nodes, orc.NODE_DTYPE))
+    assert ok, why
+    assert depth == t["max_depth"]
+    walk, stack = [], [0]  # children pushed 0..7, popped 7..0 (:160-241)
+    while stack:
+        v = stack.pop()
+        walk.append(v)
+        if not t["nodes"][v]["is_leaf"]:
+            stack += [c for c in t["nodes"][v]["children"] if c >= 0]
+    assert [pre_of[v] for v in walk] == list(range(len(walk)))
+
+
 @pytest.mark.parametrize("kind,n", CASES)
 def test_tree_invariants(orc, nbx, kind, n):
     h = tree_inputs(nbx, kind, n)
