@@ -74,7 +74,6 @@ struct BhState {
   int32_t deepest;      // deepest leaf
   int32_t multi_count;  // depth-32 leaves holding several particles
   int32_t run_count;    // runs of positions sharing the sorted path digits
-  int32_t span_depth;   // deepest cell that straddles a block of positions
   int32_t use_seq;
   int32_t final_count;  // node_count_ of the reference
   int32_t max_depth;    // max_depth_ of the reference
@@ -165,7 +164,6 @@ __global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int
   st->deepest = 0;
   st->multi_count = 0;
   st->run_count = 0;
-  st->span_depth = 0;
   st->overflow = 0;
 }
 
@@ -331,10 +329,17 @@ struct PathOrder {
   int32_t* bsum;      // per block of kPosBlock positions: cells, then their offsets
   int32_t* span;      // per block: [kMaxDepth + 1] straddling cells by depth, or -1
   int32_t* runs;      // starts of runs sharing 20 digits
+  int8_t* lmin1;      // min lcp per chunk of 64 positions
+  int8_t* lmin2;      // min lcp per block of kPosBlock positions
+  int8_t* lmin3;      // min lcp per kSuperBlocks blocks
+  int32_t* bstat;     // per block: deepest leaf, depth-32 multi leaves, deepest straddling cell
+  int32_t* ppart;     // particle index at every position: the traversal's order
 };
 
-constexpr int kPosBlock = 2048;  // positions per workgroup in the cell passes
+constexpr int kPosBlock = 1024;  // positions per workgroup in the cell passes (16 chunks)
 constexpr int kPosRounds = kPosBlock / kThreads;
+constexpr int kSuperBlocks = 64;
+constexpr int kLocalCap = 2048;  // cells of one block held in LDS by cell_local
 constexpr int kRefineMax = 1024;
 constexpr int kSortDigits = 20;
 
@@ -373,9 +378,8 @@ __global__ __launch_bounds__(kThreads) void path_lcp_kernel(BhState* st,
                                                             PathOrder O) {
   const int n = st->n;
   const int j = blockIdx.x * kThreads + threadIdx.x;
-  if (j > n) return;
-  if (j == n) {
-    O.lcp[n] = 0;
+  if (j >= n) {  // lcp[n] = 0, and zeros to the end of its chunk of 64
+    if (j < ((n + 64) & ~63)) O.lcp[j] = 0;
     return;
   }
   const uint4 a = path[O.rank[j]];
@@ -457,49 +461,87 @@ __global__ __launch_bounds__(kThreads) void path_refine(BhState* st,
   }
 }
 
-// Cells per block of positions; the deepest leaf and the depth-32 leaves that
-// hold several particles.
+// Cells per block of positions; the deepest leaf; the depth-32 leaves that
+// hold several particles; and the minima of lcp per chunk of 64 positions and
+// per block, two levels of the index next_lower() searches.
 __global__ __launch_bounds__(kThreads) void cell_count(BhState* st, PathOrder O) {
-  __shared__ int sh_sum[kThreads / 64];
+  __shared__ int sh_sum[kThreads / 64], sh_min[kThreads / 64];
   const int n = st->n;
   const int a = blockIdx.x * kPosBlock;
-  int sum = 0, deep = 0, multi = 0;
+  int sum = 0, deep = 0, multi = 0, bmin = 127;
   for (int i = 0; i < kPosRounds; ++i) {
     const int j = a + i * kThreads + (int)threadIdx.x;
-    if (j >= n) break;
-    const int l0 = O.lcp[j], l1 = O.lcp[j + 1];
-    const int L = leaf_depth(l0, l1);
-    sum += L - l0;
-    deep = max(deep, L);
-    multi += (l0 < kMaxDepth && l1 == kMaxDepth) ? 1 : 0;
+    int lv = 0;
+    if (j < n) {
+      const int l0 = O.lcp[j], l1 = O.lcp[j + 1];
+      const int L = leaf_depth(l0, l1);
+      sum += L - l0;
+      deep = max(deep, L);
+      multi += (l0 < kMaxDepth && l1 == kMaxDepth) ? 1 : 0;
+      lv = l0;
+    }
+    int cm = lv;  // positions at and past n count as 0: every search stops at n
+    for (int off = 32; off > 0; off >>= 1) cm = min(cm, __shfl_xor(cm, off, 64));
+    if ((threadIdx.x & 63) == 0) O.lmin1[(a + i * kThreads + (int)threadIdx.x) >> 6] = (int8_t)cm;
+    bmin = min(bmin, cm);
   }
   for (int off = 32; off > 0; off >>= 1) {
     sum += __shfl_xor(sum, off, 64);
     deep = max(deep, __shfl_xor(deep, off, 64));
     multi += __shfl_xor(multi, off, 64);
   }
+  __shared__ int sh_deep[kThreads / 64], sh_multi[kThreads / 64];
   if ((threadIdx.x & 63) == 0) {
     sh_sum[threadIdx.x >> 6] = sum;
-    if (deep) atomicMax(&st->deepest, deep);
-    if (multi) atomicAdd(&st->multi_count, multi);
+    sh_min[threadIdx.x >> 6] = bmin;
+    sh_deep[threadIdx.x >> 6] = deep;
+    sh_multi[threadIdx.x >> 6] = multi;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    int t = 0;
-    for (int w = 0; w < kThreads / 64; ++w) t += sh_sum[w];
+    int t = 0, m = 127, dp = 0, mu = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+      t += sh_sum[w];
+      m = min(m, sh_min[w]);
+      dp = max(dp, sh_deep[w]);
+      mu += sh_multi[w];
+    }
     O.bsum[blockIdx.x] = t;
+    O.lmin2[blockIdx.x] = (int8_t)m;
+    O.bstat[3 * blockIdx.x] = dp;
+    O.bstat[3 * blockIdx.x + 1] = mu;
   }
 }
 
-// One workgroup: block offsets, the cell count, the builder choice, the root.
+// One workgroup: block offsets, the cell count, the builder choice, the root,
+// and the third level of the lcp index (minimum per kSuperBlocks blocks).
 __global__ __launch_bounds__(1024) void cell_offsets(BhState* st, PathOrder O, TmpNodes T,
                                                      int max_nodes) {
   __shared__ int wsum[16];
-  __shared__ int sh_carry;
+  __shared__ int sh_carry, sh_deep, sh_multi;
   const int n = st->n;
-  const int nb = (n + kPosBlock - 1) / kPosBlock;
+  const int nb = n / kPosBlock + 1;  // blocks of cell_count (through position n)
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (threadIdx.x == 0) sh_carry = 0;
+  if (threadIdx.x == 0) {
+    sh_carry = 0;
+    sh_deep = 0;
+    sh_multi = 0;
+  }
+  __syncthreads();
+  {
+    int dp = 0, mu = 0;
+    for (int k = threadIdx.x; k < nb; k += 1024) {
+      dp = max(dp, O.bstat[3 * k]);
+      mu += O.bstat[3 * k + 1];
+    }
+    if (dp) atomicMax(&sh_deep, dp);
+    if (mu) atomicAdd(&sh_multi, mu);
+  }
+  for (int s = threadIdx.x; s * kSuperBlocks < nb; s += 1024) {
+    int m = 127;
+    for (int k = s * kSuperBlocks; k < min(nb, (s + 1) * kSuperBlocks); ++k) m = min(m, (int)O.lmin2[k]);
+    O.lmin3[s] = (int8_t)m;
+  }
   __syncthreads();
   for (int base = 0; base < nb; base += 1024) {
     const int i = base + (int)threadIdx.x;
@@ -528,7 +570,9 @@ __global__ __launch_bounds__(1024) void cell_offsets(BhState* st, PathOrder O, T
   st->final_count = cells;
   st->use_seq = seq;
   st->sort_n = seq ? 0 : cells;
-  st->max_depth = st->multi_count > 0 ? kMaxDepth : st->deepest - 1;
+  st->deepest = sh_deep;
+  st->multi_count = sh_multi;
+  st->max_depth = sh_multi > 0 ? kMaxDepth : sh_deep - 1;
   st->budget_merges = 0;
   st->depth_merges = 0;
   O.first_pre[n] = cells;
@@ -561,10 +605,68 @@ __global__ __launch_bounds__(kThreads) void cell_first_pre(const BhState* __rest
   }
 }
 
+// Offset of the first of the 16 bytes at p, from byte `from` on, whose value
+// (signed when sgn) is below d; 16 if none. p is 16-byte aligned.
+__device__ __forceinline__ int first_below16(const void* p, int from, int d, bool sgn) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int b = 0; b < 16; ++b) {
+    const uint32_t u = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
+    const int val = sgn ? (int)(int8_t)u : (int)u;
+    if (b >= from && val < d) return b;
+  }
+  return 16;
+}
+
+// First of the 64 bytes at p (64-aligned), from `from` on, below d; 64 if none.
+__device__ __forceinline__ int first_below64(const void* p, int from, int d, bool sgn) {
+  const uint8_t* q = static_cast<const uint8_t*>(p);
+  int r[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) r[k] = first_below16(q + 16 * k, from - 16 * k, d, sgn);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (r[k] < 16) return 16 * k + r[k];
+  return 64;
+}
+
+// First position x >= x0 with lcp[x] < d (d >= 1, x0 <= n): the end of the run
+// of a depth-d cell. lcp[n] = 0, so the answer is at most n. The index levels:
+// lcp per position (chunks of 64), lmin1 per chunk (16 per block), lmin2 per
+// block (kSuperBlocks per super block), lmin3 per super block.
+__device__ int next_lower(const PathOrder& O, int x0, int d, int n, int nsuper) {
+  int c = x0 >> 6;
+  int r = first_below64(O.lcp + (c << 6), x0 & 63, d, false);
+  if (r < 64) return (c << 6) + r;
+  int k = c >> 4;
+  r = first_below16(O.lmin1 + (k << 4), (c & 15) + 1, d, true);
+  if (r >= 16) {
+    int s = k / kSuperBlocks;
+    r = first_below64(O.lmin2 + s * kSuperBlocks, k % kSuperBlocks + 1, d, true);
+    if (r >= 64) {
+      for (++s; s < nsuper; s += 64) {  // super blocks, 64 per look (never past the end)
+        const int f = first_below64(O.lmin3 + (s & ~63), s & 63, d, true);
+        if (f < 64) {
+          s = (s & ~63) + f;
+          break;
+        }
+        s = (s & ~63);
+      }
+      if (s >= nsuper) return n;  // not reached: lcp[n] = 0
+      r = first_below64(O.lmin2 + s * kSuperBlocks, 0, d, true);
+    }
+    k = s * kSuperBlocks + r;
+    r = first_below16(O.lmin1 + (k << 4), 0, d, true);
+  }
+  c = (k << 4) + r;
+  return (c << 6) + first_below64(O.lcp + (c << 6), 0, d, false);
+}
+
 // Every cell a position starts: box (the reference's centre chain, :375-379),
-// depth, octant, particle count, skip; leaves completely (their particle, or
-// the depth-32 merge of :355-365 and :490-507 in insertion order) with their
-// traversal record.
+// depth, octant, particle count, subtree end; leaves completely (their
+// particle, or the depth-32 merge of :355-365 and :490-507 in insertion order)
+// with their traversal record.
 __global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict__ st, PathOrder O,
                                                       TmpNodes T, const int32_t* __restrict__ sidx,
                                                       const float4* __restrict__ sbody,
@@ -573,6 +675,8 @@ __global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict_
   const int n = st->n;
   const int j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= n) return;
+  const int r = O.rank[j];
+  O.ppart[j] = sidx[r];
   const int l0 = O.lcp[j], l1 = O.lcp[j + 1];
   const int L = leaf_depth(l0, l1);
   if (L <= l0) return;  // not the first particle of any cell (depth-32 duplicate)
@@ -596,7 +700,6 @@ __global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict_
       T.pidx[p] = -1;
       continue;
     }
-    const int r = O.rank[j];
     int e = j + 1;
     float4 q = sbody[r];
     int t2 = INT_MAX;
@@ -621,54 +724,20 @@ __global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict_
     write_trav(rec, p, 1, q, box, true, tp);
   }
   // Ends of the internal cells, innermost first: the run of (j, d) extends the
-  // run of (j, d + 1) while positions keep d leading digits (gallop, then bisect).
+  // run of (j, d + 1) to the next position with fewer than d shared digits.
+  const int nsuper = (n / kPosBlock + 1 + kSuperBlocks - 1) / kSuperBlocks;
   int e = j + 1;
   while (e < n && O.lcp[e] == kMaxDepth) ++e;
   for (int d = L - 1; d > l0; --d) {
-    if (e < n && O.lcp[e] >= d) {
-      int lo = e, step = 1, hi = e + 1;
-      while (hi < n && path_lcp(pj, O.spath[hi]) >= d) {
-        lo = hi;
-        step <<= 1;
-        hi = lo + step;
-      }
-      if (hi > n) hi = n;
-      while (hi - lo > 1) {  // in(lo), !in(hi) (or hi == n)
-        const int mid = lo + ((hi - lo) >> 1);
-        if (path_lcp(pj, O.spath[mid]) >= d) lo = mid;
-        else hi = mid;
-      }
-      e = hi;
-    }
+    if (O.lcp[e] >= d) e = next_lower(O, e + 1, d, n, nsuper);
     const int p = pre0 + d;
     T.count[p] = e - j;
     T.skip[p] = O.first_pre[e];
   }
 }
 
-// Children of every internal cell: first child p + 1, each next child at the
-// previous one's skip (children 7..0), stored by octant.
-__global__ __launch_bounds__(kThreads) void cell_link(const BhState* __restrict__ st, TmpNodes T) {
-  if (st->use_seq) return;
-  const int nn = st->final_count;
-  for (int p = blockIdx.x * kThreads + threadIdx.x; p < nn; p += gridDim.x * kThreads) {
-    if (T.info[p] & (1u << 11)) continue;
-    int c[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-    const int end = T.skip[p];
-    // at most 8 children, and skips only grow: the bounds keep the walk finite
-    for (int q = p + 1, it = 0; q < end && it < 8; ++it) {
-      c[(T.info[q] >> 8) & 7u] = q;
-      const int next = T.skip[q];
-      q = next > q ? next : end;
-    }
-    int4* out = reinterpret_cast<int4*>(T.child + (size_t)p * 8);
-    out[0] = make_int4(c[0], c[1], c[2], c[3]);
-    out[1] = make_int4(c[4], c[5], c[6], c[7]);
-  }
-}
-
-// Internal cell p from its children, in octant order (:514-538): mass, centre
-// of mass, the two first ranks, the children's creation steps, the record.
+// Internal cell p from the child table, in octant order (:514-538): mass,
+// centre of mass, the two first ranks, the children's creation steps, the record.
 __device__ void cell_reduce(int p, TmpNodes T, float4* rec, TravParams tp) {
   const int4* cp = reinterpret_cast<const int4*>(T.child + (size_t)p * 8);
   const int4 c0 = cp[0], c1 = cp[1];
@@ -707,31 +776,74 @@ __device__ void cell_reduce(int p, TmpNodes T, float4* rec, TravParams tp) {
   write_trav(rec, p, T.skip[p] - p, com, T.box[p], false, tp);
 }
 
+constexpr uint32_t kLeafBit = 1u << 11;
+constexpr uint32_t kSpanBit = 1u << 12;  // cell_local only: the cell straddles the block's end
+
+// Children of local cell i (index within the block's cells) in visiting order
+// (octants descending): i + 1, then each previous child's skip, while the depth
+// is d + 1. Returns their number.
+__device__ __forceinline__ int local_children(int i, int d, int ncell, int base,
+                                              const uint32_t* info, const int* skip, int ch[8]) {
+  int nc = 0;
+  for (int q = i + 1; nc < 8 && q < ncell && (int)(info[q] & 0xFFu) == d + 1;) {
+    ch[nc++] = q;
+    const int next = skip[q] - base;
+    q = next > q ? next : ncell;
+  }
+  return nc;
+}
+
 // Bottom-up over the internal cells whose runs lie inside this block of
-// positions, deepest level first. The cells that contain the block's last
-// position and the next one (at most one per depth) are listed for cell_span.
+// positions, deepest level first, with the block's cells in LDS (in global
+// memory when there are more than kLocalCap). The cells that contain the
+// block's last position and the next one (at most one per depth; the root is
+// always one) are listed for cell_span.
 __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O, TmpNodes T,
                                                        float4* rec, TravParams tp) {
-  __shared__ int8_t smin[kPosBlock + 1];  // smin[x - a] = min(lcp[x..b]), -1 past n
+  __shared__ float4 s_com[kLocalCap];
+  __shared__ int s_t1[kLocalCap], s_t2[kLocalCap], s_skip[kLocalCap];
+  __shared__ uint32_t s_info[kLocalCap];
+  __shared__ int8_t smin[kPosBlock + 1];  // smin[x - a] = min(lcp[x..b]), -1 at n
   __shared__ int seg[kThreads];
   __shared__ int sh_top, sh_span;
   if (st->use_seq) return;
   const int n = st->n;
   const int a = blockIdx.x * kPosBlock;
   const int b = min(a + kPosBlock, n);  // the block's positions are [a, b)
+  const int P0 = O.first_pre[a];
+  const int ncell = O.first_pre[b] - P0;
+  const bool fits = ncell <= kLocalCap;
+  float4* com = fits ? s_com : T.com + P0;
+  int* t1 = fits ? s_t1 : T.t1 + P0;
+  int* t2 = fits ? s_t2 : T.t2 + P0;
+  int* skip = fits ? s_skip : T.skip + P0;
+  uint32_t* info = fits ? s_info : T.info + P0;
   int* span = O.span + (size_t)blockIdx.x * (kMaxDepth + 1);
   if (threadIdx.x <= kMaxDepth) span[threadIdx.x] = -1;
   if (threadIdx.x == 0) {
     sh_top = 0;
     sh_span = -1;
   }
+  if (fits)
+    for (int i = threadIdx.x; i < ncell; i += kThreads) {
+      const int p = P0 + i;
+      const uint32_t u = T.info[p];
+      s_info[i] = u;
+      s_skip[i] = T.skip[p];
+      if (u & kLeafBit) {
+        s_com[i] = T.com[p];
+        s_t1[i] = T.t1[p];
+        s_t2[i] = T.t2[p];
+      }
+    }
   // suffix minima of lcp over (a, b], with lcp[b] = -1 when b == n
   {
     const int t = threadIdx.x;
-    int v[kPosRounds];
+    constexpr int kSeg = kPosBlock / kThreads;
+    int v[kSeg];
     int m = 127;
-    for (int i = kPosRounds - 1; i >= 0; --i) {
-      const int x = a + 1 + t * kPosRounds + i;
+    for (int i = kSeg - 1; i >= 0; --i) {
+      const int x = a + 1 + t * kSeg + i;
       int l = 127;
       if (x <= b) l = x < n ? (int)O.lcp[x] : -1;
       m = min(m, l);
@@ -741,7 +853,7 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     __syncthreads();
     int after = 127;
     for (int q = t + 1; q < kThreads; ++q) after = min(after, seg[q]);
-    for (int i = 0; i < kPosRounds; ++i) smin[1 + t * kPosRounds + i] = (int8_t)min(v[i], after);
+    for (int i = 0; i < kSeg; ++i) smin[1 + t * kSeg + i] = (int8_t)min(v[i], after);
   }
   __syncthreads();
   int l0v[kPosRounds], Lv[kPosRounds], pv[kPosRounds], mv[kPosRounds];
@@ -757,11 +869,12 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     const int L = leaf_depth(l0, O.lcp[j + 1]);
     l0v[i] = l0;
     Lv[i] = L;
-    pv[i] = O.first_pre[j] - l0 - 1;
+    pv[i] = O.first_pre[j] - l0 - 1 - P0;  // local index of cell (j, d) = pv + d
     mv[i] = smin[j + 1 - a];
     top = max(top, L - 1);
     for (int d = l0 + 1; d < L && d <= mv[i]; ++d) {  // internal cells containing position b
-      span[d] = pv[i] + d;
+      span[d] = P0 + pv[i] + d;
+      info[pv[i] + d] |= kSpanBit;
       spantop = max(spantop, d);
     }
   }
@@ -769,28 +882,111 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
   if (spantop >= 0) atomicMax(&sh_span, spantop);
   __syncthreads();
   for (int d = sh_top; d >= 1; --d) {
-    for (int i = 0; i < kPosRounds; ++i)
-      if (l0v[i] < d && d < Lv[i] && mv[i] < d) cell_reduce(pv[i] + d, T, rec, tp);
+    for (int i = 0; i < kPosRounds; ++i) {
+      if (!(l0v[i] < d && d < Lv[i] && mv[i] < d)) continue;
+      const int ci = pv[i] + d;
+      int ch[8];
+      const int nc = local_children(ci, d, ncell, P0, info, skip, ch);
+      float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+      int r1 = INT_MAX, r2 = INT_MAX;
+      for (int q = nc - 1; q >= 0; --q) {  // octants ascending (:514-538)
+        const float4 u = com[ch[q]];
+        tm = add_rn(tm, u.w);
+        sx = add_rn(sx, mul_rn(u.x, u.w));
+        sy = add_rn(sy, mul_rn(u.y, u.w));
+        sz = add_rn(sz, mul_rn(u.z, u.w));
+        const int ra = t1[ch[q]], rb = t2[ch[q]];
+        if (ra < r1) {
+          r2 = min(r1, rb);
+          r1 = ra;
+        } else if (ra < r2) {
+          r2 = ra;
+        }
+      }
+      float4 res = make_float4(0.f, 0.f, 0.f, tm);
+      if (tm > 0.0f) {
+        res.x = div_rn(sx, tm);
+        res.y = div_rn(sy, tm);
+        res.z = div_rn(sz, tm);
+      }
+      com[ci] = res;
+      t1[ci] = r1;
+      t2[ci] = r2;
+    }
     __syncthreads();
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (b == n) cell_reduce(0, T, rec, tp);
-    else span[0] = 0;
+  // results of the block's internal cells (not the straddling ones) to global
+  // memory: centre of mass, ranks, child table, the children's creation steps,
+  // the traversal record
+  for (int i = threadIdx.x; i < ncell; i += kThreads) {
+    const uint32_t u = info[i];
+    if (u & (kLeafBit | kSpanBit)) continue;
+    const int p = P0 + i;
+    const float4 res = com[i];
+    const int r2 = t2[i];
+    if (fits) {
+      T.com[p] = res;
+      T.t1[p] = t1[i];
+      T.t2[p] = r2;
+    }
+    int ch[8];
+    const int nc = local_children(i, (int)(u & 0xFFu), ncell, P0, info, skip, ch);
+    int slot[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    for (int q = 0; q < nc; ++q) {
+      slot[(info[ch[q]] >> 8) & 7u] = P0 + ch[q];
+      T.k[P0 + ch[q]] = max(r2, t1[ch[q]]);
+    }
+    int4* out = reinterpret_cast<int4*>(T.child + (size_t)p * 8);
+    out[0] = make_int4(slot[0], slot[1], slot[2], slot[3]);
+    out[1] = make_int4(slot[4], slot[5], slot[6], slot[7]);
+    write_trav(rec, p, skip[i] - p, res, T.box[p], false, tp);
+  }
+  if (!fits) {  // the span marks went to the global info array: clear them
+    __syncthreads();
+    for (int d = 1; d <= kMaxDepth; ++d)
+      if (threadIdx.x == 0 && span[d] >= 0) T.info[span[d]] &= ~kSpanBit;
+  }
+  // child tables of this block's straddling cells (the root's too, in block 0),
+  // one thread per depth: every skip is known, so the walk needs no other block
+  if (threadIdx.x <= kMaxDepth) {
+    const int p = (threadIdx.x == 0) ? (blockIdx.x == 0 ? 0 : -1) : span[threadIdx.x];
+    if (p >= 0) {
+      int c[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+      const int end = T.skip[p];
+      for (int q = p + 1, it = 0; q < end && it < 8; ++it) {  // children 7..0
+        c[(T.info[q] >> 8) & 7u] = q;
+        const int next = T.skip[q];
+        q = next > q ? next : end;
+      }
+      int4* out = reinterpret_cast<int4*>(T.child + (size_t)p * 8);
+      out[0] = make_int4(c[0], c[1], c[2], c[3]);
+      out[1] = make_int4(c[4], c[5], c[6], c[7]);
+    }
   }
   if (threadIdx.x == 0) {
-    const int sd = (blockIdx.x == 0 && b < n) ? max(sh_span, 0) : sh_span;
-    if (sd >= 0) atomicMax(&st->span_depth, sd);
+    if (blockIdx.x == 0) span[0] = 0;
+    O.bstat[3 * blockIdx.x + 2] = blockIdx.x == 0 ? max(sh_span, 0) : sh_span;
   }
 }
 
-// One workgroup: the straddling cells, deepest first. Each one's children are
-// inside some block (done by cell_local) or straddling one level deeper.
+// One workgroup: the straddling cells, deepest first (cell_local wrote their
+// child tables); each one's children were done by cell_local, by cell_emit
+// (leaves), or one level deeper here.
 __global__ __launch_bounds__(1024) void cell_span(const BhState* __restrict__ st, PathOrder O,
                                                   TmpNodes T, float4* rec, TravParams tp) {
+  __shared__ int sh_top;
   if (st->use_seq) return;
   const int nb = (st->n + kPosBlock - 1) / kPosBlock;
-  if (nb <= 1) return;
-  for (int d = st->span_depth; d >= 0; --d) {
+  if (threadIdx.x == 0) sh_top = 0;
+  __syncthreads();
+  {
+    int t = 0;
+    for (int k = threadIdx.x; k < nb; k += 1024) t = max(t, O.bstat[3 * k + 2]);
+    if (t) atomicMax(&sh_top, t);
+  }
+  __syncthreads();
+  const int top = sh_top;
+  for (int d = top; d >= 0; --d) {
     for (int k = threadIdx.x; k < nb; k += 1024) {
       const int p = O.span[(size_t)k * (kMaxDepth + 1) + d];
       if (p >= 0) cell_reduce(p, T, rec, tp);
@@ -1197,17 +1393,21 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 
 __global__ __launch_bounds__(kThreads) void bh_traverse(
     const float4* __restrict__ rec, const BhState* __restrict__ st,
-    const int32_t* __restrict__ order, const float* __restrict__ x, const float* __restrict__ y,
+    const int32_t* __restrict__ morton_order, const int32_t* __restrict__ path_order,
+    const float* __restrict__ x, const float* __restrict__ y,
     const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
     float eps2, float* ax, float* ay, float* az) {
   const int lane = threadIdx.x & 63;
-  // Workgroups are dealt to the 8 XCDs round-robin; each XCD takes one
-  // contiguous run of Morton order instead, so that its L2 holds the part of
-  // the tree its waves walk.
+  // Waves take 64 consecutive particles in the tree's own order (path order;
+  // Morton order after the sequential builder): bodies of one cell share a
+  // wave, however many share a Morton code. Workgroups are dealt to the 8 XCDs
+  // round-robin; each XCD takes one contiguous run of that order instead, so
+  // that its L2 holds the part of the tree its waves walk.
+  const int32_t* __restrict__ order = st->use_seq ? morton_order : path_order;
   const int nb = (int)gridDim.x, per = nb / 8, rem = nb % 8;
   const int xcd = (int)blockIdx.x % 8, slot = (int)blockIdx.x / 8;
   const int blk = xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
-  // wave w: particles [w * 64, (w + 1) * 64) in Morton order
+  // wave w: particles [w * 64, (w + 1) * 64) of that order
   const int r = (blk * (kThreads / 64) + (int)(threadIdx.x >> 6)) * 64 + lane;
   const bool valid = r < n;
   const int ip = valid ? order[r] : 0;
@@ -1451,16 +1651,22 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->T.skip, NN);
   A(&t->T.size, NN);
   A(&t->T.pre, NN);
-  t->pos_blocks = (int)((P + kPosBlock - 1) / kPosBlock);
+  t->pos_blocks = (int)(P / kPosBlock + 1);
+  const size_t supers = (size_t)t->pos_blocks / kSuperBlocks + 1;
   A(&t->O.key, P);
   A(&t->O.key_hi, P);
   A(&t->O.rank, P);
   A(&t->O.spath, P);
-  A(&t->O.lcp, P + 1);
+  A(&t->O.lcp, (size_t)t->pos_blocks * kPosBlock + 64);
+  A(&t->O.lmin1, (size_t)t->pos_blocks * (kPosBlock / 64) + 64);
+  A(&t->O.lmin2, supers * kSuperBlocks + 64);
+  A(&t->O.lmin3, (supers / 64 + 1) * 64 + 64);
   A(&t->O.first_pre, P + 1);
   A(&t->O.bsum, (size_t)t->pos_blocks);
   A(&t->O.span, (size_t)t->pos_blocks * (kMaxDepth + 1));
   A(&t->O.runs, P);
+  A(&t->O.bstat, (size_t)t->pos_blocks * 3);
+  A(&t->O.ppart, P);
   A(&t->nkeys, NN);
   A(&t->nvals, NN);
   A(&t->final_of, NN);
@@ -1517,12 +1723,12 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   hipLaunchKernelGGL(path_gather_key, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, &t->st->n,
                      t->O);
   NBX_TRY(t->sort_particles.sort(t->O.key, t->O.rank, &t->st->n, 30, s));
-  hipLaunchKernelGGL(path_lcp_kernel, dim3(div_up((size_t)n + 1, kThreads)), dim3(kThreads), 0, s,
+  hipLaunchKernelGGL(path_lcp_kernel, dim3(div_up((size_t)n + 64, kThreads)), dim3(kThreads), 0, s,
                      t->st, t->path, t->O);
   hipLaunchKernelGGL(path_refine, dim3(grid_for((size_t)n * 64, 1024)), dim3(kThreads), 0, s,
                      t->st, t->path, t->O);
   const int pb = (int)div_up(n, kPosBlock);
-  hipLaunchKernelGGL(cell_count, dim3(pb), dim3(kThreads), 0, s, t->st, t->O);
+  hipLaunchKernelGGL(cell_count, dim3(n / kPosBlock + 1), dim3(kThreads), 0, s, t->st, t->O);
   hipLaunchKernelGGL(cell_offsets, dim3(1), dim3(1024), 0, s, t->st, t->O, t->T, t->max_nodes);
   NBX_LAUNCHED();
   if (t->force_seq) {
@@ -1535,8 +1741,6 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   hipLaunchKernelGGL(cell_first_pre, dim3(pb), dim3(kThreads), 0, s, t->st, t->O);
   hipLaunchKernelGGL(cell_emit, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->st, t->O,
                      t->T, t->idx, t->sbody, t->trav_rec, t->prep);
-  hipLaunchKernelGGL(cell_link, dim3(grid_for((size_t)t->max_nodes)), dim3(kThreads), 0, s, t->st,
-                     t->T);
   NBX_LAUNCHED();
   mark(t, 5, s);
   hipLaunchKernelGGL(cell_local, dim3(pb), dim3(kThreads), 0, s, t->st, t->O, t->T, t->trav_rec,
@@ -1574,7 +1778,7 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
     t->prep = want;
   }
   hipLaunchKernelGGL(bh_traverse, dim3(div_up(n, kThreads)), dim3(kThreads),
-                     0, s, t->trav_rec, t->st, t->idx, d->pos_x, d->pos_y, d->pos_z,
+                     0, s, t->trav_rec, t->st, t->idx, t->O.ppart, d->pos_x, d->pos_y, d->pos_z,
                      d->mass, n, theta2, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   mark(t, 9, s);
