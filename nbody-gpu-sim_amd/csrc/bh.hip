@@ -340,6 +340,7 @@ constexpr int kPosBlock = 1024;  // positions per workgroup in the cell passes (
 constexpr int kPosRounds = kPosBlock / kThreads;
 constexpr int kSuperBlocks = 64;
 constexpr int kLocalCap = 2048;  // cells of one block held in LDS by cell_local
+constexpr uint32_t kLeafBit = 1u << 11;  // in TmpNodes::info
 constexpr int kRefineMax = 1024;
 constexpr int kSortDigits = 20;
 
@@ -676,12 +677,14 @@ __global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict_
   const int j = blockIdx.x * kThreads + threadIdx.x;
   if (j >= n) return;
   const int r = O.rank[j];
-  O.ppart[j] = sidx[r];
+  const int pid = sidx[r];
+  O.ppart[j] = pid;
   const int l0 = O.lcp[j], l1 = O.lcp[j + 1];
   const int L = leaf_depth(l0, l1);
   if (L <= l0) return;  // not the first particle of any cell (depth-32 duplicate)
   const int pre0 = O.first_pre[j] - l0 - 1;  // pre(j, d) = pre0 + d
   const uint4 pj = O.spath[j];
+  float4 q = sbody[r];  // the leaf's particle: load issued before the chain
   float cx = st->root_c[0], cy = st->root_c[1], cz = st->root_c[2], h = st->root_h;
   for (int d = 1; d <= L; ++d) {
     const int o = path_digit(pj, d - 1);
@@ -692,16 +695,13 @@ __global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict_
     h = hs;
     if (d <= l0) continue;
     const int p = pre0 + d;
-    const bool leaf = d == L;
-    const float4 box = make_float4(cx, cy, cz, h);
-    T.box[p] = box;
-    T.info[p] = (uint32_t)d | ((uint32_t)o << 8) | (leaf ? (1u << 11) : 0u);
-    if (!leaf) {
-      T.pidx[p] = -1;
-      continue;
-    }
+    T.box[p] = make_float4(cx, cy, cz, h);
+    T.info[p] = (uint32_t)d | ((uint32_t)o << 8) | (d == L ? kLeafBit : 0u);
+    if (d < L) T.pidx[p] = -1;
+  }
+  {  // the leaf (j, L): its particle, or the depth-32 merge in insertion order
+    const int p = pre0 + L;
     int e = j + 1;
-    float4 q = sbody[r];
     int t2 = INT_MAX;
     if (l1 == kMaxDepth) {  // depth-32 cell with several particles, ranks ascending
       t2 = O.rank[j + 1];
@@ -716,12 +716,12 @@ __global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict_
       }
     }
     T.com[p] = q;
-    T.pidx[p] = sidx[r];
+    T.pidx[p] = pid;
     T.t1[p] = r;
     T.t2[p] = t2;
     T.count[p] = e - j;
     T.skip[p] = p + 1;
-    write_trav(rec, p, 1, q, box, true, tp);
+    write_trav(rec, p, 1, q, make_float4(cx, cy, cz, h), true, tp);
   }
   // Ends of the internal cells, innermost first: the run of (j, d) extends the
   // run of (j, d + 1) to the next position with fewer than d shared digits.
@@ -776,7 +776,6 @@ __device__ void cell_reduce(int p, TmpNodes T, float4* rec, TravParams tp) {
   write_trav(rec, p, T.skip[p] - p, com, T.box[p], false, tp);
 }
 
-constexpr uint32_t kLeafBit = 1u << 11;
 constexpr uint32_t kSpanBit = 1u << 12;  // cell_local only: the cell straddles the block's end
 
 // Children of local cell i (index within the block's cells) in visiting order
@@ -793,6 +792,40 @@ __device__ __forceinline__ int local_children(int i, int d, int ncell, int base,
   return nc;
 }
 
+// Local cell ci at depth d from its children (visiting order, so summed in
+// reverse: octants ascending, :514-538).
+__device__ __forceinline__ void local_reduce(int ci, int d, int ncell, int base,
+                                             const uint32_t* info, const int* skip, float4* com,
+                                             int* t1, int* t2) {
+  int ch[8];
+  const int nc = local_children(ci, d, ncell, base, info, skip, ch);
+  float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+  int r1 = INT_MAX, r2 = INT_MAX;
+  for (int q = nc - 1; q >= 0; --q) {
+    const float4 u = com[ch[q]];
+    tm = add_rn(tm, u.w);
+    sx = add_rn(sx, mul_rn(u.x, u.w));
+    sy = add_rn(sy, mul_rn(u.y, u.w));
+    sz = add_rn(sz, mul_rn(u.z, u.w));
+    const int ra = t1[ch[q]], rb = t2[ch[q]];
+    if (ra < r1) {
+      r2 = min(r1, rb);
+      r1 = ra;
+    } else if (ra < r2) {
+      r2 = ra;
+    }
+  }
+  float4 res = make_float4(0.f, 0.f, 0.f, tm);
+  if (tm > 0.0f) {
+    res.x = div_rn(sx, tm);
+    res.y = div_rn(sy, tm);
+    res.z = div_rn(sz, tm);
+  }
+  com[ci] = res;
+  t1[ci] = r1;
+  t2[ci] = r2;
+}
+
 // Bottom-up over the internal cells whose runs lie inside this block of
 // positions, deepest level first, with the block's cells in LDS (in global
 // memory when there are more than kLocalCap). The cells that contain the
@@ -805,6 +838,8 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
   __shared__ uint32_t s_info[kLocalCap];
   __shared__ int8_t smin[kPosBlock + 1];  // smin[x - a] = min(lcp[x..b]), -1 at n
   __shared__ int seg[kThreads];
+  __shared__ uint16_t lvl_list[kLocalCap];  // local internal cells grouped by depth
+  __shared__ int lvl_cnt[kMaxDepth + 1], lvl_start[kMaxDepth + 1], lvl_fill[kMaxDepth + 1];
   __shared__ int sh_top, sh_span;
   if (st->use_seq) return;
   const int n = st->n;
@@ -880,40 +915,46 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
   }
   if (top) atomicMax(&sh_top, top);
   if (spantop >= 0) atomicMax(&sh_span, spantop);
+  if (threadIdx.x <= kMaxDepth) {
+    lvl_cnt[threadIdx.x] = 0;
+    lvl_fill[threadIdx.x] = 0;
+  }
   __syncthreads();
-  for (int d = sh_top; d >= 1; --d) {
-    for (int i = 0; i < kPosRounds; ++i) {
-      if (!(l0v[i] < d && d < Lv[i] && mv[i] < d)) continue;
-      const int ci = pv[i] + d;
-      int ch[8];
-      const int nc = local_children(ci, d, ncell, P0, info, skip, ch);
-      float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-      int r1 = INT_MAX, r2 = INT_MAX;
-      for (int q = nc - 1; q >= 0; --q) {  // octants ascending (:514-538)
-        const float4 u = com[ch[q]];
-        tm = add_rn(tm, u.w);
-        sx = add_rn(sx, mul_rn(u.x, u.w));
-        sy = add_rn(sy, mul_rn(u.y, u.w));
-        sz = add_rn(sz, mul_rn(u.z, u.w));
-        const int ra = t1[ch[q]], rb = t2[ch[q]];
-        if (ra < r1) {
-          r2 = min(r1, rb);
-          r1 = ra;
-        } else if (ra < r2) {
-          r2 = ra;
-        }
-      }
-      float4 res = make_float4(0.f, 0.f, 0.f, tm);
-      if (tm > 0.0f) {
-        res.x = div_rn(sx, tm);
-        res.y = div_rn(sy, tm);
-        res.z = div_rn(sz, tm);
-      }
-      com[ci] = res;
-      t1[ci] = r1;
-      t2[ci] = r2;
+  if (fits) {
+    // the local internal cells by depth, then one parallel round per level
+    for (int i = threadIdx.x; i < ncell; i += kThreads) {
+      const uint32_t u = s_info[i];
+      if (!(u & (kLeafBit | kSpanBit))) atomicAdd(&lvl_cnt[u & 0xFFu], 1);
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+      int run = 0;
+      for (int d = 0; d <= kMaxDepth; ++d) {
+        lvl_start[d] = run;
+        run += lvl_cnt[d];
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ncell; i += kThreads) {
+      const uint32_t u = s_info[i];
+      if (!(u & (kLeafBit | kSpanBit))) {
+        const int d = (int)(u & 0xFFu);
+        lvl_list[lvl_start[d] + atomicAdd(&lvl_fill[d], 1)] = (uint16_t)i;
+      }
+    }
+    __syncthreads();
+    for (int d = sh_top; d >= 1; --d) {
+      for (int q = threadIdx.x; q < lvl_cnt[d]; q += kThreads)
+        local_reduce(lvl_list[lvl_start[d] + q], d, ncell, P0, s_info, s_skip, s_com, s_t1, s_t2);
+      __syncthreads();
+    }
+  } else {
+    for (int d = sh_top; d >= 1; --d) {
+      for (int i = 0; i < kPosRounds; ++i)
+        if (l0v[i] < d && d < Lv[i] && mv[i] < d)
+          local_reduce(pv[i] + d, d, ncell, P0, info, skip, com, t1, t2);
+      __syncthreads();
+    }
   }
   // results of the block's internal cells (not the straddling ones) to global
   // memory: centre of mass, ranks, child table, the children's creation steps,
