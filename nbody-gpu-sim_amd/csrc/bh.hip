@@ -169,16 +169,22 @@ __global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int
 
 // ---------------------------------------------------------------- 2: morton
 
+// Also packs each particle as (x, y, z, m), so that the paths pass gathers one
+// 16-byte record per particle through the sort permutation instead of four.
 __global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restrict__ x,
                                                           const float* __restrict__ y,
-                                                          const float* __restrict__ z, int n,
+                                                          const float* __restrict__ z,
+                                                          const float* __restrict__ m, int n,
                                                           const BhState* __restrict__ st,
-                                                          uint32_t* codes, int32_t* idx) {
+                                                          uint32_t* codes, int32_t* idx,
+                                                          float4* packed) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
-  const float nx = mul_rn(sub_rn(x[i], st->bmin[0]), st->inv[0]);
-  const float ny = mul_rn(sub_rn(y[i], st->bmin[1]), st->inv[1]);
-  const float nz = mul_rn(sub_rn(z[i], st->bmin[2]), st->inv[2]);
+  const float px = x[i], py = y[i], pz = z[i];
+  packed[i] = make_float4(px, py, pz, m[i]);
+  const float nx = mul_rn(sub_rn(px, st->bmin[0]), st->inv[0]);
+  const float ny = mul_rn(sub_rn(py, st->bmin[1]), st->inv[1]);
+  const float nz = mul_rn(sub_rn(pz, st->bmin[2]), st->inv[2]);
   codes[i] = (expand_bits(quantise(nx)) << 2) | (expand_bits(quantise(ny)) << 1) |
              expand_bits(quantise(nz));
   idx[i] = i;
@@ -188,10 +194,7 @@ __global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restric
 
 // Also copies each particle to its rank as (x, y, z, m), and writes the keys of
 // the path sort: digits 10..19 (sorted first) and digits 0..9 (see section 5).
-__global__ __launch_bounds__(kThreads) void path_kernel(const float* __restrict__ x,
-                                                        const float* __restrict__ y,
-                                                        const float* __restrict__ z,
-                                                        const float* __restrict__ m,
+__global__ __launch_bounds__(kThreads) void path_kernel(const float4* __restrict__ packed,
                                                         const int32_t* __restrict__ sidx, int n,
                                                         const BhState* __restrict__ st,
                                                         uint4* path, float4* sbody,
@@ -199,9 +202,9 @@ __global__ __launch_bounds__(kThreads) void path_kernel(const float* __restrict_
                                                         int32_t* rank) {
   const int r = blockIdx.x * kThreads + threadIdx.x;
   if (r >= n) return;
-  const int p = sidx[r];
-  const float px = x[p], py = y[p], pz = z[p];
-  sbody[r] = make_float4(px, py, pz, m[p]);
+  const float4 b = packed[sidx[r]];
+  const float px = b.x, py = b.y, pz = b.z;
+  sbody[r] = b;
   float cx = st->root_c[0], cy = st->root_c[1], cz = st->root_c[2], h = st->root_h;
   uint32_t w[4] = {0, 0, 0, 0};
   for (int d = 0; d < kMaxDepth; ++d) {
@@ -1748,15 +1751,16 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   hipLaunchKernelGGL(bbox_final, dim3(1), dim3(kThreads), 0, s, t->bbox_part, bb, n, t->st);
   NBX_LAUNCHED();
   mark(t, 1, s);
+  // the packed particles borrow the path-order buffer, which path_lcp_kernel fills later
+  float4* packed = reinterpret_cast<float4*>(t->O.spath);
   hipLaunchKernelGGL(morton_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, d->pos_x,
-                     d->pos_y, d->pos_z, n, t->st, t->codes, t->idx);
+                     d->pos_y, d->pos_z, d->mass, n, t->st, t->codes, t->idx, packed);
   NBX_LAUNCHED();
   mark(t, 2, s);
   NBX_TRY(t->sort_particles.sort(t->codes, t->idx, &t->st->n, 30, s));
   mark(t, 3, s);
-  hipLaunchKernelGGL(path_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, d->pos_x,
-                     d->pos_y, d->pos_z, d->mass, t->idx, n, t->st, t->path, t->sbody, t->O.key,
-                     t->O.key_hi, t->O.rank);
+  hipLaunchKernelGGL(path_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, packed,
+                     t->idx, n, t->st, t->path, t->sbody, t->O.key, t->O.key_hi, t->O.rank);
   NBX_LAUNCHED();
   mark(t, 4, s);
   // path order: stable sort by digits 10..19, then by digits 0..9

@@ -24,9 +24,9 @@ __global__ __launch_bounds__(kThreads) void radix_count(const uint32_t* __restri
   table[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major table, in two launches: each workgroup scans
-// one digit's row over the tiles (and records the row total); then one small
-// workgroup turns the 256 row totals into digit bases, added during scatter.
+// Exclusive scan of each digit's row of the digit-major table over the tiles,
+// one workgroup per digit, recording the row totals; every scatter workgroup
+// turns the 256 totals into digit bases itself.
 __global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* table, int tiles,
                                                        uint32_t* row_total) {
   __shared__ uint32_t wsum[4];
@@ -55,20 +55,24 @@ __global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* table, int tile
   if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(256) void radix_scan_bases(uint32_t* row_total) {
-  __shared__ uint32_t sh[256];
-  sh[threadIdx.x] = row_total[threadIdx.x];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int d = 0; d < 256; ++d) {
-      const uint32_t v = sh[d];
-      sh[d] = run;
-      run += v;
-    }
+// Exclusive scan of one value per thread over the workgroup (kThreads).
+__device__ int block_scan_u32(uint32_t v, int* total) {
+  __shared__ uint32_t ws[kThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
   }
+  if (lane == 63) ws[w] = incl;
   __syncthreads();
-  row_total[threadIdx.x] = sh[threadIdx.x];
+  uint32_t before = 0, tot = 0;
+  for (int q = 0; q < kThreads / 64; ++q) {
+    if (q < w) before += ws[q];
+    tot += ws[q];
+  }
+  *total = (int)tot;
+  return (int)(before + incl - v);
 }
 
 __device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid, int nbits) {
@@ -84,14 +88,20 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t digit, bool valid, int 
 __global__ __launch_bounds__(kThreads) void radix_scatter(
     const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     int32_t* __restrict__ vout, const uint32_t* __restrict__ table,
-    const uint32_t* __restrict__ bases, const int32_t* __restrict__ n_dev, int shift, int nbits,
-    int tiles) {
+    const uint32_t* __restrict__ row_total, const int32_t* __restrict__ n_dev, int shift,
+    int nbits, int tiles) {
   constexpr int R = RadixSort::kRounds;
   constexpr int W = RadixSort::kWaves;
+  constexpr int T = RadixSort::kTile;
   __shared__ uint32_t wave_hist[W][kBins];
+  __shared__ uint32_t local_start[kBins];  // first slot of each digit in the tile's sorted order
+  __shared__ uint32_t gstart[kBins];       // first output position of each digit for this tile
+  __shared__ uint32_t skey[T];
+  __shared__ int32_t sval[T];
   const int n = *n_dev;
-  const int base = blockIdx.x * RadixSort::kTile;
+  const int base = blockIdx.x * T;
   if (base >= n) return;
+  const int tile_n = min(T, n - base);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   for (int k = threadIdx.x; k < W * kBins; k += kThreads) (&wave_hist[0][0])[k] = 0;
@@ -117,10 +127,20 @@ __global__ __launch_bounds__(kThreads) void radix_scatter(
     if (valid && (m & lt) == 0) wave_hist[w][dg] += (uint32_t)__popcll(m);
   }
   __syncthreads();
-  // per-wave starting offsets: tile offset + earlier waves of the tile
+  // tile-local digit starts (exclusive scan of the tile's digit counts), each
+  // wave's first slot per digit, and the tile's output position per digit
   {
     const int d = threadIdx.x;  // kThreads == kBins
-    uint32_t run = bases[d] + table[d * tiles + blockIdx.x];
+    uint32_t tot = 0;
+    for (int q = 0; q < W; ++q) tot += wave_hist[q][d];
+    int total;
+    const uint32_t start = (uint32_t)block_scan_u32(tot, &total);
+    __syncthreads();  // block_scan_u32's LDS is reused below
+    local_start[d] = start;
+    // digit bases: exclusive scan of the row totals (every workgroup does it)
+    const uint32_t dbase = (uint32_t)block_scan_u32(row_total[d], &total);
+    gstart[d] = dbase + table[d * tiles + blockIdx.x];
+    uint32_t run = start;
     for (int q = 0; q < W; ++q) {
       const uint32_t c = wave_hist[q][d];
       wave_hist[q][d] = run;
@@ -128,7 +148,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter(
     }
   }
   __syncthreads();
-  // pass 2: stable scatter
+  // pass 2: stable rank of every key in the tile's sorted order, into LDS
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const bool valid = wbase + r * 64 + lane < n;
@@ -137,11 +157,21 @@ __global__ __launch_bounds__(kThreads) void radix_scatter(
     const uint32_t pos = (valid ? wave_hist[w][dg] : 0u) + (uint32_t)__popcll(m & lt);
     __builtin_amdgcn_wave_barrier();
     if (valid) {
-      kout[pos] = key[r];
-      vout[pos] = val[r];
+      skey[pos] = key[r];
+      sval[pos] = val[r];
       if ((m & lt) == 0) wave_hist[w][dg] += (uint32_t)__popcll(m);
     }
     __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // pass 3: the tile in sorted order, each digit's run to its output range
+  // (consecutive slots go to consecutive addresses)
+  for (int i = threadIdx.x; i < tile_n; i += kThreads) {
+    const uint32_t k = skey[i];
+    const uint32_t dg = (k >> shift) & dmask;
+    const uint32_t o = gstart[dg] + ((uint32_t)i - local_start[dg]);
+    kout[o] = k;
+    vout[o] = sval[i];
   }
 }
 
@@ -186,7 +216,6 @@ int RadixSort::sort(uint32_t* keys, int32_t* vals, const int32_t* n_dev, int key
     hipLaunchKernelGGL(radix_count, dim3(max_tiles), dim3(kThreads), 0, s, ka, n_dev, shift, table,
                        max_tiles);
     hipLaunchKernelGGL(radix_scan_rows, dim3(kBins), dim3(256), 0, s, table, max_tiles, bases);
-    hipLaunchKernelGGL(radix_scan_bases, dim3(1), dim3(256), 0, s, bases);
     hipLaunchKernelGGL(radix_scatter, dim3(max_tiles), dim3(kThreads), 0, s, ka, va, kb, vb, table,
                        bases, n_dev, shift, nbits, max_tiles);
     NBX_LAUNCHED();

@@ -5,10 +5,12 @@
 // (src/cuda/force_barnes_hut.cu:324-328). 8-bit digits; per pass:
 //   count   - per-tile digit histogram (LDS atomics)
 //   scan    - exclusive scan of each digit's row over the tiles (one workgroup
-//             per digit), then the 256 digit bases
-//   scatter - in-tile stable ranking by wave ballots (match-any over the digit
-//             bits, rank = popcount of lower matching lanes), waves own
-//             contiguous sub-tiles so order is kept, then scattered writes.
+//             per digit), with the row totals
+//   scatter - digit bases from the row totals; in-tile stable ranking by wave
+//             ballots (match-any over the digit bits, rank = popcount of lower
+//             matching lanes), waves own contiguous sub-tiles so order is kept;
+//             the tile is staged in LDS in sorted order, so each digit's run is
+//             written to consecutive addresses.
 // The element count is read from device memory, so a sort can follow a kernel
 // that produced its length without a host round trip.
 #pragma once
@@ -19,14 +21,14 @@ namespace nbx {
 
 struct RadixSort {
   static constexpr int kWaves = 4;
-  static constexpr int kRounds = 16;
+  static constexpr int kRounds = 8;
   static constexpr int kTile = kWaves * kRounds * 64;  // keys per workgroup
   int max_n = 0;
   int max_tiles = 0;
   uint32_t* keys_tmp = nullptr;
   int32_t* vals_tmp = nullptr;
   uint32_t* table = nullptr;  // [256][max_tiles]
-  uint32_t* bases = nullptr;  // [256] digit bases of the current pass
+  uint32_t* bases = nullptr;  // [256] row totals (keys per digit) of the current pass
 
   int init(int max_n);
   void release();
