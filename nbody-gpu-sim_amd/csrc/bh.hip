@@ -317,6 +317,7 @@ struct TmpNodes {
   float4* box;      // centre, half size
   float4* com;      // centre of mass, total mass
   int32_t* skip;    // preorder position after the cell's subtree
+  int32_t* sowner;  // internal cell straddling a block of positions: its block; else -1
   int32_t* pre;     // sequential builder: scratch
   int32_t* size;    // sequential builder: scratch
 };
@@ -330,19 +331,43 @@ struct PathOrder {
   uint8_t* lcp;       // n + 1 entries
   int32_t* first_pre; // n + 1 entries
   int32_t* bsum;      // per block of kPosBlock positions: cells, then their offsets
-  int32_t* span;      // per block: [kMaxDepth + 1] straddling cells by depth, or -1
+  int32_t* lvl;       // straddling cells by (hierarchy level, unit, depth), or -1
   int32_t* runs;      // starts of runs sharing 20 digits
   int8_t* lmin1;      // min lcp per chunk of 64 positions
   int8_t* lmin2;      // min lcp per block of kPosBlock positions
   int8_t* lmin3;      // min lcp per kSuperBlocks blocks
-  int32_t* bstat;     // per block: deepest leaf, depth-32 multi leaves, deepest straddling cell
+  int32_t* bstat;     // per block: deepest leaf, depth-32 multi leaves
   int32_t* ppart;     // particle index at every position: the traversal's order
 };
 
-constexpr int kPosBlock = 1024;  // positions per workgroup in the cell passes (16 chunks)
+constexpr int kPosBlock = 1024;  // positions per workgroup of the counting passes (16 chunks)
 constexpr int kPosRounds = kPosBlock / kThreads;
+// positions per block of the bottom-up pass: small enough that four workgroups'
+// cells fit in a CU's LDS at once
+constexpr int kLocalBlock = 512;
+constexpr int kLocalRounds = kLocalBlock / kThreads;
 constexpr int kSuperBlocks = 64;
-constexpr int kLocalCap = 2048;  // cells of one block held in LDS by cell_local
+constexpr int kLocalCap = 1024;  // cells of one block held in LDS by cell_local
+
+// Cells that straddle blocks are reduced by a hierarchy of groups: level 2 takes
+// those whose run stays inside one group of 64 blocks, level 3 inside one group
+// of 64 level-2 groups, level 4 the rest. A cell's unit at level L is its owner
+// block >> 6 (L - 2); at one depth a unit holds at most one such cell (the one
+// containing the unit's last position). lvl[] files them by (L, unit, depth).
+constexpr int kFan = 64;
+constexpr int kMaxLevel = 4;
+__host__ __device__ inline int lvl_units(int nb, int L) {
+  int u = nb;
+  for (int l = 2; l < L; ++l) u = (u + kFan - 1) / kFan;
+  return u;
+}
+__host__ __device__ inline int lvl_base(int nb, int L) {
+  int off = 0;
+  for (int l = 2; l < L; ++l) off += lvl_units(nb, l) * (kMaxDepth + 1);
+  return off;
+}
+// the level that reduces the root (it contains every position)
+__host__ __device__ inline int root_level(int nb) { return nb <= kFan ? 2 : (nb <= kFan * kFan ? 3 : 4); }
 constexpr uint32_t kLeafBit = 1u << 11;  // in TmpNodes::info
 constexpr int kRefineMax = 1024;
 constexpr int kSortDigits = 20;
@@ -567,6 +592,14 @@ __global__ __launch_bounds__(1024) void cell_offsets(BhState* st, PathOrder O, T
     if (threadIdx.x == 0) sh_carry += tot;
     __syncthreads();
   }
+  {
+    const int nbl = (n + kLocalBlock - 1) / kLocalBlock;
+    const int total = lvl_base(nbl, kMaxLevel + 1);
+    for (int i = threadIdx.x; i < total; i += 1024) O.lvl[i] = -1;
+    __syncthreads();
+    if (threadIdx.x == 0) O.lvl[lvl_base(nbl, root_level(nbl))] = 0;  // unit 0, depth 0
+    T.sowner[0] = root_level(nbl);  // owner block 0
+  }
   if (threadIdx.x != 0) return;
   const int cells = 1 + sh_carry;
   const int seq = (st->overflow || cells + kMaxDepth + 1 >= max_nodes) ? 1 : 0;  // :449
@@ -724,59 +757,62 @@ __global__ __launch_bounds__(kThreads) void cell_emit(const BhState* __restrict_
     T.t2[p] = t2;
     T.count[p] = e - j;
     T.skip[p] = p + 1;
+    T.sowner[p] = -1;  // leaves are done here, never by cell_group
     write_trav(rec, p, 1, q, make_float4(cx, cy, cz, h), true, tp);
   }
-  // Ends of the internal cells, innermost first: the run of (j, d) extends the
-  // run of (j, d + 1) to the next position with fewer than d shared digits.
+  // internal cells: their first position, for cell_ends
+  for (int d = l0 + 1; d < L; ++d) T.skip[pre0 + d] = j;
+}
+
+// End of every internal cell's run, one thread per cell so that the searches
+// run side by side: the run of (j, d) ends at the next position with fewer
+// than d shared digits. Gives the subtree end (skip) and the particle count.
+__global__ __launch_bounds__(kThreads) void cell_ends(const BhState* __restrict__ st, PathOrder O,
+                                                      TmpNodes T) {
+  if (st->use_seq) return;
+  const int n = st->n, nn = st->final_count;
   const int nsuper = (n / kPosBlock + 1 + kSuperBlocks - 1) / kSuperBlocks;
-  int e = j + 1;
-  while (e < n && O.lcp[e] == kMaxDepth) ++e;
-  for (int d = L - 1; d > l0; --d) {
-    if (O.lcp[e] >= d) e = next_lower(O, e + 1, d, n, nsuper);
-    const int p = pre0 + d;
+  for (int p = 1 + blockIdx.x * kThreads + threadIdx.x; p < nn; p += gridDim.x * kThreads) {
+    const uint32_t u = T.info[p];
+    if (u & kLeafBit) continue;
+    const int j = T.skip[p];  // the first position, written by cell_emit
+    const int e = next_lower(O, j + 1, (int)(u & 0xFFu), n, nsuper);
     T.count[p] = e - j;
     T.skip[p] = O.first_pre[e];
+    const int kb = j / kLocalBlock;
+    if (kb == (e - 1) / kLocalBlock) {
+      T.sowner[p] = -1;  // inside its block: cell_local reduces it
+      continue;
+    }
+    int L = 2;
+    while (L < kMaxLevel &&
+           (long long)e > (((long long)(kb >> (6 * (L - 1))) + 1) << (6 * (L - 1))) * kLocalBlock)
+      ++L;
+    const int nbl = (n + kLocalBlock - 1) / kLocalBlock;
+    T.sowner[p] = kb * 8 + L;
+    O.lvl[lvl_base(nbl, L) + (kb >> (6 * (L - 2))) * (kMaxDepth + 1) + (int)(u & 0xFFu)] = p;
   }
 }
 
-// Internal cell p from the child table, in octant order (:514-538): mass,
-// centre of mass, the two first ranks, the children's creation steps, the record.
-__device__ void cell_reduce(int p, TmpNodes T, float4* rec, TravParams tp) {
-  const int4* cp = reinterpret_cast<const int4*>(T.child + (size_t)p * 8);
-  const int4 c0 = cp[0], c1 = cp[1];
-  const int c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-  float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-  int r1 = INT_MAX, r2 = INT_MAX;
-  for (int o = 0; o < 8; ++o) {
-    if (c[o] < 0) continue;
-    const float4 q = T.com[c[o]];
-    tm = add_rn(tm, q.w);
-    sx = add_rn(sx, mul_rn(q.x, q.w));
-    sy = add_rn(sy, mul_rn(q.y, q.w));
-    sz = add_rn(sz, mul_rn(q.z, q.w));
-    const int a = T.t1[c[o]], b = T.t2[c[o]];
-    if (a < r1) {
-      r2 = min(r1, b);
-      r1 = a;
-    } else if (a < r2) {
-      r2 = a;
+// Children of every internal cell: first child p + 1, each next child at the
+// previous one's skip (children 7..0), stored by octant.
+__global__ __launch_bounds__(kThreads) void cell_link(const BhState* __restrict__ st, TmpNodes T) {
+  if (st->use_seq) return;
+  const int nn = st->final_count;
+  for (int p = blockIdx.x * kThreads + threadIdx.x; p < nn; p += gridDim.x * kThreads) {
+    if (T.info[p] & kLeafBit) continue;
+    int c[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    const int end = T.skip[p];
+    // at most 8 children, and skips only grow: the bounds keep the walk finite
+    for (int q = p + 1, it = 0; q < end && it < 8; ++it) {
+      c[(T.info[q] >> 8) & 7u] = q;
+      const int next = T.skip[q];
+      q = next > q ? next : end;
     }
+    int4* out = reinterpret_cast<int4*>(T.child + (size_t)p * 8);
+    out[0] = make_int4(c[0], c[1], c[2], c[3]);
+    out[1] = make_int4(c[4], c[5], c[6], c[7]);
   }
-  float4 com = make_float4(0.f, 0.f, 0.f, tm);
-  if (tm > 0.0f) {
-    com.x = div_rn(sx, tm);
-    com.y = div_rn(sy, tm);
-    com.z = div_rn(sz, tm);
-  }
-  T.com[p] = com;
-  T.t1[p] = r1;
-  T.t2[p] = r2;
-  for (int o = 0; o < 8; ++o)
-    if (c[o] >= 0) {
-      const int t1c = T.t1[c[o]];
-      T.k[c[o]] = p == 0 ? t1c : max(r2, t1c);  // root's children: created by their first particle
-    }
-  write_trav(rec, p, T.skip[p] - p, com, T.box[p], false, tp);
 }
 
 constexpr uint32_t kSpanBit = 1u << 12;  // cell_local only: the cell straddles the block's end
@@ -833,21 +869,22 @@ __device__ __forceinline__ void local_reduce(int ci, int d, int ncell, int base,
 // positions, deepest level first, with the block's cells in LDS (in global
 // memory when there are more than kLocalCap). The cells that contain the
 // block's last position and the next one (at most one per depth; the root is
-// always one) are listed for cell_span.
+// always one) are left to cell_group.
 __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O, TmpNodes T,
                                                        float4* rec, TravParams tp) {
   __shared__ float4 s_com[kLocalCap];
   __shared__ int s_t1[kLocalCap], s_t2[kLocalCap], s_skip[kLocalCap];
   __shared__ uint32_t s_info[kLocalCap];
-  __shared__ int8_t smin[kPosBlock + 1];  // smin[x - a] = min(lcp[x..b]), -1 at n
+  __shared__ int8_t smin[kLocalBlock + 1];  // smin[x - a] = min(lcp[x..b]), -1 at n
   __shared__ int seg[kThreads];
   __shared__ uint16_t lvl_list[kLocalCap];  // local internal cells grouped by depth
   __shared__ int lvl_cnt[kMaxDepth + 1], lvl_start[kMaxDepth + 1], lvl_fill[kMaxDepth + 1];
-  __shared__ int sh_top, sh_span;
+  __shared__ int sh_top;
+  __shared__ int span[kMaxDepth + 1];  // straddling cells of this block, to unmark (fallback path)
   if (st->use_seq) return;
   const int n = st->n;
-  const int a = blockIdx.x * kPosBlock;
-  const int b = min(a + kPosBlock, n);  // the block's positions are [a, b)
+  const int a = blockIdx.x * kLocalBlock;
+  const int b = min(a + kLocalBlock, n);  // the block's positions are [a, b)
   const int P0 = O.first_pre[a];
   const int ncell = O.first_pre[b] - P0;
   const bool fits = ncell <= kLocalCap;
@@ -856,12 +893,8 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
   int* t2 = fits ? s_t2 : T.t2 + P0;
   int* skip = fits ? s_skip : T.skip + P0;
   uint32_t* info = fits ? s_info : T.info + P0;
-  int* span = O.span + (size_t)blockIdx.x * (kMaxDepth + 1);
   if (threadIdx.x <= kMaxDepth) span[threadIdx.x] = -1;
-  if (threadIdx.x == 0) {
-    sh_top = 0;
-    sh_span = -1;
-  }
+  if (threadIdx.x == 0) sh_top = 0;
   if (fits)
     for (int i = threadIdx.x; i < ncell; i += kThreads) {
       const int p = P0 + i;
@@ -877,7 +910,7 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
   // suffix minima of lcp over (a, b], with lcp[b] = -1 when b == n
   {
     const int t = threadIdx.x;
-    constexpr int kSeg = kPosBlock / kThreads;
+    constexpr int kSeg = kLocalBlock / kThreads;
     int v[kSeg];
     int m = 127;
     for (int i = kSeg - 1; i >= 0; --i) {
@@ -894,9 +927,9 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     for (int i = 0; i < kSeg; ++i) smin[1 + t * kSeg + i] = (int8_t)min(v[i], after);
   }
   __syncthreads();
-  int l0v[kPosRounds], Lv[kPosRounds], pv[kPosRounds], mv[kPosRounds];
-  int top = 0, spantop = -1;
-  for (int i = 0; i < kPosRounds; ++i) {
+  int l0v[kLocalRounds], Lv[kLocalRounds], pv[kLocalRounds], mv[kLocalRounds];
+  int top = 0;
+  for (int i = 0; i < kLocalRounds; ++i) {
     const int j = a + i * kThreads + (int)threadIdx.x;
     l0v[i] = 0;
     Lv[i] = 0;
@@ -913,11 +946,9 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     for (int d = l0 + 1; d < L && d <= mv[i]; ++d) {  // internal cells containing position b
       span[d] = P0 + pv[i] + d;
       info[pv[i] + d] |= kSpanBit;
-      spantop = max(spantop, d);
     }
   }
   if (top) atomicMax(&sh_top, top);
-  if (spantop >= 0) atomicMax(&sh_span, spantop);
   if (threadIdx.x <= kMaxDepth) {
     lvl_cnt[threadIdx.x] = 0;
     lvl_fill[threadIdx.x] = 0;
@@ -953,7 +984,7 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     }
   } else {
     for (int d = sh_top; d >= 1; --d) {
-      for (int i = 0; i < kPosRounds; ++i)
+      for (int i = 0; i < kLocalRounds; ++i)
         if (l0v[i] < d && d < Lv[i] && mv[i] < d)
           local_reduce(pv[i] + d, d, ncell, P0, info, skip, com, t1, t2);
       __syncthreads();
@@ -975,14 +1006,7 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     }
     int ch[8];
     const int nc = local_children(i, (int)(u & 0xFFu), ncell, P0, info, skip, ch);
-    int slot[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-    for (int q = 0; q < nc; ++q) {
-      slot[(info[ch[q]] >> 8) & 7u] = P0 + ch[q];
-      T.k[P0 + ch[q]] = max(r2, t1[ch[q]]);
-    }
-    int4* out = reinterpret_cast<int4*>(T.child + (size_t)p * 8);
-    out[0] = make_int4(slot[0], slot[1], slot[2], slot[3]);
-    out[1] = make_int4(slot[4], slot[5], slot[6], slot[7]);
+    for (int q = 0; q < nc; ++q) T.k[P0 + ch[q]] = max(r2, t1[ch[q]]);
     write_trav(rec, p, skip[i] - p, res, T.box[p], false, tp);
   }
   if (!fits) {  // the span marks went to the global info array: clear them
@@ -990,50 +1014,105 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     for (int d = 1; d <= kMaxDepth; ++d)
       if (threadIdx.x == 0 && span[d] >= 0) T.info[span[d]] &= ~kSpanBit;
   }
-  // child tables of this block's straddling cells (the root's too, in block 0),
-  // one thread per depth: every skip is known, so the walk needs no other block
-  if (threadIdx.x <= kMaxDepth) {
-    const int p = (threadIdx.x == 0) ? (blockIdx.x == 0 ? 0 : -1) : span[threadIdx.x];
-    if (p >= 0) {
-      int c[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-      const int end = T.skip[p];
-      for (int q = p + 1, it = 0; q < end && it < 8; ++it) {  // children 7..0
-        c[(T.info[q] >> 8) & 7u] = q;
-        const int next = T.skip[q];
-        q = next > q ? next : end;
-      }
-      int4* out = reinterpret_cast<int4*>(T.child + (size_t)p * 8);
-      out[0] = make_int4(c[0], c[1], c[2], c[3]);
-      out[1] = make_int4(c[4], c[5], c[6], c[7]);
-    }
-  }
-  if (threadIdx.x == 0) {
-    if (blockIdx.x == 0) span[0] = 0;
-    O.bstat[3 * blockIdx.x + 2] = blockIdx.x == 0 ? max(sh_span, 0) : sh_span;
-  }
 }
 
-// One workgroup: the straddling cells, deepest first (cell_local wrote their
-// child tables); each one's children were done by cell_local, by cell_emit
-// (leaves), or one level deeper here.
-__global__ __launch_bounds__(1024) void cell_span(const BhState* __restrict__ st, PathOrder O,
-                                                  TmpNodes T, float4* rec, TravParams tp) {
-  __shared__ int sh_top;
+// Level L of the hierarchy: one workgroup per group of kFan units reduces the
+// group's straddling cells, deepest first. The group's cells and their child
+// tables are loaded once into LDS; each depth's results stay in LDS, where a
+// child reduced at the same level is found by its unit (T.sowner); every other
+// child was finished before (cell_emit, cell_local, or a lower level).
+__global__ __launch_bounds__(kThreads) void cell_group(const BhState* __restrict__ st, PathOrder O,
+                                                       TmpNodes T, float4* rec, TravParams tp,
+                                                       int L) {
+  __shared__ int cells[kMaxDepth + 1][kFan];
+  __shared__ int4 tab[kMaxDepth + 1][kFan][2];
+  __shared__ float4 r_com[2][kFan];
+  __shared__ int r_t1[2][kFan], r_t2[2][kFan];
+  __shared__ int sh_dmax;
   if (st->use_seq) return;
-  const int nb = (st->n + kPosBlock - 1) / kPosBlock;
-  if (threadIdx.x == 0) sh_top = 0;
+  const int nb = (st->n + kLocalBlock - 1) / kLocalBlock;
+  const int units = lvl_units(nb, L);
+  const int u0 = blockIdx.x * kFan;
+  const int nu = min(kFan, units - u0);
+  const int* lv = O.lvl + lvl_base(nb, L) + (size_t)u0 * (kMaxDepth + 1);
+  if (threadIdx.x == 0) sh_dmax = -1;
   __syncthreads();
-  {
-    int t = 0;
-    for (int k = threadIdx.x; k < nb; k += 1024) t = max(t, O.bstat[3 * k + 2]);
-    if (t) atomicMax(&sh_top, t);
+  int dmax = -1;
+  for (int idx = threadIdx.x; idx < kFan * (kMaxDepth + 1); idx += kThreads) {
+    const int u = idx / (kMaxDepth + 1), d = idx % (kMaxDepth + 1);
+    const int p = u < nu ? lv[idx] : -1;
+    cells[d][u] = p;
+    if (p >= 0) {
+      const int4* cp = reinterpret_cast<const int4*>(T.child + (size_t)p * 8);
+      tab[d][u][0] = cp[0];
+      tab[d][u][1] = cp[1];
+      dmax = max(dmax, d);
+    }
   }
+  if (dmax >= 0) atomicMax(&sh_dmax, dmax);
   __syncthreads();
-  const int top = sh_top;
-  for (int d = top; d >= 0; --d) {
-    for (int k = threadIdx.x; k < nb; k += 1024) {
-      const int p = O.span[(size_t)k * (kMaxDepth + 1) + d];
-      if (p >= 0) cell_reduce(p, T, rec, tp);
+  const int shift = 6 * (L - 2);  // owner block -> unit at this level
+  for (int d = sh_dmax; d >= 0; --d) {
+    const int cur = d & 1;
+    const int u = threadIdx.x;
+    const int p = u < kFan ? cells[d][u] : -1;
+    if (p >= 0) {
+      const int4 c0 = tab[d][u][0], c1 = tab[d][u][1];
+      const int c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+      float4 q[8];
+      int a[8], b[8], own[8];
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        q[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a[o] = b[o] = INT_MAX;
+        own[o] = -1;
+        if (c[o] >= 0) {
+          own[o] = T.sowner[c[o]];
+          q[o] = T.com[c[o]];
+          a[o] = T.t1[c[o]];
+          b[o] = T.t2[c[o]];
+        }
+      }
+      const float4 box = T.box[p];
+      const int skip = T.skip[p];
+      float tm = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+      int r1 = INT_MAX, r2 = INT_MAX;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {  // octants ascending (:514-538)
+        if (c[o] < 0) continue;
+        if (own[o] >= 0 && (own[o] & 7) == L) {  // reduced one depth below, here
+          const int cu = ((own[o] >> 3) >> shift) - u0;
+          q[o] = r_com[cur ^ 1][cu];
+          a[o] = r_t1[cur ^ 1][cu];
+          b[o] = r_t2[cur ^ 1][cu];
+        }
+        tm = add_rn(tm, q[o].w);
+        sx = add_rn(sx, mul_rn(q[o].x, q[o].w));
+        sy = add_rn(sy, mul_rn(q[o].y, q[o].w));
+        sz = add_rn(sz, mul_rn(q[o].z, q[o].w));
+        if (a[o] < r1) {
+          r2 = min(r1, b[o]);
+          r1 = a[o];
+        } else if (a[o] < r2) {
+          r2 = a[o];
+        }
+      }
+      float4 com = make_float4(0.f, 0.f, 0.f, tm);
+      if (tm > 0.0f) {
+        com.x = div_rn(sx, tm);
+        com.y = div_rn(sy, tm);
+        com.z = div_rn(sz, tm);
+      }
+      r_com[cur][u] = com;
+      r_t1[cur][u] = r1;
+      r_t2[cur][u] = r2;
+      T.com[p] = com;
+      T.t1[p] = r1;
+      T.t2[p] = r2;
+#pragma unroll
+      for (int o = 0; o < 8; ++o)
+        if (c[o] >= 0) T.k[c[o]] = p == 0 ? a[o] : max(r2, a[o]);  // root's children: own t1
+      write_trav(rec, p, skip - p, com, box, false, tp);
     }
     __syncthreads();
   }
@@ -1693,6 +1772,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->T.box, NN);
   A(&t->T.com, NN);
   A(&t->T.skip, NN);
+  A(&t->T.sowner, NN);
   A(&t->T.size, NN);
   A(&t->T.pre, NN);
   t->pos_blocks = (int)(P / kPosBlock + 1);
@@ -1707,7 +1787,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->O.lmin3, (supers / 64 + 1) * 64 + 64);
   A(&t->O.first_pre, P + 1);
   A(&t->O.bsum, (size_t)t->pos_blocks);
-  A(&t->O.span, (size_t)t->pos_blocks * (kMaxDepth + 1));
+  A(&t->O.lvl, (size_t)lvl_base((int)(P / kLocalBlock + 1), kMaxLevel + 1) + 64);
   A(&t->O.runs, P);
   A(&t->O.bstat, (size_t)t->pos_blocks * 3);
   A(&t->O.ppart, P);
@@ -1786,12 +1866,20 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   hipLaunchKernelGGL(cell_first_pre, dim3(pb), dim3(kThreads), 0, s, t->st, t->O);
   hipLaunchKernelGGL(cell_emit, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->st, t->O,
                      t->T, t->idx, t->sbody, t->trav_rec, t->prep);
+  hipLaunchKernelGGL(cell_ends, dim3(grid_for((size_t)t->max_nodes, 4096)), dim3(kThreads), 0, s,
+                     t->st, t->O, t->T);
+  hipLaunchKernelGGL(cell_link, dim3(grid_for((size_t)t->max_nodes, 4096)), dim3(kThreads), 0, s,
+                     t->st, t->T);
   NBX_LAUNCHED();
   mark(t, 5, s);
-  hipLaunchKernelGGL(cell_local, dim3(pb), dim3(kThreads), 0, s, t->st, t->O, t->T, t->trav_rec,
-                     t->prep);
-  hipLaunchKernelGGL(cell_span, dim3(1), dim3(1024), 0, s, t->st, t->O, t->T, t->trav_rec,
-                     t->prep);
+  hipLaunchKernelGGL(cell_local, dim3(div_up(n, kLocalBlock)), dim3(kThreads), 0, s, t->st, t->O,
+                     t->T, t->trav_rec, t->prep);
+  {
+    const int nbl = (int)div_up(n, kLocalBlock);
+    for (int L = 2; L <= root_level(nbl); ++L)
+      hipLaunchKernelGGL(cell_group, dim3(div_up(lvl_units(nbl, L), kFan)), dim3(kThreads), 0, s,
+                         t->st, t->O, t->T, t->trav_rec, t->prep, L);
+  }
   NBX_LAUNCHED();
   mark(t, 6, s);
   t->nodes_pending = true;  // the reference node array is numbered and emitted on demand

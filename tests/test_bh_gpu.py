@@ -111,6 +111,26 @@ def test_run_longer_than_a_workgroup_uses_sequential_builder(nbx, orc):
     assert t.build_info()["builder"] == "sequential"
 
 
+def test_block_with_more_cells_than_lds_holds(nbx, orc):
+    # 300 pairs 2e-7 apart inside a 3e-4 cube: their chains put over 2,048 cells
+    # in some blocks of 1,024 positions, which the bottom-up pass then reduces in
+    # global memory instead of LDS
+    h = nbx.gen_plummer(10600, seed=5)
+    rng = np.random.default_rng(5)
+    base = (rng.uniform(0, 3e-4, (300, 3)).astype(np.float32) + np.float32(0.2)).astype(np.float32)
+    for a, k in enumerate(("pos_x", "pos_y", "pos_z")):
+        h[k][10000::2] = base[:, a]
+        h[k][10001::2] = (base[:, a] + np.float32(2e-7)).astype(np.float32)
+    _, t, ref = check_exact(nbx, orc, h)
+    assert t.build_info()["builder"] == "parallel" and ref["max_depth"] == 27
+
+
+def test_plummer_4m_three_levels_of_straddling_cells(nbx, orc):
+    # 4,300,000 bodies: 4,200 blocks, so straddling cells reach the third group
+    # level (runs longer than 64 x 64 blocks); the node array is bit-exact
+    check_exact(nbx, orc, nbx.gen_plummer(4_300_000, seed=9))
+
+
 def force_check(nbx, orc, h, theta, G, eps, targets=None):
     d, t = gpu_build(nbx, h)
     t.forces(d, theta, G, eps)
