@@ -1514,24 +1514,15 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_waitcnt lgkmcnt(0)\n"                                                                  \
   "s_mov_b64 %[M], s[56:57]\n"
 
-__global__ __launch_bounds__(kThreads) void bh_traverse(
-    const float4* __restrict__ rec, const BhState* __restrict__ st,
-    const int32_t* __restrict__ morton_order, const int32_t* __restrict__ path_order,
-    const float* __restrict__ x, const float* __restrict__ y,
-    const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
-    float eps2, float* ax, float* ay, float* az) {
-  const int lane = threadIdx.x & 63;
-  // Waves take 64 consecutive particles in the tree's own order (path order;
-  // Morton order after the sequential builder): bodies of one cell share a
-  // wave, however many share a Morton code. Workgroups are dealt to the 8 XCDs
-  // round-robin; each XCD takes one contiguous run of that order instead, so
-  // that its L2 holds the part of the tree its waves walk.
-  const int32_t* __restrict__ order = st->use_seq ? morton_order : path_order;
-  const int nb = (int)gridDim.x, per = nb / 8, rem = nb % 8;
-  const int xcd = (int)blockIdx.x % 8, slot = (int)blockIdx.x / 8;
-  const int blk = xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
-  // wave w: particles [w * 64, (w + 1) * 64) of that order
-  const int r = (blk * (kThreads / 64) + (int)(threadIdx.x >> 6)) * 64 + lane;
+// One task: the 64 particles [64 blk, 64 blk + 64) of the order.
+__device__ __forceinline__ void walk_task(int blk, int lane, const float4* __restrict__ rec,
+                                          const BhState* __restrict__ st,
+                                          const int32_t* __restrict__ order,
+                                          const float* __restrict__ x, const float* __restrict__ y,
+                                          const float* __restrict__ z, const float* __restrict__ m,
+                                          int n, float theta2, float G, float eps2, float* ax,
+                                          float* ay, float* az) {
+  const int r = blk * 64 + lane;
   const bool valid = r < n;
   const int ip = valid ? order[r] : 0;
   const float xi = x[ip], yi = y[ip], zi = z[ip], mi = m[ip];
@@ -1654,6 +1645,62 @@ __global__ __launch_bounds__(kThreads) void bh_traverse(
   }
 }
 
+// Persistent waves, one per workgroup, one grid of resident waves. Tasks are
+// 64 consecutive particles in the tree's own order (path order; Morton order
+// after the sequential builder): bodies of one cell share a wave, however many
+// share a Morton code. Workgroups are dealt to the 8 XCDs round-robin; each
+// XCD owns one contiguous eighth of the tasks, so that its L2 holds the part
+// of the tree its waves walk, and its waves take that eighth's tasks in order
+// from a counter. A walk is bound by load latency, so a slow task cannot be
+// sped up once it runs alone: taking tasks as slots free up (not two fixed
+// tasks per slot) keeps the tail to about one task. Every wave exits once its
+// XCD's counter is past the range. (Taking the other XCDs' leftovers was
+// slower: those walks miss the L2 that holds their part of the tree.)
+__global__ __launch_bounds__(64) void bh_traverse(
+    const float4* __restrict__ rec, const BhState* __restrict__ st,
+    const int32_t* __restrict__ morton_order, const int32_t* __restrict__ path_order,
+    int* queue, int ntasks, const float* __restrict__ x, const float* __restrict__ y,
+    const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
+    float eps2, float* ax, float* ay, float* az) {
+  const int lane = threadIdx.x;
+  // The walk's asm clobbers s20-s75. The per-task values below are only used as
+  // per-lane addresses, so they are held in VGPRs (an empty asm hides their
+  // uniformity): otherwise they fill the remaining SGPRs and cut the waves per
+  // SIMD from 8 to 7, which a latency-bound walk pays in full.
+  auto vreg = [](const void* p) {
+    uint64_t v = reinterpret_cast<uint64_t>(p);
+    asm volatile("" : "+v"(v));
+    return v;
+  };
+  const int32_t* order =
+      reinterpret_cast<const int32_t*>(vreg(st->use_seq ? morton_order : path_order));
+  const float* vx = reinterpret_cast<const float*>(vreg(x));
+  const float* vy = reinterpret_cast<const float*>(vreg(y));
+  const float* vz = reinterpret_cast<const float*>(vreg(z));
+  const float* vm = reinterpret_cast<const float*>(vreg(m));
+  float* vax = reinterpret_cast<float*>(vreg(ax));
+  float* vay = reinterpret_cast<float*>(vreg(ay));
+  float* vaz = reinterpret_cast<float*>(vreg(az));
+  const int xq = (int)blockIdx.x % 8;
+  const int per = ntasks / 8, rem = ntasks % 8;
+  int b = xq < rem ? xq * (per + 1) : rem * (per + 1) + (xq - rem) * per;
+  int len = per + (xq < rem ? 1 : 0);
+  asm volatile("" : "+v"(b), "+v"(len));
+  int* ctr = reinterpret_cast<int*>(vreg(queue + 16 * xq));
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(ctr, 1);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= __builtin_amdgcn_readfirstlane(len)) break;
+    walk_task(__builtin_amdgcn_readfirstlane(b) + t, lane, rec, st, order, vx, vy, vz, vm, n,
+              theta2, G, eps2, vax, vay, vaz);
+  }
+}
+
+__global__ void trav_queue_reset(int* queue) {
+  if (threadIdx.x < 8) queue[16 * threadIdx.x] = 0;
+}
+
 }  // namespace
 
 }  // namespace nbx
@@ -1685,6 +1732,8 @@ struct nbx_bh {
   // (1 / theta^2, G) the traversal records hold; set by the last trav_prep
   nbx::TravParams prep{0.0f, 0.0f, 0};
   bool nodes_pending = false;  // the reference node array is emitted on demand
+  int* trav_queue = nullptr;     // 8 task counters, 64 bytes apart
+  int trav_waves = 0;            // resident traversal waves on this device
   int force_seq = 0;
   bool timing = false;
   hipEvent_t ev[10] = {};
@@ -1796,6 +1845,16 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->final_of, NN);
   A(&t->nodes, NN);
   A(&t->trav_rec, NN * 4);
+  A(&t->trav_queue, 8 * 16);
+  if (rc == NBX_OK) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(bh_traverse), 64, 0) !=
+            hipSuccess)
+      rc = fail(NBX_ERR_HIP, "Barnes-Hut: occupancy query failed");
+    t->trav_waves = std::max(8, cus * per_cu);
+  }
   if (rc == NBX_OK) rc = t->sort_particles.init((int)P);
   if (rc == NBX_OK) rc = t->sort_nodes.init((int)NN);
   for (auto& e : t->ev)
@@ -1910,9 +1969,12 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
     NBX_LAUNCHED();
     t->prep = want;
   }
-  hipLaunchKernelGGL(bh_traverse, dim3(div_up(n, kThreads)), dim3(kThreads),
-                     0, s, t->trav_rec, t->st, t->idx, t->O.ppart, d->pos_x, d->pos_y, d->pos_z,
-                     d->mass, n, theta2, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
+  const int ntasks = (int)div_up(n, 64);
+  const int waves = std::max(8, std::min(ntasks, t->trav_waves) & ~7);
+  hipLaunchKernelGGL(trav_queue_reset, dim3(1), dim3(64), 0, s, t->trav_queue);
+  hipLaunchKernelGGL(bh_traverse, dim3(waves), dim3(64), 0, s, t->trav_rec, t->st, t->idx,
+                     t->O.ppart, t->trav_queue, ntasks, d->pos_x, d->pos_y, d->pos_z, d->mass, n,
+                     theta2, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   mark(t, 9, s);
   return NBX_OK;
