@@ -13,7 +13,7 @@ namespace nbx {
 
 // Shapes X(IPL, W) of the symmetric kernel: IPL targets per lane, W waves per
 // workgroup; super-blocks of 64 * IPL * W bodies.
-#define NBX_SYM_VARIANTS(X) X(8, 4) X(4, 4) X(4, 8) X(8, 2)
+#define NBX_SYM_VARIANTS(X) X(8, 4) X(4, 4) X(4, 8) X(8, 2) X(4, 2) X(2, 4) X(2, 2) X(4, 1)
 
 // Bytes of FP32 partial sums a symmetric evaluation of n bodies needs with
 // super-blocks of S bodies: (2 * floor(K/2) + 1) * 3 * K * S floats, K = ceil(n/S).

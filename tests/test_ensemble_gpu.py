@@ -40,10 +40,11 @@ def standalone(nbx, h, sl, variant):
     return d.acc()
 
 
-# one-sided shapes 0, 4, 6, 7 and the pair-symmetric shapes 8 (sym_ipl8_ws4) and
-# 9 (sym_ipl4_ws4): the systems of 1, 65 and up to 3000 bodies give 1 to 3
+# one-sided shapes 0, 4, 6, 7 and the pair-symmetric shapes 8 (sym_ipl8_ws4),
+# 9 (sym_ipl4_ws4), 12 (sym_ipl4_ws2, 512-body super-blocks) and 15
+# (sym_ipl4_ws1, 256): the systems of 1, 65 and up to 3000 bodies give 1 to 12
 # super-blocks, so diagonal-only, even and odd tile sets all occur
-@pytest.mark.parametrize("variant", [0, 4, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [0, 4, 6, 7, 8, 9, 12, 15])
 def test_batched_forces_bit_identical_to_standalone(nbx, small, variant):
     counts, h = small
     ens = nbx.Ensemble(counts)
@@ -57,7 +58,7 @@ def test_batched_forces_bit_identical_to_standalone(nbx, small, variant):
         assert np.array_equal(acc[ens.system(q)], ref), f"system {q} (n={counts[q]})"
 
 
-@pytest.mark.parametrize("variant", [4, 9])
+@pytest.mark.parametrize("variant", [4, 9, 15])
 def test_batched_forces_vs_fp64(nbx, orc, small, variant):
     counts, h = small
     ens = nbx.Ensemble(counts)
@@ -81,6 +82,7 @@ def test_variant_names_match_single_system(nbx):
     # a batched shape index names the same tile shape as the single-system one
     assert nbx.lib.nbx_ens_variant_count() == nbx.lib.nbx_direct_variant_count()
     assert nbx.variant_names()[9] == "sym_ipl4_ws4" and nbx.variant_names()[8] == "sym_ipl8_ws4"
+    assert nbx.variant_names()[12] == "sym_ipl4_ws2" and nbx.variant_names()[15] == "sym_ipl4_ws1"
 
 
 @pytest.mark.parametrize("variant", [2, 9])
