@@ -978,6 +978,7 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     }
     __syncthreads();
     for (int d = sh_top; d >= 1; --d) {
+      if (lvl_cnt[d] == 0) continue;  // the same for every thread: no barrier needed
       for (int q = threadIdx.x; q < lvl_cnt[d]; q += kThreads)
         local_reduce(lvl_list[lvl_start[d] + q], d, ncell, P0, s_info, s_skip, s_com, s_t1, s_t2);
       __syncthreads();
