@@ -1400,18 +1400,20 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 // rotate: the step on set X loads v + 1 into Y and skip[v] into Z, and
 // continues with Y or Z (A -> B/C, B -> C/A, C -> A/B). Pops change only the
 // mask, never v, so the next record is always one of the two. Prefetch
-// addresses are clamped to the last record, and every exit waits for the
-// loads in flight. The common step is: no lane's particle inside a leaf and
+// addresses are clamped to the pad record after the last one (v + 1 <= nn
+// inside a step), and every exit waits for the loads in flight. The common step is: no lane's particle inside a leaf and
 // every lane's theta decision clear of the band where the division decides.
 // Any other step leaves the asm and runs as C code below (the own-leaf
 // correction, the IEEE division); the walk then re-enters the asm.
 //
 // Registers: s[20:31], s[32:43], s[44:55] the record sets (dwords 0..11 of a
-// record), s[56:57] M, s[58:59] the lanes inside the cell, s[60:61] the lanes
-// that open it, s[62:63] the lanes that clearly accept it, s[64:69] further
-// lane sets, s70 = v + 1, s71 = skip clamped to at least v + 1, s72/s73 load
-// offsets, s74 = nn - 1, s75 the pop lane. Ordering: the result of v_rsq is
-// first read two instructions later; lane sets pass from vector compares to
+// record), s[56:57] M, s[58:59] the lanes inside the cell (then the undecided
+// lanes), s[60:61] the lanes that open it, s[62:63] the lanes that clearly
+// accept it, s[66:69] the lanes that take the force and those that descend,
+// s70 = v + 1, s71 = skip clamped to at least v + 1. s66/s67 hold the two load
+// offsets until the loads issue (scalar loads read their offsets at issue),
+// s58 the pop lane. s64/s65 stay free, so that the kernel needs no more than
+// 78 SGPRs. Ordering: the result of v_rsq is first read two instructions later; lane sets pass from vector compares to
 // scalar instructions (interlocked) and from scalar instructions to v_cndmask
 // (no hazard); readlane takes its lane from a scalar instruction and writelane
 // from m0, written one instruction before.
@@ -1425,14 +1427,13 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_waitcnt lgkmcnt(0)\n"                                                       \
   "s_add_i32 s70, %[v], 1\n"                                                     \
   "s_max_i32 s71, " sk ", s70\n"                                                 \
-  "s_min_i32 s72, s70, s74\n"                                                    \
-  "s_lshl_b32 s72, s72, 6\n"                                                     \
-  "s_min_i32 s73, s71, s74\n"                                                    \
-  "s_lshl_b32 s73, s73, 6\n"                                                     \
-  "s_load_dwordx8 " y8 ", %[rb], s72\n"                                          \
-  "s_load_dwordx4 " y4 ", %[rb], s72 offset:32\n"                                \
-  "s_load_dwordx8 " z8 ", %[rb], s73\n"                                          \
-  "s_load_dwordx4 " z4 ", %[rb], s73 offset:32\n"                                \
+  "s_lshl_b32 s66, s70, 6\n"                                                     \
+  "s_min_i32 s67, s71, %[nn]\n"                                                  \
+  "s_lshl_b32 s67, s67, 6\n"                                                     \
+  "s_load_dwordx8 " y8 ", %[rb], s66\n"                                          \
+  "s_load_dwordx4 " y4 ", %[rb], s66 offset:32\n"                                \
+  "s_load_dwordx8 " z8 ", %[rb], s67\n"                                          \
+  "s_load_dwordx4 " z4 ", %[rb], s67 offset:32\n"                                \
   "v_subrev_f32 %[t0], " b0 ", %[xi]\n"                                          \
   "v_subrev_f32 %[t1], " b1 ", %[yi]\n"                                          \
   "v_subrev_f32 %[t2], " b2 ", %[zi]\n"                                          \
@@ -1456,8 +1457,8 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_and_b64 s[58:59], s[58:59], s[56:57]\n"                                     \
   "s_cbranch_scc1 L_inside_" X "_%=\n"                                           \
   "L_back_" X "_%=:\n"                                                           \
-  "s_or_b64 s[64:65], s[60:61], s[62:63]\n"                                      \
-  "s_andn2_b64 s[64:65], s[56:57], s[64:65]\n"                                   \
+  "s_or_b64 s[58:59], s[60:61], s[62:63]\n"                                      \
+  "s_andn2_b64 s[58:59], s[56:57], s[58:59]\n"                                   \
   "s_cbranch_scc1 L_exit_%=\n"                                                   \
   "s_andn2_b64 s[66:67], s[56:57], s[60:61]\n"                                   \
   "s_and_b64 s[68:69], s[56:57], s[60:61]\n"                                     \
@@ -1487,10 +1488,10 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_add_i32 %[depth], %[depth], -1\n"                                           \
   "v_readlane_b32 s56, %[klo], %[depth]\n"                                       \
   "v_readlane_b32 s57, %[khi], %[depth]\n"                                       \
-  "s_add_i32 s75, %[depth], -1\n"                                                \
-  "s_cmp_lt_i32 s75, 0\n"                                                        \
+  "s_add_i32 s58, %[depth], -1\n"                                                \
+  "s_cmp_lt_i32 s58, 0\n"                                                        \
   "s_cbranch_scc1 L_root_" X "_%=\n"                                             \
-  "v_readlane_b32 %[top], %[kat], s75\n"                                         \
+  "v_readlane_b32 %[top], %[kat], s58\n"                                         \
   "s_branch L_popchk_" X "_%=\n"                                                 \
   "L_root_" X "_%=:\n"                                                           \
   "s_brev_b32 %[top], -2\n"                                                      \
@@ -1498,12 +1499,11 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 
 #define NBX_WALK_FAST                                                                       \
   "s_mov_b64 s[56:57], %[M]\n"                                                              \
-  "s_add_i32 s74, %[nn], -1\n"                                                              \
-  "s_min_i32 s72, %[v], s74\n"                                                              \
-  "s_max_i32 s72, s72, 0\n"                                                                 \
-  "s_lshl_b32 s72, s72, 6\n"                                                                \
-  "s_load_dwordx8 s[20:27], %[rb], s72\n"                                                   \
-  "s_load_dwordx4 s[28:31], %[rb], s72 offset:32\n"                                         \
+  "s_min_i32 s66, %[v], %[nn]\n"                                                            \
+  "s_max_i32 s66, s66, 0\n"                                                                 \
+  "s_lshl_b32 s66, s66, 6\n"                                                                \
+  "s_load_dwordx8 s[20:27], %[rb], s66\n"                                                   \
+  "s_load_dwordx4 s[28:31], %[rb], s66 offset:32\n"                                         \
   "s_branch L_next_A_%=\n"                                                                  \
   NBX_WALK_STEP("A", "B", "C", "s20", "s21", "s22", "s23", "s24", "s25", "s26", "s27", "s28", \
                 "s30", "s31", "s[32:39]", "s[40:43]", "s[44:51]", "s[52:55]")               \
@@ -1551,14 +1551,13 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
                    [klo] "+v"(keep_lo), [khi] "+v"(keep_hi), [t0] "=&v"(t0), [t1] "=&v"(t1),
                    [t2] "=&v"(t2), [dx] "=&v"(dxa), [dy] "=&v"(dya), [dz] "=&v"(dza),
                    [d2] "=&v"(d2a), [inv] "=&v"(inva)
-                 : [rb] "s"(rec), [nn] "s"(nn), [eps2] "s"(eps2), [xi] "v"(xi), [yi] "v"(yi),
+                 : [rb] "s"(rec), [nn] "s"(nn), [eps2] "v"(eps2), [xi] "v"(xi), [yi] "v"(yi),
                    [zi] "v"(zi)
                  : "s20", "s21", "s22", "s23", "s24", "s25", "s26", "s27", "s28", "s29", "s30",
                    "s31", "s32", "s33", "s34", "s35", "s36", "s37", "s38", "s39", "s40", "s41",
                    "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52",
                    "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63",
-                   "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74",
-                   "s75", "m0", "scc", "memory");
+                   "s66", "s67", "s68", "s69", "s70", "s71", "m0", "scc", "memory");
     v = __builtin_amdgcn_readfirstlane(v);
     top = __builtin_amdgcn_readfirstlane(top);
     depth = __builtin_amdgcn_readfirstlane(depth);
@@ -1664,10 +1663,13 @@ __global__ __launch_bounds__(64) void bh_traverse(
     const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
     float eps2, float* ax, float* ay, float* az) {
   const int lane = threadIdx.x;
-  // The walk's asm clobbers s20-s75. The per-task values below are only used as
-  // per-lane addresses, so they are held in VGPRs (an empty asm hides their
-  // uniformity): otherwise they fill the remaining SGPRs and cut the waves per
-  // SIMD from 8 to 7, which a latency-bound walk pays in full.
+  // The walk's asm clobbers s20-s71, which keeps the kernel at 78 SGPRs: on
+  // gfx950, 8 waves per SIMD stay resident up to 78 SGPRs and only 7 from 82
+  // on (measured with spinning waves), although the compiler's occupancy and
+  // the occupancy query report 8 either way. The per-task values below are only
+  // used as per-lane addresses, so they are held in VGPRs (an empty asm hides
+  // their uniformity): in SGPRs they would raise the count past that limit,
+  // and a latency-bound walk pays for the lost wave in full.
   auto vreg = [](const void* p) {
     uint64_t v = reinterpret_cast<uint64_t>(p);
     asm volatile("" : "+v"(v));
@@ -1688,6 +1690,7 @@ __global__ __launch_bounds__(64) void bh_traverse(
   int len = per + (xq < rem ? 1 : 0);
   asm volatile("" : "+v"(b), "+v"(len));
   int* ctr = reinterpret_cast<int*>(vreg(queue + 16 * xq));
+  asm volatile("" : "+v"(theta2), "+v"(G), "+v"(eps2));  // operands of vector instructions only
   for (;;) {
     int t = 0;
     if (lane == 0) t = atomicAdd(ctr, 1);
@@ -1845,7 +1848,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->nvals, NN);
   A(&t->final_of, NN);
   A(&t->nodes, NN);
-  A(&t->trav_rec, NN * 4);
+  A(&t->trav_rec, (NN + 1) * 4);  // + the pad record the walk's prefetch may read
   A(&t->trav_queue, 8 * 16);
   if (rc == NBX_OK) {
     int dev = 0, cus = 0, per_cu = 0;
