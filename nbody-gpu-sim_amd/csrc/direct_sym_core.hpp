@@ -264,11 +264,31 @@ __device__ __forceinline__ void sym_reduce_body(const SymGeom& g, int b, double 
                                                 float* ay, float* az, const KickArgs& kick) {
   const int C = b / g.S;
   double t[3] = {0.0, 0.0, 0.0};
+  // Four distances' loads are issued before their adds (12 loads in flight per
+  // lane instead of 3: the pass is bound by HBM latency otherwise); each t[k]
+  // still adds its partials in ascending d.
+  auto add4 = [&](const float* base, int d0) {
+    float v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) v[u][k] = base[((long)(d0 + u) * 3 + k) * g.npad + b];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t[k] += (double)v[u][k];
+  };
   const int DC = sym_rowlen(C, g.K);
-  for (int d = 0; d <= DC; ++d)
+  int d = 0;
+  for (; d + 3 <= DC; d += 4) add4(g.pi, d);
+  for (; d <= DC; ++d)
 #pragma unroll
     for (int k = 0; k < 3; ++k) t[k] += (double)g.pi[((long)d * 3 + k) * g.npad + b];
-  for (int d = 1; d <= g.Dmax; ++d) {
+  // j side: every row holds distances 1 .. floor(K / 2) - 1 (sym_rowlen), so
+  // only d = Dmax (of even K) can be missing.
+  d = 1;
+  for (; d + 3 <= g.Dmax - 1; d += 4) add4(g.pj, d - 1);
+  for (; d <= g.Dmax; ++d) {
     const int A = (C - d + g.K) % g.K;
     if (d > sym_rowlen(A, g.K)) continue;
 #pragma unroll
