@@ -240,7 +240,6 @@ __device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], int j,
                                            float xi, float yi, float zi, float cutoff2,
                                            float eps2, float& fx, float& fy, float& fz) {
   float rx[kNB], ry[kNB], rz[kNB], w[kNB];
-  uint64_t keep[kNB];
 #pragma unroll
   for (int q = 0; q < kNB; ++q) {
     rx[q] = sub_rn(b[q].x, xi);
@@ -250,15 +249,16 @@ __device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], int j,
   __builtin_amdgcn_sched_barrier(0);
   load_bodies<kNB>(body, nxt_j, nxt);
   __builtin_amdgcn_sched_barrier(0);
+  // A dropped pair enters v_rsq as +inf, whose result +0 makes its term an
+  // exact +0 (the same value the per-lane walk's mask gives): no lane mask
+  // has to live across the batch, which keeps the SGPRs to the bodies.
 #pragma unroll
   for (int q = 0; q < kNB; ++q) {
     const float d2 = grid_d2(rx[q], ry[q], rz[q]);
-    uint64_t k = __builtin_amdgcn_ballot_w64(d2 < cutoff2);
-    if (SELF) k &= __builtin_amdgcn_ballot_w64(j + q != self);
-    if (MEMBER)
-      k &= __builtin_amdgcn_ballot_w64(j + q >= own_lo) & __builtin_amdgcn_ballot_w64(j + q < own_hi);
-    keep[q] = k;
-    w[q] = add_rn(d2, eps2);
+    bool k = d2 < cutoff2;
+    if (SELF) k = k && j + q != self;
+    if (MEMBER) k = k && j + q >= own_lo && j + q < own_hi;
+    w[q] = k ? add_rn(d2, eps2) : __builtin_inff();
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -267,7 +267,7 @@ __device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], int j,
 #pragma unroll
   for (int q = 0; q < kNB; ++q) {
     const float inv = w[q];
-    w[q] = keep_or_zero(keep[q], mul_rn(b[q].w, mul_rn(mul_rn(inv, inv), inv)));
+    w[q] = mul_rn(b[q].w, mul_rn(mul_rn(inv, inv), inv));
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
