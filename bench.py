@@ -117,13 +117,15 @@ def run_direct(nbx, args):
     d = nbx.DeviceParticles.from_host(h)
     s = nbx.Stream()
     lib = nbx.lib
-    nbx.check(lib.nbx_direct_forces(d.ptr, G, e2, 256, s.h))  # a(0)
+    ws = nbx.DirectWorkspace(n)  # DirectForceCalculator's workspace, allocated before timing
+    plan = ws.plan(n)
+    nbx.check(lib.nbx_direct_compute(ws.h, d.ptr, G, e2, s.h))  # a(0)
 
     def step(ev_a=None, ev_b=None):
         nbx.check(lib.nbx_verlet_drift(d.ptr, dt, s.h))
         if ev_a is not None:
             ev_a.record(s)
-        nbx.check(lib.nbx_direct_forces_kick(d.ptr, G, e2, dt, s.h))
+        nbx.check(lib.nbx_direct_compute_kick(ws.h, d.ptr, G, e2, dt, s.h))
         if ev_b is not None:
             ev_b.record(s)
 
@@ -187,14 +189,15 @@ def run_direct(nbx, args):
         log(f"direct cpu baseline: {cpu['value']:.3f} Ginteractions/s on {threads} threads, "
             f"{cpu['single_thread']['value']:.3f} on 1")
     names = nbx.variant_names()
-    v = args.variant if args.variant >= 0 else int(nbx.lib.nbx_direct_variant_for(n))
+    v = plan["variant"]
     sym = names[v].startswith("sym_")
     return {
         "value": value, "ms_per_step": total_ms / K,
         "config": {"workload": "direct N^2 FP32 + Velocity Verlet, uniform sphere R=1, "
                                f"N={n}, m=1/N, sigma_v=0.05, eps=0.005, dt=5e-4",
                    "n": n, "eps": eps, "dt": dt, "G": G, "parallelism": "single GPU",
-                   "tile_variant": names[v]},
+                   "tile_variant": names[v], "rounds": plan["rounds"],
+                   "partial_scratch_bytes": plan["partial_bytes"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": traffic_from_profile(),
@@ -206,6 +209,7 @@ def run_direct(nbx, args):
                                     "one-sided: N(N-1) pair evaluations"),
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "interactions_per_launch": inter,
+                     "pair_evaluations_per_launch": inter / 2 if sym else inter,
                      "launch_ms_mean": float(force_ms.mean())},
         "force_error_fp64": err,
         "cpu_baseline": cpu,
@@ -565,6 +569,7 @@ def run_precision(nbx, args):
     h = nbx.gen_plummer(n, seed=42)
     plain = nbx.DeviceParticles.from_host(h)
     ds = nbx.DSDeviceParticles.from_host(h)
+    ws = nbx.DirectWorkspace(n)
     s = nbx.Stream()
     C = nbx.C
 
@@ -589,16 +594,16 @@ def run_precision(nbx, args):
     res = {}
     for name in ("plain", "double_single"):
         if name == "plain":
-            nbx.check(nbx.lib.nbx_direct_forces(plain.ptr, G, e2, 256, s.h))
+            nbx.check(nbx.lib.nbx_direct_compute(ws.h, plain.ptr, G, e2, s.h))
             energy = plain_energy
-            def step(): nbx.check(nbx.lib.nbx_direct_step(plain.ptr, G, e2, dt, s.h))
+            def step(): nbx.check(nbx.lib.nbx_direct_compute_step(ws.h, plain.ptr, G, e2, dt, s.h))
             def state():
                 a = plain.acc().astype(np.float64)
                 return a, [plain.read(k).astype(np.float64) for k in ("pos_x", "pos_y", "pos_z")]
         else:
-            nbx.check(nbx.lib.nbx_ds_forces(ds.ptr, G, e2, s.h))
+            nbx.check(nbx.lib.nbx_ds_forces(ws.h, ds.ptr, G, e2, s.h))
             energy = ds_energy
-            def step(): nbx.check(nbx.lib.nbx_ds_step(ds.ptr, G, e2, dt, s.h))
+            def step(): nbx.check(nbx.lib.nbx_ds_step(ws.h, ds.ptr, G, e2, dt, s.h))
             def state(): return ds.acc().astype(np.float64), ds.positions64()
         s.synchronize()
         E0, P0 = energy()
@@ -739,6 +744,7 @@ def main() -> int:
         line["barnes_hut"] = run_bh(nbx, args)
         line["ensemble"] = run_ensemble(nbx, args)
         line["spatial_hash"] = run_spatial_hash(nbx, args)
+        line["precision"] = run_precision(nbx, args)
         if not args.no_cpu_baseline:
             line["cpu_path_n4096"] = run_cpu_path(nbx, args)
     print(json.dumps(line), flush=True)

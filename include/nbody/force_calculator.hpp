@@ -8,6 +8,8 @@
 
 #include "nbody/types.hpp"
 
+struct nbx_direct;  // workspace handle of the C ABI (nbx.h)
+
 namespace nbody {
 
 class ForceCalculator {
@@ -30,16 +32,27 @@ class ForceCalculator {
   float G_ = 1.0f;
 };
 
+// Owns the device workspace of the pair-symmetric kernel. It is allocated by
+// reserve(), or by the first computeForces call (and again by a call on more
+// bodies than it holds); other calls allocate nothing.
 class DirectForceCalculator : public ForceCalculator {
  public:
   explicit DirectForceCalculator(int block_size = 256);
+  ~DirectForceCalculator() override;
+  DirectForceCalculator(const DirectForceCalculator&) = delete;
+  DirectForceCalculator& operator=(const DirectForceCalculator&) = delete;
   void computeForces(ParticleData* d_particles) override;
+  // computeForces with the Verlet kick v += (a_old + a) dt/2 fused (Integrator::integrate)
+  void computeForcesKick(ParticleData* d_particles, float dt);
+  void reserve(size_t max_particles);
   ForceMethod getMethod() const noexcept override { return ForceMethod::DIRECT_N2; }
   void setBlockSize(int size) { block_size_ = size; }
   int getBlockSize() const noexcept { return block_size_; }
 
  private:
   int block_size_;
+  nbx_direct* workspace_ = nullptr;
+  size_t capacity_ = 0;
 };
 
 class BarnesHutCalculator : public ForceCalculator {

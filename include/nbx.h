@@ -86,8 +86,10 @@ int nbx_particles_download(nbx_particles* host, const nbx_particles* dev);
 /* ------------------------------------------------------------ direct N^2 */
 /* launchDirectForceKernel(ParticleData*, float G, float eps2, int block_size)
  * (src/cuda/force_direct.cu:88-98): acc_i = G sum_{j!=i} m_j r_ij (|r_ij|^2+eps2)^{-3/2}.
- * block_size is validated (1..1024) as the reference launch would; the tile
- * shape is chosen by the autotuner (nbx_direct_set_variant overrides it). */
+ * block_size is validated (1..1024) as the reference launch would. Like the
+ * reference launcher it takes no workspace and allocates nothing: it runs the
+ * one-sided tiled kernel (the tile shape from the autotuner; a forced
+ * pair-symmetric shape falls back to the one-sided default). */
 int nbx_direct_forces(nbx_particles* d, float G, float eps2, int block_size, nbx_stream s);
 /* Forces for a contiguous block of targets [i0, i0+ni) against all j (sample
  * checks and the ensemble's standalone comparison). acc written at i0..i0+ni-1. */
@@ -96,11 +98,41 @@ int nbx_direct_forces_range(nbx_particles* d, size_t i0, size_t ni, float G, flo
 /* One whole Velocity Verlet step with direct forces (Integrator::integrate,
  * src/cuda/integrator.cu:175-189): store+drift fused, then the force kernel
  * with the kick fused into its epilogue. Same arithmetic as the four separate
- * reference kernels. */
+ * reference kernels. One-sided kernel, no workspace. */
 int nbx_direct_step(nbx_particles* d, float G, float eps2, float dt, nbx_stream s);
 /* Second half of nbx_direct_step: the force kernel with the kick
  * v += (a_old + a_new)*(0.5f*dt) in its epilogue (after nbx_verlet_drift). */
 int nbx_direct_forces_kick(nbx_particles* d, float G, float eps2, float dt, nbx_stream s);
+
+/* DirectForceCalculator (include/nbody/force_calculator.hpp:29-39,
+ * src/cuda/force_direct.cu:101-106) with the workspace of the pair-symmetric
+ * kernel, which evaluates each unordered pair once and applies it to both bodies.
+ * The workspace is allocated once by nbx_direct_create for up to max_particles
+ * bodies and released by nbx_direct_destroy; the compute calls allocate nothing.
+ * scratch_bytes bounds the FP32 partial sums (0: the one-round need, at most
+ * 1 GiB); a smaller workspace runs the tiles in rounds of tile distances and
+ * gives the same accelerations bit for bit. Below the one-sided cut-over (or
+ * when a one-sided shape is forced or tuned) the one-sided kernel runs.
+ * Calls that share a workspace must be ordered (one stream, or synchronised):
+ * overlapping calls race on its scratch. NBX_ERR_CAPACITY for more bodies than
+ * the workspace was created for; NBX_ERR_STATE on another device. */
+typedef struct nbx_direct nbx_direct;
+int nbx_direct_create(size_t max_particles, size_t scratch_bytes, nbx_direct** out);
+int nbx_direct_destroy(nbx_direct* op);
+/* DirectForceCalculator::computeForces(ParticleData*) (force_direct.cu:101-106). */
+int nbx_direct_compute(nbx_direct* op, nbx_particles* d, float G, float eps2, nbx_stream s);
+/* The forces with the kick v += (a_old + a_new)*(0.5f*dt) fused (after nbx_verlet_drift). */
+int nbx_direct_compute_kick(nbx_direct* op, nbx_particles* d, float G, float eps2, float dt,
+                            nbx_stream s);
+/* Integrator::integrate (src/cuda/integrator.cu:175-189) with these forces:
+ * store+drift, forces, kick. */
+int nbx_direct_compute_step(nbx_direct* op, nbx_particles* d, float G, float eps2, float dt,
+                            nbx_stream s);
+/* What a compute call on n bodies runs now: the tile variant, the number of
+ * rounds and the bytes of partial sums each round uses (0 for a one-sided
+ * variant). No reference counterpart (diagnostics). */
+int nbx_direct_plan(const nbx_direct* op, size_t n, int* variant, int* rounds,
+                    size_t* partial_bytes);
 
 /* Tile variants of the direct kernel, named iplI_nbB_wsW: I targets per lane,
  * stage groups of B bodies, W waves splitting the j range of one tile. */
@@ -212,14 +244,17 @@ typedef struct nbx_ds_particles {
 } nbx_ds_particles;
 int nbx_ds_alloc(nbx_ds_particles* p, size_t count); /* lo parts zeroed */
 int nbx_ds_free(nbx_ds_particles* p);
-int nbx_ds_forces(nbx_ds_particles* d, float G, float eps2, nbx_stream s);
-/* Kernel of nbx_ds_forces / nbx_ds_step: -1 pair-symmetric from 2^15 bodies (the
- * default), 0 one-sided, 1 pair-symmetric. */
+/* Forces; op is a direct workspace (nbx_direct_create) for the pair-symmetric
+ * kernel, or NULL for the one-sided kernel (no scratch). */
+int nbx_ds_forces(nbx_direct* op, nbx_ds_particles* d, float G, float eps2, nbx_stream s);
+/* Kernel of nbx_ds_forces / nbx_ds_step: -1 pair-symmetric from 2^15 bodies when
+ * a workspace is given (the default), 0 one-sided, 1 pair-symmetric (needs one). */
 int nbx_ds_set_kernel(int kind);
 /* store+drift in double-single: (hi,lo) += v*dt + a*(0.5*dt*dt) with renormalisation. */
 int nbx_ds_drift(nbx_ds_particles* d, float dt, nbx_stream s);
 /* drift, forces, kick. */
-int nbx_ds_step(nbx_ds_particles* d, float G, float eps2, float dt, nbx_stream s);
+int nbx_ds_step(nbx_direct* op, nbx_ds_particles* d, float G, float eps2, float dt,
+                nbx_stream s);
 int nbx_ds_energy(const nbx_ds_particles* d, float G, float eps2, double* ke, double* pe,
                   double momentum[3], nbx_stream s);
 
@@ -239,7 +274,8 @@ int nbx_ens_forces(nbx_ensemble* e, float G, float eps2, nbx_stream s);
 int nbx_ens_steps(nbx_ensemble* e, float G, float eps2, float dt, int steps, int graph_chunk,
                   nbx_stream s);
 /* Baseline for the batched form: the same `steps`, advancing each system on
- * its own with the single-system entry points (nbx_direct_step per system). */
+ * its own with the single-system entry point (nbx_direct_compute_step per
+ * system, with a workspace the ensemble holds for its largest system). */
 int nbx_ens_steps_sequential(nbx_ensemble* e, float G, float eps2, float dt, int steps,
                              nbx_stream s);
 /* Tile shapes: the same (targets per lane, waves per tile) list and order as the

@@ -1,8 +1,10 @@
 // Direct N^2 force kernels, launch dispatch over tile variants, and the
 // in-project autotuner. Replaces launchDirectForceKernel / DirectForceCalculator
 // (src/cuda/force_direct.cu:88-106).
+#include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "common.hpp"
@@ -31,14 +33,14 @@ struct DirectVariant {
   // Super-block size of the symmetric kernel, 0 for one-sided shapes. Symmetric
   // shapes only evaluate whole systems ([i0, iend) = [0, n)).
   int sym_block;
-  // launches over targets [i0, iend) of the system d
+  // launches over targets [i0, iend) of the system d; symmetric shapes use w
   int (*launch)(const nbx_particles* d, int i0, int iend, double G, float eps2, bool checked,
-                const KickArgs* kick, hipStream_t s);
+                const KickArgs* kick, const SymWork* w, hipStream_t s);
 };
 
 template <int IPL, int NB, int WS>
 static int launch_variant(const nbx_particles* d, int i0, int iend, double G, float eps2,
-                           bool checked, const KickArgs* kick, hipStream_t s) {
+                           bool checked, const KickArgs* kick, const SymWork*, hipStream_t s) {
   const int n = (int)d->count;
   const int tiles = (int)div_up((size_t)(iend - i0), 64 * IPL);
   dim3 grid(tiles), block(64 * WS);
@@ -69,8 +71,8 @@ static int launch_variant(const nbx_particles* d, int i0, int iend, double G, fl
 
 template <int IPL, int W>
 static int launch_sym(const nbx_particles* d, int, int, double G, float eps2, bool checked,
-                      const KickArgs* kick, hipStream_t s) {
-  return direct_sym_launch<IPL, W>(d, G, eps2, checked, kick, s);
+                      const KickArgs* kick, const SymWork* w, hipStream_t s) {
+  return direct_sym_launch<IPL, W>(d, G, eps2, checked, kick, *w, s);
 }
 
 // One-sided shapes first, in the order the batched kernel shares, then the
@@ -97,7 +99,7 @@ static int size_class(size_t n) {
 
 // Defaults when nothing was tuned, from sweeps on MI355X (uniform-sphere
 // bodies, equal masses): the one-sided kernel for small systems and for calls
-// the symmetric kernel cannot take, the symmetric one from 2^16 bodies.
+// that bring no workspace, the symmetric one from 2^16 bodies.
 #define NBX_COUNT_V(I, B, W) +1
 constexpr int kNumOneSided = 0 NBX_DIRECT_VARIANTS(NBX_COUNT_V);
 #undef NBX_COUNT_V
@@ -126,17 +128,76 @@ int direct_pick_variant(size_t n) {
 }
 
 int direct_launch(const nbx_particles* d, size_t i0, size_t ni, float G, float eps2,
-                  const KickArgs* kick, hipStream_t s, int variant) {
+                  const KickArgs* kick, hipStream_t s, int variant, const SymWork* w) {
   if (ni == 0) return NBX_OK;
   if (!(eps2 >= 0.0f) || !std::isfinite(eps2))
     return fail(NBX_ERR_INVALID, "softening eps2 must be finite and >= 0 (got %g)", (double)eps2);
   int v = variant >= 0 ? variant : direct_pick_variant(d->count);
-  if (kVariants[v].sym_block &&
-      (i0 != 0 || ni != d->count ||
-       direct_sym_partial_bytes(d->count, kVariants[v].sym_block) > kSymMaxPartialBytes))
-    v = one_sided_default(d->count);
+  if (kVariants[v].sym_block && (!w || i0 != 0 || ni != d->count)) v = one_sided_default(d->count);
   const bool checked = eps2 < kUncheckedMinEps2;
-  return kVariants[v].launch(d, (int)i0, (int)(i0 + ni), (double)G, eps2, checked, kick, s);
+  return kVariants[v].launch(d, (int)i0, (int)(i0 + ni), (double)G, eps2, checked, kick, w, s);
+}
+
+int direct_sym_block_of(int variant) { return kVariants[variant].sym_block; }
+
+// Workspace sizes for up to max_particles bodies: the FP64 carry covers every
+// super-block size (all divide kSymMaxBlock); the partials default to the
+// largest one-round need over the symmetric shapes, capped at kDefaultPartialCap,
+// and are at least the three slots of a one-distance round.
+constexpr size_t kDefaultPartialCap = size_t(1) << 30;
+
+static size_t padded(size_t n) { return (n + kSymMaxBlock - 1) / kSymMaxBlock * kSymMaxBlock; }
+static size_t min_partial_bytes(size_t n) { return 3 * 3 * padded(n) * sizeof(float); }
+
+int direct_ws_create(size_t max_particles, size_t scratch_bytes, nbx_direct** out) {
+  if (!out) return fail(NBX_ERR_INVALID, "nbx_direct_create: null output");
+  *out = nullptr;
+  if (max_particles == 0 || max_particles > (size_t)INT32_MAX)
+    return fail(NBX_ERR_INVALID, "nbx_direct_create: max_particles %zu outside [1, 2^31-1]",
+                max_particles);
+  size_t one_round = 0;
+  for (const DirectVariant& V : kVariants)
+    if (V.sym_block)
+      one_round = std::max(one_round, direct_sym_partial_bytes(max_particles, V.sym_block));
+  const size_t part = std::min(one_round, scratch_bytes ? scratch_bytes : kDefaultPartialCap);
+  if (part < one_round && part < min_partial_bytes(max_particles))
+    return fail(NBX_ERR_INVALID,
+                "nbx_direct_create: scratch of %zu bytes is below the %zu that %zu bodies need",
+                scratch_bytes, min_partial_bytes(max_particles), max_particles);
+  nbx_direct* op = new (std::nothrow) nbx_direct();
+  if (!op) return fail(NBX_ERR_NOMEM, "nbx_direct_create: host allocation");
+  op->max_particles = max_particles;
+  if (hipGetDevice(&op->device) != hipSuccess) {
+    delete op;
+    return fail(NBX_ERR_HIP, "nbx_direct_create: hipGetDevice failed");
+  }
+  const size_t carry = part < one_round ? 3 * padded(max_particles) * sizeof(double) : 0;
+  if (hipMalloc(reinterpret_cast<void**>(&op->work.part), part) != hipSuccess ||
+      (carry && hipMalloc(reinterpret_cast<void**>(&op->work.carry), carry) != hipSuccess)) {
+    (void)hipGetLastError();
+    if (op->work.part) (void)hipFree(op->work.part);
+    delete op;
+    return fail(NBX_ERR_NOMEM, "nbx_direct_create: %zu + %zu bytes of device scratch", part,
+                carry);
+  }
+  op->work.part_bytes = part;
+  op->work.carry_bytes = carry;
+  *out = op;
+  return NBX_OK;
+}
+
+static int check_ws(const nbx_direct* op, const nbx_particles* d, const char* who) {
+  if (!op) return fail(NBX_ERR_INVALID, "%s: null workspace", who);
+  NBX_TRY(check_particles(d, who));
+  if (d->count > op->max_particles)
+    return fail(NBX_ERR_CAPACITY, "%s: %zu bodies exceed the workspace's %zu", who, d->count,
+                op->max_particles);
+  int dev = 0;
+  NBX_HIP(hipGetDevice(&dev));
+  if (dev != op->device)
+    return fail(NBX_ERR_STATE, "%s: workspace of device %d used on device %d", who, op->device,
+                dev);
+  return NBX_OK;
 }
 
 }  // namespace nbx
@@ -149,7 +210,7 @@ int nbx_direct_forces(nbx_particles* d, float G, float eps2, int block_size, nbx
   NBX_TRY(check_particles(d, "nbx_direct_forces"));
   if (block_size < 1 || block_size > 1024)
     return fail(NBX_ERR_INVALID, "block size must be between 1 and 1024 (got %d)", block_size);
-  return direct_launch(d, 0, d->count, G, eps2, nullptr, as_stream(s), -1);
+  return direct_launch(d, 0, d->count, G, eps2, nullptr, as_stream(s), -1, nullptr);
 }
 
 int nbx_direct_forces_range(nbx_particles* d, size_t i0, size_t ni, float G, float eps2,
@@ -158,7 +219,7 @@ int nbx_direct_forces_range(nbx_particles* d, size_t i0, size_t ni, float G, flo
   if (i0 > d->count || ni > d->count - i0)
     return fail(NBX_ERR_INVALID, "target range [%zu, %zu) outside [0, %zu)", i0, i0 + ni,
                 d->count);
-  return direct_launch(d, i0, ni, G, eps2, nullptr, as_stream(s), -1);
+  return direct_launch(d, i0, ni, G, eps2, nullptr, as_stream(s), -1, nullptr);
 }
 
 int nbx_direct_step(nbx_particles* d, float G, float eps2, float dt, nbx_stream s) {
@@ -166,13 +227,71 @@ int nbx_direct_step(nbx_particles* d, float G, float eps2, float dt, nbx_stream 
   hipStream_t st = as_stream(s);
   NBX_TRY(verlet_drift_launch(d, dt, true, st));
   KickArgs k{d->vel_x, d->vel_y, d->vel_z, d->acc_old_x, d->acc_old_y, d->acc_old_z, 0.5f * dt};
-  return direct_launch(d, 0, d->count, G, eps2, &k, st, -1);
+  return direct_launch(d, 0, d->count, G, eps2, &k, st, -1, nullptr);
 }
 
 int nbx_direct_forces_kick(nbx_particles* d, float G, float eps2, float dt, nbx_stream s) {
   NBX_TRY(check_particles(d, "nbx_direct_forces_kick"));
   KickArgs k{d->vel_x, d->vel_y, d->vel_z, d->acc_old_x, d->acc_old_y, d->acc_old_z, 0.5f * dt};
-  return direct_launch(d, 0, d->count, G, eps2, &k, as_stream(s), -1);
+  return direct_launch(d, 0, d->count, G, eps2, &k, as_stream(s), -1, nullptr);
+}
+
+int nbx_direct_create(size_t max_particles, size_t scratch_bytes, nbx_direct** out) {
+  return direct_ws_create(max_particles, scratch_bytes, out);
+}
+
+int nbx_direct_destroy(nbx_direct* op) {
+  if (!op) return NBX_OK;
+  int rc = NBX_OK;
+  if (op->work.part && hipFree(op->work.part) != hipSuccess)
+    rc = fail(NBX_ERR_HIP, "nbx_direct_destroy: hipFree failed");
+  if (op->work.carry && hipFree(op->work.carry) != hipSuccess)
+    rc = fail(NBX_ERR_HIP, "nbx_direct_destroy: hipFree failed");
+  delete op;
+  return rc;
+}
+
+int nbx_direct_compute(nbx_direct* op, nbx_particles* d, float G, float eps2, nbx_stream s) {
+  NBX_TRY(check_ws(op, d, "nbx_direct_compute"));
+  return direct_launch(d, 0, d->count, G, eps2, nullptr, as_stream(s), -1, &op->work);
+}
+
+int nbx_direct_compute_kick(nbx_direct* op, nbx_particles* d, float G, float eps2, float dt,
+                            nbx_stream s) {
+  NBX_TRY(check_ws(op, d, "nbx_direct_compute_kick"));
+  KickArgs k{d->vel_x, d->vel_y, d->vel_z, d->acc_old_x, d->acc_old_y, d->acc_old_z, 0.5f * dt};
+  return direct_launch(d, 0, d->count, G, eps2, &k, as_stream(s), -1, &op->work);
+}
+
+int nbx_direct_compute_step(nbx_direct* op, nbx_particles* d, float G, float eps2, float dt,
+                            nbx_stream s) {
+  NBX_TRY(check_ws(op, d, "nbx_direct_compute_step"));
+  hipStream_t st = as_stream(s);
+  NBX_TRY(verlet_drift_launch(d, dt, true, st));
+  KickArgs k{d->vel_x, d->vel_y, d->vel_z, d->acc_old_x, d->acc_old_y, d->acc_old_z, 0.5f * dt};
+  return direct_launch(d, 0, d->count, G, eps2, &k, st, -1, &op->work);
+}
+
+int nbx_direct_plan(const nbx_direct* op, size_t n, int* variant, int* rounds,
+                    size_t* partial_bytes) {
+  if (!op) return fail(NBX_ERR_INVALID, "nbx_direct_plan: null workspace");
+  if (n == 0 || n > op->max_particles)
+    return fail(NBX_ERR_CAPACITY, "nbx_direct_plan: %zu bodies outside [1, %zu]", n,
+                op->max_particles);
+  const int v = direct_pick_variant(n);
+  const int S = kVariants[v].sym_block;
+  int r = 1;
+  size_t bytes = 0;
+  if (S) {
+    const int R = direct_sym_round_len(n, S, op->work.part_bytes);
+    if (R < 0) return fail(NBX_ERR_CAPACITY, "nbx_direct_plan: scratch too small for %zu", n);
+    r = direct_sym_rounds(n, S, R);
+    bytes = std::min(op->work.part_bytes, direct_sym_partial_bytes(n, S));
+  }
+  if (variant) *variant = v;
+  if (rounds) *rounds = r;
+  if (partial_bytes) *partial_bytes = bytes;
+  return NBX_OK;
 }
 
 int nbx_direct_variant_count(void) { return kNumVariants; }
@@ -218,14 +337,21 @@ int nbx_direct_autotune(size_t n, int reps, int* best_variant, double* best_ms) 
     nbx_particles_free(&d);
     return fail(NBX_ERR_HIP, "hipEventCreate failed");
   }
+  nbx_direct* ws = nullptr;
+  if ((rc = direct_ws_create(n, 0, &ws))) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    nbx_particles_free(&d);
+    return rc;
+  }
   int best = -1;
   float best_t = 1e30f;
   for (int v = 0; v < kNumVariants && rc == NBX_OK; ++v) {
-    rc = direct_launch(&d, 0, n, 1.0f, 1e-4f, nullptr, nullptr, v);  // warm-up
+    rc = direct_launch(&d, 0, n, 1.0f, 1e-4f, nullptr, nullptr, v, &ws->work);  // warm-up
     if (rc) break;
     (void)hipEventRecord(e0, nullptr);
     for (int r = 0; r < reps && rc == NBX_OK; ++r)
-      rc = direct_launch(&d, 0, n, 1.0f, 1e-4f, nullptr, nullptr, v);
+      rc = direct_launch(&d, 0, n, 1.0f, 1e-4f, nullptr, nullptr, v, &ws->work);
     (void)hipEventRecord(e1, nullptr);
     if (hipEventSynchronize(e1) != hipSuccess) {
       rc = fail(NBX_ERR_HIP, "autotune synchronisation failed");
@@ -241,6 +367,7 @@ int nbx_direct_autotune(size_t n, int reps, int* best_variant, double* best_ms) 
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  (void)nbx_direct_destroy(ws);
   nbx_particles_free(&d);
   if (rc) return rc;
   {

@@ -21,14 +21,20 @@ __device__ __forceinline__ float ror1(float v, int src_addr) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(src_addr, __float_as_int(v)));
 }
 
+// One round of a system's tiles: distances d_lo .. d_hi (d_lo = 0 includes the
+// diagonal). A whole evaluation is one round (d_lo = 0, d_hi = Dmax) when its
+// partial sums fit the scratch, or several consecutive rounds otherwise.
 struct SymGeom {
   int n;        // bodies
   int S;        // bodies per super-block
   int K;        // super-blocks
   int Dmax;     // floor(K / 2)
+  int d_lo;     // first distance of the round
+  int d_hi;     // last distance of the round
   long npad;    // K * S
-  float* pi;    // i-side partials [Dmax + 1][3][npad]
-  float* pj;    // j-side partials [Dmax][3][npad]
+  float* pi;    // i-side partials [d_hi - d_lo + 1][3][npad], slot d - d_lo
+  float* pj;    // j-side partials [d_hi - pj_lo() + 1][3][npad], slot d - pj_lo()
+  __host__ __device__ int pj_lo() const { return d_lo > 0 ? d_lo : 1; }
 };
 
 // Off-diagonal tiles of row A: distances 1 .. sym_rowlen(A, K).
@@ -182,7 +188,7 @@ __device__ __forceinline__ void sym_tile(const float* __restrict__ x,
 #pragma unroll
         for (int w = 0; w < W; ++w) t += (double)buf[w][k][l];
         if (UNI) t *= (double)mref;
-        g.pj[((long)(d - 1) * 3 + k) * g.npad + jbase + c * 64 + l] = (float)t;
+        g.pj[((long)(d - g.pj_lo()) * 3 + k) * g.npad + jbase + c * 64 + l] = (float)t;
       }
       // red[c & 1] is rewritten at chunk c + 2, after every thread has
       // passed the barrier of chunk c + 1, so one barrier per chunk suffices.
@@ -192,9 +198,9 @@ __device__ __forceinline__ void sym_tile(const float* __restrict__ x,
 #pragma unroll
   for (int q = 0; q < IPL; ++q) {
     const long i = ibase + q * 64 + lane;
-    g.pi[((long)d * 3 + 0) * g.npad + i] = (float)sx[q];
-    g.pi[((long)d * 3 + 1) * g.npad + i] = (float)sy[q];
-    g.pi[((long)d * 3 + 2) * g.npad + i] = (float)sz[q];
+    g.pi[((long)(d - g.d_lo) * 3 + 0) * g.npad + i] = (float)sx[q];
+    g.pi[((long)(d - g.d_lo) * 3 + 1) * g.npad + i] = (float)sy[q];
+    g.pi[((long)(d - g.d_lo) * 3 + 2) * g.npad + i] = (float)sz[q];
   }
 }
 
@@ -213,16 +219,21 @@ __device__ __forceinline__ void sym_dispatch(const float* __restrict__ x,
     sym_tile<IPL, W, UNI, true, false>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
 }
 
-// Tile t of a system: diagonal tiles first (t < K), then row-major over
-// (A, d). Returns false for a tile owned by the other row (even K, d = K/2).
+// Tile t of a round: its diagonal tiles first (t < K, round starting at d = 0),
+// then row-major over (A, d), d = pj_lo() .. d_hi. Returns false for a tile
+// owned by the other row (even K, d = K/2).
 __device__ __forceinline__ bool sym_tile_of(const SymGeom& g, int t, int& A, int& d) {
-  if (t < g.K) {
-    A = t;
-    d = 0;
-    return true;
+  if (g.d_lo == 0) {
+    if (t < g.K) {
+      A = t;
+      d = 0;
+      return true;
+    }
+    t -= g.K;
   }
-  A = (t - g.K) / g.Dmax;
-  d = (t - g.K) % g.Dmax + 1;
+  const int nd = g.d_hi - g.pj_lo() + 1;
+  A = t / nd;
+  d = t % nd + g.pj_lo();
   return d <= sym_rowlen(A, g.K);
 }
 
@@ -256,43 +267,68 @@ __device__ __forceinline__ void sym_run_tile(const float* __restrict__ x,
     sym_dispatch<IPL, W, false, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
 }
 
-// a_b = G * (sum of body b's i-side partials, d ascending, then its j-side
-// partials, d ascending), in FP64; optional kick as in direct_tile(). b is
-// system-local; a* and kick point at the system's first body.
+// Body b's partials of one round added to t in the canonical order: for d
+// ascending, the i-side partial of distance d, then the j-side one. The order
+// does not depend on how the distances are split into rounds, and t is carried
+// between rounds in FP64 (exact), so every round size gives the same bits.
+// b is system-local.
+__device__ __forceinline__ void sym_round_add(const SymGeom& g, int b, double (&t)[3]) {
+  const int C = b / g.S;
+  int d = g.d_lo;
+  if (d == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[k] += (double)g.pi[((long)k) * g.npad + b];
+    d = 1;
+  }
+  // Both sides exist for every row at d <= Dmax - 1 (sym_rowlen >= floor(K/2) - 1).
+  // Four distances' loads (24 per lane) are issued before their adds: the pass
+  // is bound by HBM latency otherwise.
+  const int dsafe = min(g.d_hi, g.Dmax - 1);
+  for (; d + 3 <= dsafe; d += 4) {
+    float vi[4][3], vj[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        vi[u][k] = g.pi[((long)(d + u - g.d_lo) * 3 + k) * g.npad + b];
+        vj[u][k] = g.pj[((long)(d + u - g.pj_lo()) * 3 + k) * g.npad + b];
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        t[k] += (double)vi[u][k];
+        t[k] += (double)vj[u][k];
+      }
+  }
+  for (; d <= g.d_hi; ++d) {
+    const bool own = d <= sym_rowlen(C, g.K);
+    const bool other = d <= sym_rowlen((C - d + g.K) % g.K, g.K);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (own) t[k] += (double)g.pi[((long)(d - g.d_lo) * 3 + k) * g.npad + b];
+      if (other) t[k] += (double)g.pj[((long)(d - g.pj_lo()) * 3 + k) * g.npad + b];
+    }
+  }
+}
+
+// One round's reduction for body b: t starts at zero (first round) or from the
+// FP64 carry acc64[3][npad]; after the last round a_b = G * t is written with
+// the optional kick (as in direct_tile()), otherwise t goes back to the carry.
+// b is system-local; a*, kick and acc64 point at the system's first body.
 template <bool KICK>
 __device__ __forceinline__ void sym_reduce_body(const SymGeom& g, int b, double G, float* ax,
-                                                float* ay, float* az, const KickArgs& kick) {
-  const int C = b / g.S;
+                                                float* ay, float* az, const KickArgs& kick,
+                                                double* acc64, bool last) {
   double t[3] = {0.0, 0.0, 0.0};
-  // Four distances' loads are issued before their adds (12 loads in flight per
-  // lane instead of 3: the pass is bound by HBM latency otherwise); each t[k]
-  // still adds its partials in ascending d.
-  auto add4 = [&](const float* base, int d0) {
-    float v[4][3];
+  if (g.d_lo > 0)
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int k = 0; k < 3; ++k) t[k] = acc64[(long)k * g.npad + b];
+  sym_round_add(g, b, t);
+  if (!last) {
 #pragma unroll
-      for (int k = 0; k < 3; ++k) v[u][k] = base[((long)(d0 + u) * 3 + k) * g.npad + b];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) t[k] += (double)v[u][k];
-  };
-  const int DC = sym_rowlen(C, g.K);
-  int d = 0;
-  for (; d + 3 <= DC; d += 4) add4(g.pi, d);
-  for (; d <= DC; ++d)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) t[k] += (double)g.pi[((long)d * 3 + k) * g.npad + b];
-  // j side: every row holds distances 1 .. floor(K / 2) - 1 (sym_rowlen), so
-  // only d = Dmax (of even K) can be missing.
-  d = 1;
-  for (; d + 3 <= g.Dmax - 1; d += 4) add4(g.pj, d - 1);
-  for (; d <= g.Dmax; ++d) {
-    const int A = (C - d + g.K) % g.K;
-    if (d > sym_rowlen(A, g.K)) continue;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) t[k] += (double)g.pj[((long)(d - 1) * 3 + k) * g.npad + b];
+    for (int k = 0; k < 3; ++k) acc64[(long)k * g.npad + b] = t[k];
+    return;
   }
   const float a0 = (float)(G * t[0]), a1 = (float)(G * t[1]), a2 = (float)(G * t[2]);
   ax[b] = a0;
@@ -305,26 +341,41 @@ __device__ __forceinline__ void sym_reduce_body(const SymGeom& g, int b, double 
   }
 }
 
-// Geometry of a system of n bodies in super-blocks of S; partial sums at pi.
+// Geometry of a whole system of n bodies in super-blocks of S as one round;
+// partial sums at pi.
 __host__ __device__ inline SymGeom sym_geom(int n, int S, float* pi) {
   SymGeom g{};
   g.n = n;
   g.S = S;
   g.K = (n + S - 1) / S;
   g.Dmax = g.K / 2;
+  g.d_lo = 0;
+  g.d_hi = g.Dmax;
   g.npad = (long)g.K * S;
   g.pi = pi;
   g.pj = pi + (size_t)(g.Dmax + 1) * 3 * g.npad;
   return g;
 }
 
-// Floats of partial sums for n bodies in super-blocks of S.
+// The round of distances d_lo .. d_hi of g's system, partial sums at pi.
+__host__ __device__ inline SymGeom sym_round(SymGeom g, int d_lo, int d_hi, float* pi) {
+  g.d_lo = d_lo;
+  g.d_hi = d_hi;
+  g.pi = pi;
+  g.pj = pi + (size_t)(d_hi - d_lo + 1) * 3 * g.npad;
+  return g;
+}
+
+// Floats of partial sums for n bodies in super-blocks of S, as one round.
 __host__ __device__ inline size_t sym_partial_floats(size_t n, int S) {
   const size_t K = (n + S - 1) / S;
   return (2 * (K / 2) + 1) * 3 * K * (size_t)S;
 }
 
-// Tiles of a system (including the skipped ones of even K).
-__host__ __device__ inline int sym_tiles(const SymGeom& g) { return g.K + g.K * g.Dmax; }
+// Tiles of a round (including the skipped ones of even K).
+__host__ __device__ inline int sym_tiles(const SymGeom& g) {
+  const int off = g.d_hi >= g.pj_lo() ? g.K * (g.d_hi - g.pj_lo() + 1) : 0;
+  return (g.d_lo == 0 ? g.K : 0) + off;
+}
 
 }  // namespace nbx

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void ens_sym_reduce(
     k.aoy += off;
     k.aoz += off;
   }
-  sym_reduce_body<KICK>(g, b, G, ax + off, ay + off, az + off, k);
+  sym_reduce_body<KICK>(g, b, G, ax + off, ay + off, az + off, k, nullptr, true);
 }
 
 // Per-system FP64 kinetic energy and softened potential, one workgroup per
@@ -224,6 +224,8 @@ struct nbx_ensemble {
   float* d_part = nullptr;                          // symmetric partial sums (largest shape)
   int grid[nbx::kEnsNumVariants][2][2] = {};
   hipStream_t own_stream = nullptr;
+  // workspace of the one-system-at-a-time baseline (sized for the largest system)
+  nbx_direct* seq = nullptr;
   // captured step chunk
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -298,6 +300,7 @@ void ens_release(nbx_ensemble* e) {
   for (auto* it : e->d_part_off)
     if (it) (void)hipFree(it);
   if (e->d_part) (void)hipFree(e->d_part);
+  if (e->seq) (void)nbx_direct_destroy(e->seq);
   if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
 }
 
@@ -417,6 +420,7 @@ int nbx_ens_create(int nsys, const int64_t* counts, nbx_ensemble** out) {
     rc = fail(NBX_ERR_HIP, "energy block table upload failed");
   if (rc == NBX_OK && hipStreamCreateWithFlags(&e->own_stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(NBX_ERR_HIP, "stream creation failed");
+  if (rc == NBX_OK) rc = nbx_direct_create((size_t)*std::max_element(n.begin(), n.end()), 0, &e->seq);
   e->variant = 4;  // ipl1_nb8_ws16: smallest tile, best for systems of a few thousand bodies
   if (rc != NBX_OK) {
     ens_release(e);
@@ -498,7 +502,7 @@ int nbx_ens_steps_sequential(nbx_ensemble* e, float G, float eps2, float dt, int
                        &v.acc_y, &v.acc_z, &v.acc_old_x, &v.acc_old_y, &v.acc_old_z, &v.mass};
       for (float** a : f) *a += off;
       v.count = (size_t)(e->offsets[q + 1] - e->offsets[q]);
-      NBX_TRY(nbx_direct_step(&v, G, eps2, dt, s));
+      NBX_TRY(nbx_direct_compute_step(e->seq, &v, G, eps2, dt, s));
     }
   return NBX_OK;
 }

@@ -8,8 +8,7 @@
 //  * each target's sum is Kahan-compensated in FP32 over its whole j chunk;
 //  * FP64 appears only in the cross-wave reduction of the compensated sums, the
 //    hi/lo renormalisation of the drift, and the energy/momentum reductions.
-#include <map>
-#include <mutex>
+#include <algorithm>
 
 #include "common.hpp"
 #include "direct_core.hpp"
@@ -279,7 +278,7 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
 #pragma unroll
         for (int w = 0; w < W; ++w) t += buf[w][k][l];
         if (UNI) t *= (double)mref;
-        g.pj[((long)(dd - 1) * 3 + k) * g.npad + jbase + ch * 64 + l] = (float)t;
+        g.pj[((long)(dd - g.pj_lo()) * 3 + k) * g.npad + jbase + ch * 64 + l] = (float)t;
       }
     }
   }
@@ -288,7 +287,7 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
   for (int q = 0; q < IPL; ++q)
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      g.pi[((long)dd * 3 + a) * g.npad + ibase + q * 64 + lane] = (float)sx[q][a];
+      g.pi[((long)(dd - g.d_lo) * 3 + a) * g.npad + ibase + q * 64 + lane] = (float)sx[q][a];
 }
 
 template <int IPL, int W, bool CHECKED>
@@ -338,10 +337,11 @@ __global__ __launch_bounds__(64 * W) void ds_sym_kernel(
 
 template <bool KICK>
 __global__ __launch_bounds__(256) void ds_sym_reduce(SymGeom g, double G, float* ax, float* ay,
-                                                     float* az, KickArgs kick) {
+                                                     float* az, KickArgs kick, double* carry,
+                                                     int last) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= g.n) return;
-  sym_reduce_body<KICK>(g, b, G, ax, ay, az, kick);
+  sym_reduce_body<KICK>(g, b, G, ax, ay, az, kick, carry, last != 0);
 }
 
 // acc_old <- acc; (hi, lo) += v dt + a dt^2/2 evaluated and renormalised in FP64.
@@ -376,58 +376,63 @@ constexpr int kIpl = 2, kWs = 8;           // one-sided shape
 constexpr int kSymIpl = 4, kSymW = 4;      // pair-symmetric shape, super-blocks of 1,024
 constexpr int kSymS = 64 * kSymIpl * kSymW;
 
-// -1: symmetric from 2^15 bodies (partials permitting), 0: one-sided, 1: symmetric.
+// -1: symmetric from 2^15 bodies when the call brings a workspace, 0: one-sided,
+// 1: symmetric (needs a workspace).
 int g_ds_kernel = -1;
 
-std::mutex g_ds_mu;
-std::map<std::pair<int, hipStream_t>, Scratch> g_ds_scratch;
-
+// The symmetric tiles in rounds of distances, as direct_sym_launch.
 int ds_sym_launch(nbx_ds_particles* d, float G, float eps2, const KickArgs* kick,
-                  hipStream_t s) {
+                  const SymWork& w, hipStream_t s) {
   nbx_particles& p = d->p;
   const int n = (int)p.count;
-  float* part = nullptr;
-  {
-    int dev = 0;
-    NBX_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_ds_mu);
-    Scratch& sc = g_ds_scratch[std::make_pair(dev, s)];
-    NBX_TRY(sc.ensure(sym_partial_floats((size_t)n, kSymS) * sizeof(float)));
-    part = (float*)sc.ptr;
-  }
-  const SymGeom g = sym_geom(n, kSymS, part);
-  const unsigned tiles = (unsigned)sym_tiles(g);
-  if (eps2 < kUncheckedMinEps2)
-    hipLaunchKernelGGL((ds_sym_kernel<kSymIpl, kSymW, true>), dim3(tiles), dim3(64 * kSymW), 0,
-                       s, p.pos_x, p.pos_y, p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z,
-                       p.mass, g, eps2);
-  else
-    hipLaunchKernelGGL((ds_sym_kernel<kSymIpl, kSymW, false>), dim3(tiles), dim3(64 * kSymW), 0,
-                       s, p.pos_x, p.pos_y, p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z,
-                       p.mass, g, eps2);
-  NBX_LAUNCHED();
+  const int R = direct_sym_round_len((size_t)n, kSymS, w.part_bytes);
+  const SymGeom g = sym_geom(n, kSymS, w.part);
+  if (R < 0 || !w.part)
+    return fail(NBX_ERR_CAPACITY, "pair-symmetric scratch of %zu bytes too small for %d bodies",
+                w.part_bytes, n);
+  if (R < g.Dmax && (!w.carry || w.carry_bytes < direct_sym_carry_bytes((size_t)n, kSymS)))
+    return fail(NBX_ERR_CAPACITY, "pair-symmetric evaluation of %d bodies in rounds needs a "
+                "%zu-byte carry", n, direct_sym_carry_bytes((size_t)n, kSymS));
   KickArgs k{};
   if (kick) k = *kick;
   const unsigned rb = div_up((size_t)n, 256);
-  if (kick)
-    hipLaunchKernelGGL(ds_sym_reduce<true>, dim3(rb), dim3(256), 0, s, g, (double)G, p.acc_x,
-                       p.acc_y, p.acc_z, k);
-  else
-    hipLaunchKernelGGL(ds_sym_reduce<false>, dim3(rb), dim3(256), 0, s, g, (double)G, p.acc_x,
-                       p.acc_y, p.acc_z, k);
-  NBX_LAUNCHED();
+  for (int lo = 0;;) {
+    const int hi = std::min(g.Dmax, lo == 0 ? R : lo + R - 1);
+    const int last = hi == g.Dmax ? 1 : 0;
+    const SymGeom gr = sym_round(g, lo, hi, w.part);
+    const unsigned tiles = (unsigned)sym_tiles(gr);
+    if (eps2 < kUncheckedMinEps2)
+      hipLaunchKernelGGL((ds_sym_kernel<kSymIpl, kSymW, true>), dim3(tiles), dim3(64 * kSymW), 0,
+                         s, p.pos_x, p.pos_y, p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z,
+                         p.mass, gr, eps2);
+    else
+      hipLaunchKernelGGL((ds_sym_kernel<kSymIpl, kSymW, false>), dim3(tiles), dim3(64 * kSymW),
+                         0, s, p.pos_x, p.pos_y, p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z,
+                         p.mass, gr, eps2);
+    NBX_LAUNCHED();
+    if (kick)
+      hipLaunchKernelGGL(ds_sym_reduce<true>, dim3(rb), dim3(256), 0, s, gr, (double)G, p.acc_x,
+                         p.acc_y, p.acc_z, k, w.carry, last);
+    else
+      hipLaunchKernelGGL(ds_sym_reduce<false>, dim3(rb), dim3(256), 0, s, gr, (double)G, p.acc_x,
+                         p.acc_y, p.acc_z, k, w.carry, last);
+    NBX_LAUNCHED();
+    if (last) break;
+    lo = hi + 1;
+  }
   return NBX_OK;
 }
 
-int ds_forces_launch(nbx_ds_particles* d, float G, float eps2, const KickArgs* kick,
-                     hipStream_t s) {
+int ds_forces_launch(nbx_direct* op, nbx_ds_particles* d, float G, float eps2,
+                     const KickArgs* kick, hipStream_t s) {
   nbx_particles& p = d->p;
   if (!(eps2 >= 0.0f) || !(eps2 < 3.0e38f))
     return fail(NBX_ERR_INVALID, "softening eps2 must be finite and >= 0");
   const int n = (int)p.count;
-  const bool fits = sym_partial_floats((size_t)n, kSymS) * sizeof(float) <= kSymMaxPartialBytes;
-  const bool sym = g_ds_kernel == 1 ? fits : (g_ds_kernel == -1 && n >= (1 << 15) && fits);
-  if (sym) return ds_sym_launch(d, G, eps2, kick, s);
+  if (g_ds_kernel == 1 && !op)
+    return fail(NBX_ERR_INVALID, "the pair-symmetric double-single kernel needs a workspace");
+  const bool sym = op && (g_ds_kernel == 1 || (g_ds_kernel == -1 && n >= (1 << 15)));
+  if (sym) return ds_sym_launch(d, G, eps2, kick, op->work, s);
   dim3 grid(div_up(n, 64 * kIpl)), block(64 * kWs);
   KickArgs k{};
   if (kick) k = *kick;
@@ -468,6 +473,19 @@ int check_ds(const nbx_ds_particles* d, const char* who) {
   return NBX_OK;
 }
 
+int check_ds_ws(const nbx_direct* op, const nbx_ds_particles* d, const char* who) {
+  if (!op) return NBX_OK;
+  if (d->p.count > op->max_particles)
+    return fail(NBX_ERR_CAPACITY, "%s: %zu bodies exceed the workspace's %zu", who, d->p.count,
+                op->max_particles);
+  int dev = 0;
+  NBX_HIP(hipGetDevice(&dev));
+  if (dev != op->device)
+    return fail(NBX_ERR_STATE, "%s: workspace of device %d used on device %d", who, op->device,
+                dev);
+  return NBX_OK;
+}
+
 }  // namespace
 
 }  // namespace nbx
@@ -503,9 +521,10 @@ int nbx_ds_free(nbx_ds_particles* d) {
   return nbx_particles_free(&d->p);
 }
 
-int nbx_ds_forces(nbx_ds_particles* d, float G, float eps2, nbx_stream s) {
+int nbx_ds_forces(nbx_direct* op, nbx_ds_particles* d, float G, float eps2, nbx_stream s) {
   NBX_TRY(check_ds(d, "nbx_ds_forces"));
-  return ds_forces_launch(d, G, eps2, nullptr, as_stream(s));
+  NBX_TRY(check_ds_ws(op, d, "nbx_ds_forces"));
+  return ds_forces_launch(op, d, G, eps2, nullptr, as_stream(s));
 }
 
 int nbx_ds_set_kernel(int kind) {
@@ -519,13 +538,15 @@ int nbx_ds_drift(nbx_ds_particles* d, float dt, nbx_stream s) {
   return ds_drift_launch(d, dt, as_stream(s));
 }
 
-int nbx_ds_step(nbx_ds_particles* d, float G, float eps2, float dt, nbx_stream s) {
+int nbx_ds_step(nbx_direct* op, nbx_ds_particles* d, float G, float eps2, float dt,
+                nbx_stream s) {
   NBX_TRY(check_ds(d, "nbx_ds_step"));
+  NBX_TRY(check_ds_ws(op, d, "nbx_ds_step"));
   hipStream_t st = as_stream(s);
   NBX_TRY(ds_drift_launch(d, dt, st));
   nbx_particles& p = d->p;
   KickArgs k{p.vel_x, p.vel_y, p.vel_z, p.acc_old_x, p.acc_old_y, p.acc_old_z, 0.5f * dt};
-  return ds_forces_launch(d, G, eps2, &k, st);
+  return ds_forces_launch(op, d, G, eps2, &k, st);
 }
 
 int nbx_ds_energy(const nbx_ds_particles* d, float G, float eps2, double* ke, double* pe,

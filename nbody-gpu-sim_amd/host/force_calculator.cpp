@@ -2,6 +2,7 @@
 // src/cuda/force_barnes_hut.cu:573-583, src/cuda/force_spatial_hash.cu:370-404).
 #include <algorithm>
 #include <cmath>
+#include <string>
 
 #include "nbody/barnes_hut_tree.hpp"
 #include "nbody/error_handling.hpp"
@@ -16,9 +17,38 @@ static nbx_particles* raw(ParticleData* d) { return reinterpret_cast<nbx_particl
 
 DirectForceCalculator::DirectForceCalculator(int block_size) : block_size_(block_size) {}
 
+DirectForceCalculator::~DirectForceCalculator() { (void)nbx_direct_destroy(workspace_); }
+
+void DirectForceCalculator::reserve(size_t max_particles) {
+  if (workspace_ && max_particles <= capacity_) return;
+  nbx_direct* ws = nullptr;
+  NBODY_CHECK(nbx_direct_create(max_particles, 0, &ws));
+  (void)nbx_direct_destroy(workspace_);
+  workspace_ = ws;
+  capacity_ = max_particles;
+}
+
+// The block size is checked as launchDirectForceKernel's launch would reject it
+// (force_direct.cu:88-98); the tile shape itself comes from the autotuner.
+static void check_block_size(int b) {
+  if (b < 1 || b > 1024)
+    throw ValidationException("block size must be between 1 and 1024 (got " +
+                              std::to_string(b) + ")");
+}
+
 void DirectForceCalculator::computeForces(ParticleData* d_particles) {
   NBODY_PROFILE_SCOPE("force.direct_n2");
-  NBODY_CHECK(nbx_direct_forces(raw(d_particles), G_, softening_eps2_, block_size_, nullptr));
+  check_block_size(block_size_);
+  reserve(d_particles->count);
+  NBODY_CHECK(nbx_direct_compute(workspace_, raw(d_particles), G_, softening_eps2_, nullptr));
+}
+
+void DirectForceCalculator::computeForcesKick(ParticleData* d_particles, float dt) {
+  NBODY_PROFILE_SCOPE("force.direct_n2");
+  check_block_size(block_size_);
+  reserve(d_particles->count);
+  NBODY_CHECK(
+      nbx_direct_compute_kick(workspace_, raw(d_particles), G_, softening_eps2_, dt, nullptr));
 }
 
 BarnesHutCalculator::BarnesHutCalculator(float theta) : theta_(theta) {}

@@ -27,13 +27,10 @@ Integrator::~Integrator() = default;
 
 void Integrator::integrate(ParticleData* d, ForceCalculator* force_calc, float dt) {
   NBODY_PROFILE_SCOPE("integrator.velocity_verlet");
-  if (dynamic_cast<DirectForceCalculator*>(force_calc)) {
+  if (auto* direct = dynamic_cast<DirectForceCalculator*>(force_calc)) {
     // same four reference kernels, fused into two launches (drift; forces + kick)
     NBODY_CHECK(nbx_verlet_drift(raw(d), dt, nullptr));
-    NBODY_CHECK(nbx_direct_forces_kick(raw(d), force_calc->getGravitationalConstant(),
-                                       force_calc->getSofteningParameter() *
-                                           force_calc->getSofteningParameter(),
-                                       dt, nullptr));
+    direct->computeForcesKick(d, dt);
     return;
   }
   NBODY_CHECK(nbx_verlet_drift(raw(d), dt, nullptr));

@@ -88,6 +88,13 @@ _sig("nbx_direct_forces", C.c_int, PP, C.c_float, C.c_float, C.c_int, VP)
 _sig("nbx_direct_forces_range", C.c_int, PP, C.c_size_t, C.c_size_t, C.c_float, C.c_float, VP)
 _sig("nbx_direct_step", C.c_int, PP, C.c_float, C.c_float, C.c_float, VP)
 _sig("nbx_direct_forces_kick", C.c_int, PP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_direct_create", C.c_int, C.c_size_t, C.c_size_t, C.POINTER(VP))
+_sig("nbx_direct_destroy", C.c_int, VP)
+_sig("nbx_direct_compute", C.c_int, VP, PP, C.c_float, C.c_float, VP)
+_sig("nbx_direct_compute_kick", C.c_int, VP, PP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_direct_compute_step", C.c_int, VP, PP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_direct_plan", C.c_int, VP, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int),
+     C.POINTER(C.c_size_t))
 _sig("nbx_direct_variant_count", C.c_int)
 _sig("nbx_direct_variant_name", C.c_int, C.c_int, C.c_char_p, C.c_size_t)
 _sig("nbx_direct_set_variant", C.c_int, C.c_int)
@@ -118,10 +125,10 @@ _sig("nbx_grid_copy_cells", C.c_int, VP, I32P, I32P, I32P, I32P)
 _sig("nbx_grid_set_walk", C.c_int, VP, C.c_int)
 _sig("nbx_ds_alloc", C.c_int, DSP, C.c_size_t)
 _sig("nbx_ds_free", C.c_int, DSP)
-_sig("nbx_ds_forces", C.c_int, DSP, C.c_float, C.c_float, VP)
+_sig("nbx_ds_forces", C.c_int, VP, DSP, C.c_float, C.c_float, VP)
 _sig("nbx_ds_set_kernel", C.c_int, C.c_int)
 _sig("nbx_ds_drift", C.c_int, DSP, C.c_float, VP)
-_sig("nbx_ds_step", C.c_int, DSP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_ds_step", C.c_int, VP, DSP, C.c_float, C.c_float, C.c_float, VP)
 _sig("nbx_ds_energy", C.c_int, DSP, C.c_float, C.c_float, F64P, F64P, F64P, VP)
 _sig("nbx_ens_create", C.c_int, C.c_int, I64P, C.POINTER(VP))
 _sig("nbx_ens_destroy", C.c_int, VP)
@@ -148,7 +155,8 @@ EXPORTED = [
     "nbx_event_destroy", "nbx_event_record", "nbx_event_elapsed_ms", "nbx_particles_alloc",
     "nbx_particles_free", "nbx_particles_upload", "nbx_particles_download", "nbx_direct_forces",
     "nbx_direct_forces_range", "nbx_direct_step", "nbx_direct_forces_kick",
-    "nbx_direct_variant_count",
+    "nbx_direct_create", "nbx_direct_destroy", "nbx_direct_compute", "nbx_direct_compute_kick",
+    "nbx_direct_compute_step", "nbx_direct_plan", "nbx_direct_variant_count",
     "nbx_direct_variant_name", "nbx_direct_set_variant", "nbx_direct_variant_for",
     "nbx_direct_autotune",
     "nbx_store_old_accelerations", "nbx_update_positions", "nbx_update_velocities",
@@ -325,15 +333,66 @@ def device_info() -> dict:
             "mem": mem.value}
 
 
+def eps2_of(eps: float) -> float:
+    """softening_eps2_ = eps * eps in FP32 (include/nbody/force_calculator.hpp:15-18)."""
+    return float(np.float32(np.float32(eps) * np.float32(eps)))
+
+
+class DirectWorkspace:
+    """nbx_direct: the pair-symmetric kernel's workspace, as DirectForceCalculator owns it.
+
+    Allocated once for up to max_particles bodies (scratch_bytes bounds its partial
+    sums; 0 takes the one-round need, at most 1 GiB) and freed with the object.
+    """
+
+    def __init__(self, max_particles: int, scratch_bytes: int = 0):
+        self.h = VP()
+        self.max_particles = max_particles
+        check(lib.nbx_direct_create(max_particles, scratch_bytes, C.byref(self.h)))
+
+    def forces(self, d: "DeviceParticles", G: float, eps: float, stream=None) -> None:
+        check(lib.nbx_direct_compute(self.h, d.ptr, G, eps2_of(eps), _stream(stream)))
+
+    def forces_kick(self, d: "DeviceParticles", G: float, eps: float, dt: float,
+                    stream=None) -> None:
+        check(lib.nbx_direct_compute_kick(self.h, d.ptr, G, eps2_of(eps), dt, _stream(stream)))
+
+    def step(self, d: "DeviceParticles", G: float, eps: float, dt: float, stream=None) -> None:
+        check(lib.nbx_direct_compute_step(self.h, d.ptr, G, eps2_of(eps), dt, _stream(stream)))
+
+    def plan(self, n: int) -> dict:
+        v, r, b = C.c_int(), C.c_int(), C.c_size_t()
+        check(lib.nbx_direct_plan(self.h, n, C.byref(v), C.byref(r), C.byref(b)))
+        return {"variant": v.value, "rounds": r.value, "partial_bytes": b.value}
+
+    def __del__(self):
+        try:
+            lib.nbx_direct_destroy(self.h)
+        except Exception:
+            pass
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return stream.h if isinstance(stream, Stream) else stream
+
+
 def direct_forces(d: DeviceParticles, G: float, eps: float, block_size: int = 256,
-                  stream=None) -> None:
-    eps2 = np.float32(np.float32(eps) * np.float32(eps))
-    check(lib.nbx_direct_forces(d.ptr, G, float(eps2), block_size, stream))
+                  stream=None, ws: DirectWorkspace | None = None) -> None:
+    """Forces of every body; with a workspace the pair-symmetric kernel may run."""
+    if ws is not None:
+        ws.forces(d, G, eps, stream)
+        return
+    check(lib.nbx_direct_forces(d.ptr, G, eps2_of(eps), block_size, _stream(stream)))
 
 
-def direct_step(d: DeviceParticles, G: float, eps: float, dt: float, stream=None) -> None:
-    eps2 = np.float32(np.float32(eps) * np.float32(eps))
-    check(lib.nbx_direct_step(d.ptr, G, float(eps2), dt, stream))
+def direct_step(d: DeviceParticles, G: float, eps: float, dt: float, stream=None,
+                ws: DirectWorkspace | None = None) -> None:
+    if ws is not None:
+        ws.step(d, G, eps, dt, stream)
+        return
+    check(lib.nbx_direct_step(d.ptr, G, eps2_of(eps), dt, _stream(stream)))
 
 
 def energy(d: DeviceParticles, G: float, eps: float, potential: bool = True) -> dict:

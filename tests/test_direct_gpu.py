@@ -15,11 +15,13 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def run_forces(nbx, h, G, eps, variant=-1):
+def run_forces(nbx, h, G, eps, variant=-1, scratch=0, workspace=True):
+    """Forces through DirectForceCalculator's workspace (or the workspace-free launcher)."""
     d = nbx.DeviceParticles.from_host(h)
+    ws = nbx.DirectWorkspace(len(h["mass"]), scratch) if workspace else None
     nbx.check(nbx.lib.nbx_direct_set_variant(variant))
     try:
-        nbx.direct_forces(d, G, eps)
+        nbx.direct_forces(d, G, eps, ws=ws)
         nbx.synchronize()
         return d.acc().astype(np.float64)
     finally:
@@ -252,10 +254,10 @@ def test_sym_kick_epilogue(nbx):
         d = nbx.DeviceParticles.from_host(h)
         for f in ("acc_old_x", "acc_old_y", "acc_old_z"):
             d.write(f, h[f])  # upload() leaves acc_old alone, as the reference copy does
+        ws = nbx.DirectWorkspace(n)
         nbx.check(nbx.lib.nbx_direct_set_variant(v))
         try:
-            nbx.check(nbx.lib.nbx_direct_forces_kick(d.ptr, 1.0, float(np.float32(0.01) ** 2),
-                                                     float(dt), None))
+            ws.forces_kick(d, 1.0, 0.01, float(dt))
             nbx.synchronize()
         finally:
             nbx.check(nbx.lib.nbx_direct_set_variant(-1))
@@ -283,3 +285,76 @@ def test_sym_range_call_uses_one_sided(nbx):
             nbx.check(nbx.lib.nbx_direct_set_variant(-1))
         res.append(d.acc())
     assert np.array_equal(res[0], res[1])
+
+
+def test_workspace_free_launcher_runs_one_sided(nbx):
+    # nbx_direct_forces takes no workspace: a forced symmetric shape falls back to the
+    # one-sided default, bit for bit
+    n = 70000
+    h = nbx.gen_plummer(n, seed=2)
+    sym = [v for v, _ in sym_variants(nbx)][1]
+    a = run_forces(nbx, h, 1.0, 0.01, variant=sym, workspace=False)
+    b = run_forces(nbx, h, 1.0, 0.01, variant=3, workspace=True)  # ipl1_nb8_ws8 at 2^16..2^17
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("blocks", [3, 4.5, 8, 11])
+def test_sym_rounds_bit_identical(nbx, blocks):
+    # partial scratch of 3, 5 and 8 slots gives rounds of 1, 2 and 3 tile distances;
+    # every round split sums each body's partials in the same order
+    for v, name in sym_variants(nbx):
+        S = sym_block(name)
+        n = int(blocks * S) + 17
+        if n > 24000:
+            continue
+        h = nbx.gen_uniform_sphere(n, seed=v)
+        whole = run_forces(nbx, h, 1.0, 0.01, variant=v)
+        npad = -(-n // 2048) * 2048
+        for slots in (3, 5, 8):
+            ws = nbx.DirectWorkspace(n, slots * 3 * npad * 4)
+            nbx.check(nbx.lib.nbx_direct_set_variant(v))
+            try:
+                plan = ws.plan(n)
+                d = nbx.DeviceParticles.from_host(h)
+                ws.forces(d, 1.0, 0.01)
+                nbx.synchronize()
+            finally:
+                nbx.check(nbx.lib.nbx_direct_set_variant(-1))
+            K = -(-n // S)
+            if slots * npad // (K * S) < 2 * (K // 2) + 1:
+                assert plan["rounds"] > 1, (name, slots, plan)
+            assert np.array_equal(d.acc().astype(np.float64), whole), (name, n, slots)
+
+
+def test_workspace_errors(nbx):
+    with pytest.raises(nbx.NbxError):
+        nbx.DirectWorkspace(0)
+    with pytest.raises(nbx.NbxError):
+        nbx.DirectWorkspace(100000, 1000)  # below three partial slots
+    ws = nbx.DirectWorkspace(1000)
+    d = nbx.DeviceParticles(1001)
+    with pytest.raises(nbx.NbxError):
+        ws.forces(d, 1.0, 0.01)
+    with pytest.raises(nbx.NbxError):
+        ws.plan(1001)
+    assert ws.plan(1000)["rounds"] == 1
+
+
+def test_sym_one_million_in_rounds(nbx, orc):
+    # 1,048,576 bodies: the default workspace (1 GiB of partials) runs the
+    # pair-symmetric tiles in rounds; 4,096 sample targets vs the FP64 sum
+    n = 1 << 20
+    h = nbx.gen_uniform_sphere(n, seed=42, R=1.0, sigma=0.05)
+    ws = nbx.DirectWorkspace(n)
+    plan = ws.plan(n)
+    print(f"1M plan: {plan}", flush=True)
+    assert nbx.variant_names()[plan["variant"]].startswith("sym_")
+    assert plan["rounds"] > 1 and plan["partial_bytes"] <= 1 << 30
+    d = nbx.DeviceParticles.from_host(h)
+    ws.forces(d, 1.0, 0.005)
+    nbx.synchronize()
+    t = np.sort(np.random.default_rng(7).choice(n, 4096, replace=False))
+    ref = orc.direct_f64(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 1.0, orc.eps2_of(0.005), t)
+    st = error_stats(d.acc().astype(np.float64)[t], ref)
+    print(f"1M in rounds: {st}", flush=True)
+    assert st["median"] <= 3e-7 and st["p99"] <= 1e-6

@@ -31,9 +31,10 @@ def small(nbx):
 def standalone(nbx, h, sl, variant):
     sub = {k: np.ascontiguousarray(v[sl]) for k, v in h.items()}
     d = nbx.DeviceParticles.from_host(sub)
+    ws = nbx.DirectWorkspace(len(sub["mass"]))
     nbx.check(nbx.lib.nbx_direct_set_variant(variant))
     try:
-        nbx.direct_forces(d, G, EPS)
+        nbx.direct_forces(d, G, EPS, ws=ws)
         nbx.synchronize()
     finally:
         nbx.check(nbx.lib.nbx_direct_set_variant(-1))

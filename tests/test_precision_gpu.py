@@ -17,6 +17,12 @@ def eps2(orc, eps):
     return orc.eps2_of(eps)
 
 
+@pytest.fixture(scope="module")
+def ws(nbx):
+    """Workspace of the pair-symmetric kernels for up to 8,192 bodies."""
+    return nbx.DirectWorkspace(8192)
+
+
 @pytest.fixture(params=[0, 1], ids=["one_sided", "pair_symmetric"])
 def ds_kernel(nbx, request):
     """Runs a test with each double-single force kernel forced."""
@@ -25,7 +31,7 @@ def ds_kernel(nbx, request):
     nbx.check(nbx.lib.nbx_ds_set_kernel(-1))
 
 
-def test_ds_forces_match_fp64(nbx, orc, ds_kernel):
+def test_ds_forces_match_fp64(nbx, orc, ws, ds_kernel):
     h = nbx.gen_plummer(4096, seed=42)
     rng = np.random.default_rng(0)
     x64 = [h[k].astype(np.float64) + rng.uniform(-1, 1, 4096) * 2.0 ** -26 * np.abs(h[k])
@@ -33,7 +39,7 @@ def test_ds_forces_match_fp64(nbx, orc, ds_kernel):
     d = nbx.DSDeviceParticles.from_host(h, pos64=x64)
     lo = d.read("pos_lo_x")
     assert np.any(lo != 0)
-    nbx.check(nbx.lib.nbx_ds_forces(d.ptr, 1.0, eps2(orc, 0.01), None))
+    nbx.check(nbx.lib.nbx_ds_forces(ws.h, d.ptr, 1.0, eps2(orc, 0.01), None))
     nbx.synchronize()
     got = d.acc().astype(np.float64)
     pos = d.positions64()
@@ -43,12 +49,12 @@ def test_ds_forces_match_fp64(nbx, orc, ds_kernel):
     assert np.median(e) <= 3e-7 and np.quantile(e, 0.99) <= 1e-6
 
 
-def test_ds_forces_ragged_and_zero_softening(nbx, orc, ds_kernel):
+def test_ds_forces_ragged_and_zero_softening(nbx, orc, ws, ds_kernel):
     # 1000 and 2500 bodies give 1 and 3 super-blocks of 1,024 in the symmetric kernel
     for n, eps in ((1, 0.01), (65, 0.01), (1000, 0.0), (2500, 0.01), (2500, 0.0)):
         h = nbx.gen_uniform_sphere(n, seed=n)
         d = nbx.DSDeviceParticles.from_host(h)
-        nbx.check(nbx.lib.nbx_ds_forces(d.ptr, 1.0, eps2(orc, eps), None))
+        nbx.check(nbx.lib.nbx_ds_forces(ws.h, d.ptr, 1.0, eps2(orc, eps), None))
         nbx.synchronize()
         got = d.acc().astype(np.float64)
         ref = orc.direct_f64(h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 1.0, eps2(orc, eps))
@@ -59,7 +65,7 @@ def test_ds_forces_ragged_and_zero_softening(nbx, orc, ds_kernel):
         assert np.quantile(e, 0.99) <= 1e-5, (n, eps)
 
 
-def test_ds_symmetric_mixed_masses_and_kick(nbx, orc):
+def test_ds_symmetric_mixed_masses_and_kick(nbx, orc, ws):
     # per-pair masses on both sides; then one step, whose kick is
     # v += (a_prev + a_new) (0.5 dt) in IEEE single rounding
     n = 5000
@@ -73,7 +79,7 @@ def test_ds_symmetric_mixed_masses_and_kick(nbx, orc):
     dt = np.float32(1e-3)
     nbx.check(nbx.lib.nbx_ds_set_kernel(1))
     try:
-        nbx.check(nbx.lib.nbx_ds_forces(d.ptr, 1.0, e2, None))
+        nbx.check(nbx.lib.nbx_ds_forces(ws.h, d.ptr, 1.0, e2, None))
         nbx.synchronize()
         got = d.acc().astype(np.float64)
         pos = d.positions64()
@@ -82,7 +88,7 @@ def test_ds_symmetric_mixed_masses_and_kick(nbx, orc):
         assert np.median(e) <= 3e-7 and np.quantile(e, 0.99) <= 1e-6
         a_prev = [d.read(f) for f in ("acc_x", "acc_y", "acc_z")]
         v_prev = [d.read(f) for f in ("vel_x", "vel_y", "vel_z")]
-        nbx.check(nbx.lib.nbx_ds_step(d.ptr, 1.0, e2, float(dt), None))
+        nbx.check(nbx.lib.nbx_ds_step(ws.h, d.ptr, 1.0, e2, float(dt), None))
         nbx.synchronize()
     finally:
         nbx.check(nbx.lib.nbx_ds_set_kernel(-1))
@@ -158,18 +164,18 @@ def fp64_run(h, G, eps2_, dt, steps):
     return p
 
 
-def test_ds_tracks_fp64_integration_better(nbx, orc):
+def test_ds_tracks_fp64_integration_better(nbx, orc, ws):
     n, dt, steps, eps = 256, 1e-3, 300, 0.01
     h = nbx.gen_plummer(n, seed=5)
     e2 = eps2(orc, eps)
     ref = fp64_run(h, 1.0, e2, float(np.float32(dt)), steps)
     plain = nbx.DeviceParticles.from_host(h)
-    nbx.direct_forces(plain, 1.0, eps)
+    nbx.direct_forces(plain, 1.0, eps, ws=ws)
     ds = nbx.DSDeviceParticles.from_host(h)
-    nbx.check(nbx.lib.nbx_ds_forces(ds.ptr, 1.0, e2, None))
+    nbx.check(nbx.lib.nbx_ds_forces(ws.h, ds.ptr, 1.0, e2, None))
     for _ in range(steps):
-        nbx.direct_step(plain, 1.0, eps, dt)
-        nbx.check(nbx.lib.nbx_ds_step(ds.ptr, 1.0, e2, dt, None))
+        nbx.direct_step(plain, 1.0, eps, dt, ws=ws)
+        nbx.check(nbx.lib.nbx_ds_step(ws.h, ds.ptr, 1.0, e2, dt, None))
     nbx.synchronize()
     pp = np.stack([plain.read(k) for k in ("pos_x", "pos_y", "pos_z")], 1).astype(np.float64)
     pd = np.stack(ds.positions64(), 1)
@@ -177,3 +183,38 @@ def test_ds_tracks_fp64_integration_better(nbx, orc):
     dev_ds = np.abs(pd - ref).max()
     print(f"max |x - x_fp64| after {steps} steps: plain {dev_plain:.2e}, double-single {dev_ds:.2e}")
     assert dev_ds < dev_plain
+
+
+def test_ds_symmetric_needs_workspace(nbx, orc):
+    d = nbx.DSDeviceParticles.from_host(nbx.gen_plummer(2000, seed=1))
+    nbx.check(nbx.lib.nbx_ds_set_kernel(1))
+    try:
+        with pytest.raises(nbx.NbxError):
+            nbx.check(nbx.lib.nbx_ds_forces(None, d.ptr, 1.0, eps2(orc, 0.01), None))
+    finally:
+        nbx.check(nbx.lib.nbx_ds_set_kernel(-1))
+    small = nbx.DirectWorkspace(1000)
+    with pytest.raises(nbx.NbxError):
+        nbx.check(nbx.lib.nbx_ds_forces(small.h, d.ptr, 1.0, eps2(orc, 0.01), None))
+
+
+def test_ds_symmetric_rounds_bit_identical(nbx, orc):
+    # a workspace with room for three partial slots runs the tiles in rounds of one
+    # distance; the accelerations equal the one-round run bit for bit
+    n = 7 * 1024 + 300
+    h = nbx.gen_plummer(n, seed=4)
+    x64 = [h[k].astype(np.float64) + np.random.default_rng(3).uniform(-1, 1, n) * 1e-8
+           for k in ("pos_x", "pos_y", "pos_z")]
+    e2 = eps2(orc, 0.01)
+    res = []
+    for scratch in (0, 9 * 4 * 8 * 1024):
+        w = nbx.DirectWorkspace(n, scratch)
+        d = nbx.DSDeviceParticles.from_host(h, pos64=x64)
+        nbx.check(nbx.lib.nbx_ds_set_kernel(1))
+        try:
+            nbx.check(nbx.lib.nbx_ds_forces(w.h, d.ptr, 1.0, e2, None))
+            nbx.synchronize()
+        finally:
+            nbx.check(nbx.lib.nbx_ds_set_kernel(-1))
+        res.append(d.acc())
+    assert np.array_equal(res[0], res[1])

@@ -27,6 +27,11 @@ def plummer(nbx):
 
 
 @pytest.fixture(scope="module")
+def ws(nbx):
+    return nbx.DirectWorkspace(N)
+
+
+@pytest.fixture(scope="module")
 def targets():
     return np.sort(np.random.default_rng(11).choice(N, 4096, replace=False))
 
@@ -38,18 +43,18 @@ def fp64_err(orc, h, acc, pos, tg):
 
 
 @pytest.mark.parametrize("kind", [0, 1], ids=["one_sided", "pair_symmetric"])
-def test_ds_forces_vs_fp64_start_and_after_200_steps(nbx, orc, plummer, targets, kind):
+def test_ds_forces_vs_fp64_start_and_after_200_steps(nbx, orc, ws, plummer, targets, kind):
     e2 = orc.eps2_of(EPS)
     d = nbx.DSDeviceParticles.from_host(plummer)
     nbx.check(nbx.lib.nbx_ds_set_kernel(kind))
     try:
-        nbx.check(nbx.lib.nbx_ds_forces(d.ptr, G, e2, None))
+        nbx.check(nbx.lib.nbx_ds_forces(ws.h, d.ptr, G, e2, None))
         nbx.synchronize()
         med, p99, mx = fp64_err(orc, plummer, d.acc(), d.positions64(), targets)
         print(f"kernel {kind} t=0: median {med:.2e} p99 {p99:.2e} max {mx:.2e}", flush=True)
         assert med <= 3e-7 and p99 <= 1e-6
         for _ in range(200):
-            nbx.check(nbx.lib.nbx_ds_step(d.ptr, G, e2, DT, None))
+            nbx.check(nbx.lib.nbx_ds_step(ws.h, d.ptr, G, e2, DT, None))
         nbx.synchronize()
     finally:
         nbx.check(nbx.lib.nbx_ds_set_kernel(-1))
@@ -61,13 +66,13 @@ def test_ds_forces_vs_fp64_start_and_after_200_steps(nbx, orc, plummer, targets,
     assert med <= 3e-7 and p99 <= 1e-6
 
 
-def test_ds_energy_drift_within_plain(nbx, orc, plummer):
+def test_ds_energy_drift_within_plain(nbx, orc, ws, plummer):
     e2 = orc.eps2_of(EPS)
     C = nbx.C
     plain = nbx.DeviceParticles.from_host(plummer)
     ds = nbx.DSDeviceParticles.from_host(plummer)
-    nbx.check(nbx.lib.nbx_direct_forces(plain.ptr, G, e2, 256, None))
-    nbx.check(nbx.lib.nbx_ds_forces(ds.ptr, G, e2, None))
+    nbx.check(nbx.lib.nbx_direct_compute(ws.h, plain.ptr, G, e2, None))
+    nbx.check(nbx.lib.nbx_ds_forces(ws.h, ds.ptr, G, e2, None))
 
     def e_ds():
         ke, pe = C.c_double(), C.c_double()
@@ -84,8 +89,8 @@ def test_ds_energy_drift_within_plain(nbx, orc, plummer):
     worst_p = worst_d = 0.0
     for _ in range(20):
         for _ in range(500):
-            nbx.check(nbx.lib.nbx_direct_step(plain.ptr, G, e2, DT, None))
-            nbx.check(nbx.lib.nbx_ds_step(ds.ptr, G, e2, DT, None))
+            nbx.check(nbx.lib.nbx_direct_compute_step(ws.h, plain.ptr, G, e2, DT, None))
+            nbx.check(nbx.lib.nbx_ds_step(ws.h, ds.ptr, G, e2, DT, None))
         worst_p = max(worst_p, abs(e_plain() - E0p) / abs(E0p))
         worst_d = max(worst_d, abs(e_ds() - E0d) / abs(E0d))
     print(f"max |dE/E| over 10000 steps: plain {worst_p:.3e}, double-single {worst_d:.3e}",
