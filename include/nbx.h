@@ -134,6 +134,17 @@ int nbx_direct_compute_step(nbx_direct* op, nbx_particles* d, float G, float eps
 int nbx_direct_plan(const nbx_direct* op, size_t n, int* variant, int* rounds,
                     size_t* partial_bytes);
 
+/* nbx_direct_compute without a workspace and without any allocation: no
+ * reference counterpart. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The pair-symmetric partial sums and FP64 carry live in a fixed 64 MiB device
+ * array inside the library, one per GPU, held until the process ends; the
+ * tiles run in as many rounds as it needs, with accelerations bit-identical to
+ * nbx_direct_compute. NBX_ERR_CAPACITY above about 1.1 million bodies for the
+ * pair-symmetric shapes (one-sided shapes have no limit). Every call on a GPU
+ * shares its array: calls that overlap on one GPU race. */
+int nbx_direct_compute_noalloc(nbx_particles* d, float G, float eps2, nbx_stream s);
+
 /* Tile variants of the direct kernel, named iplI_nbB_wsW: I targets per lane,
  * stage groups of B bodies, W waves splitting the j range of one tile. */
 int nbx_direct_variant_count(void);

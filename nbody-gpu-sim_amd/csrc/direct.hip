@@ -200,6 +200,60 @@ static int check_ws(const nbx_direct* op, const nbx_particles* d, const char* wh
   return NBX_OK;
 }
 
+// Scratch of the allocation-free entry point (nbx_direct_compute_noalloc): a
+// fixed device array, one copy per GPU, held until the process ends. It takes
+// the FP64 carry (3 doubles per padded body, 256-byte aligned) followed by the
+// FP32 partial sums of one round; an evaluation whose partials fit whole takes
+// it all as one round. Nothing in it needs clearing: a round's reduction reads
+// only partials its own tiles wrote, and the carry is written by the first
+// round before later ones read it. Calls that overlap on one GPU race on it.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's copy of `scratch`, looked up on the first call
+// with the device current. Host code uses this address only: the array's own
+// name is a host-side stand-in, not device memory.
+static int noalloc_scratch(void** out) {
+  int dev = 0;
+  NBX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(NBX_ERR_STATE, "nbx_direct_compute_noalloc: device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(NBX_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return NBX_OK;
+}
+
+// The SymWork of an n-body evaluation with super-blocks of S inside the fixed
+// scratch at base; NBX_ERR_CAPACITY beyond what it holds.
+static int noalloc_work(void* base, size_t n, int S, SymWork* w) {
+  unsigned char* b = static_cast<unsigned char*>(base);
+  if (direct_sym_partial_bytes(n, S) <= kNoAllocScratchBytes) {
+    *w = SymWork{reinterpret_cast<float*>(b), kNoAllocScratchBytes, nullptr, 0};
+    return NBX_OK;
+  }
+  const size_t carry = (direct_sym_carry_bytes(n, S) + 255) / 256 * 256;
+  if (carry >= kNoAllocScratchBytes ||
+      direct_sym_round_len(n, S, kNoAllocScratchBytes - carry) < 1)
+    return fail(NBX_ERR_CAPACITY,
+                "nbx_direct_compute_noalloc: %zu bodies exceed its %zu-byte scratch with "
+                "%d-body super-blocks (at most about %zu bodies)",
+                n, kNoAllocScratchBytes, S, kNoAllocScratchBytes / 60);
+  *w = SymWork{reinterpret_cast<float*>(b + carry), kNoAllocScratchBytes - carry,
+               reinterpret_cast<double*>(b), carry};
+  return NBX_OK;
+}
+
 }  // namespace nbx
 
 using namespace nbx;
@@ -270,6 +324,18 @@ int nbx_direct_compute_step(nbx_direct* op, nbx_particles* d, float G, float eps
   NBX_TRY(verlet_drift_launch(d, dt, true, st));
   KickArgs k{d->vel_x, d->vel_y, d->vel_z, d->acc_old_x, d->acc_old_y, d->acc_old_z, 0.5f * dt};
   return direct_launch(d, 0, d->count, G, eps2, &k, st, -1, &op->work);
+}
+
+int nbx_direct_compute_noalloc(nbx_particles* d, float G, float eps2, nbx_stream s) {
+  NBX_TRY(check_particles(d, "nbx_direct_compute_noalloc"));
+  const int v = direct_pick_variant(d->count);
+  const int S = kVariants[v].sym_block;
+  if (!S) return direct_launch(d, 0, d->count, G, eps2, nullptr, as_stream(s), v, nullptr);
+  void* base = nullptr;
+  NBX_TRY(noalloc_scratch(&base));
+  SymWork w;
+  NBX_TRY(noalloc_work(base, d->count, S, &w));
+  return direct_launch(d, 0, d->count, G, eps2, nullptr, as_stream(s), v, &w);
 }
 
 int nbx_direct_plan(const nbx_direct* op, size_t n, int* variant, int* rounds,

@@ -93,6 +93,7 @@ _sig("nbx_direct_destroy", C.c_int, VP)
 _sig("nbx_direct_compute", C.c_int, VP, PP, C.c_float, C.c_float, VP)
 _sig("nbx_direct_compute_kick", C.c_int, VP, PP, C.c_float, C.c_float, C.c_float, VP)
 _sig("nbx_direct_compute_step", C.c_int, VP, PP, C.c_float, C.c_float, C.c_float, VP)
+_sig("nbx_direct_compute_noalloc", C.c_int, PP, C.c_float, C.c_float, VP)
 _sig("nbx_direct_plan", C.c_int, VP, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int),
      C.POINTER(C.c_size_t))
 _sig("nbx_direct_variant_count", C.c_int)
@@ -156,7 +157,8 @@ EXPORTED = [
     "nbx_particles_free", "nbx_particles_upload", "nbx_particles_download", "nbx_direct_forces",
     "nbx_direct_forces_range", "nbx_direct_step", "nbx_direct_forces_kick",
     "nbx_direct_create", "nbx_direct_destroy", "nbx_direct_compute", "nbx_direct_compute_kick",
-    "nbx_direct_compute_step", "nbx_direct_plan", "nbx_direct_variant_count",
+    "nbx_direct_compute_step", "nbx_direct_plan", "nbx_direct_compute_noalloc",
+    "nbx_direct_variant_count",
     "nbx_direct_variant_name", "nbx_direct_set_variant", "nbx_direct_variant_for",
     "nbx_direct_autotune",
     "nbx_store_old_accelerations", "nbx_update_positions", "nbx_update_velocities",
