@@ -1,7 +1,8 @@
 // Barnes-Hut tree of the engine's C++ API, mirroring the reference's
 // include/nbody/barnes_hut_tree.hpp (build-workbench/nbody-gpu-sim). The build
-// runs entirely on the GPU (nbx_bh_build); getNodes() returns the host copy
-// made by copyNodesToHost(), in the reference's 76-byte layout and numbering.
+// runs entirely on the GPU (nbx_bh_build). getNodes() returns the nodes of the
+// last build in the reference's 76-byte layout and numbering: the first call
+// after a build copies them to the host (synchronously), as copyNodesToHost() does.
 #pragma once
 
 #include <cstdint>
@@ -45,7 +46,7 @@ class BarnesHutTree {
 
   int getNodeCount() const;
   int getMaxDepth() const;
-  const OctreeNode* getNodes() const { return h_nodes_.data(); }
+  const OctreeNode* getNodes() const;
   void copyNodesToHost();
   bool verifyMassConservation(const ParticleData* h_particles) const;
 
@@ -54,7 +55,9 @@ class BarnesHutTree {
  private:
   nbx_bh* handle_ = nullptr;
   size_t max_particles_;
-  std::vector<OctreeNode> h_nodes_;
+  mutable std::vector<OctreeNode> h_nodes_;
+  mutable bool nodes_stale_ = false;  // a build ran since the last host copy
+  void refreshNodes() const;
 };
 
 // Morton code utilities (force_barnes_hut.cu:34-50 of the reference).

@@ -1,6 +1,13 @@
-// Orchestration of one simulation, mirroring the reference's
-// include/nbody/particle_system.hpp (build-workbench/nbody-gpu-sim) without
-// the OpenGL interop. Not thread-safe.
+// One simulation run behind the reference's ParticleSystem interface
+// (include/nbody/particle_system.hpp of build-workbench/nbody-gpu-sim), without
+// the OpenGL interop: the same public methods, defaults and validation. The
+// state lives on the GPU; every public method that changes the particle set
+// recomputes a(0) so the next Verlet step has it. Not thread-safe.
+//
+// Beyond the reference: a precision form for the direct method
+// (setPrecisionMode), in which positions are double-single hi + lo pairs and the
+// pair sums are compensated; its state and checkpoints carry the low parts
+// (.nbody version 2).
 #pragma once
 
 #include <memory>
@@ -27,10 +34,10 @@ class ParticleSystem {
   void initializeWithDistribution(size_t particle_count, InitDistribution dist);
 
   void update(float dt);
-  void pause() { is_paused_ = true; }
-  void resume() { is_paused_ = false; }
+  void pause();
+  void resume();
   void reset();
-  bool isPaused() const noexcept { return is_paused_; }
+  bool isPaused() const noexcept;
 
   void setForceMethod(ForceMethod method);
   void setGravitationalConstant(float G);
@@ -39,21 +46,27 @@ class ParticleSystem {
   void setBarnesHutTheta(float theta);
   void setSpatialHashCellSize(float size);
   void setSpatialHashCutoff(float cutoff);
+  // Precision form (direct method only; ValidationException otherwise). Enabling
+  // keeps the current positions (low parts zero); disabling rounds hi + lo to FP32.
+  void setPrecisionMode(bool enabled);
 
-  ForceMethod getForceMethod() const noexcept { return force_method_; }
-  float getGravitationalConstant() const noexcept { return G_; }
-  float getSofteningParameter() const noexcept { return softening_; }
-  float getTimeStep() const noexcept { return dt_; }
-  float getSimulationTime() const noexcept { return simulation_time_; }
-  size_t getParticleCount() const noexcept { return particle_count_; }
+  ForceMethod getForceMethod() const noexcept;
+  float getGravitationalConstant() const noexcept;
+  float getSofteningParameter() const noexcept;
+  float getTimeStep() const noexcept;
+  float getSimulationTime() const noexcept;
+  size_t getParticleCount() const noexcept;
+  bool getPrecisionMode() const noexcept;
 
-  ParticleData* getDeviceData() { return &d_particles_; }
-  const ParticleData* getDeviceData() const { return &d_particles_; }
+  // Device arrays (the high parts of the positions in the precision form).
+  ParticleData* getDeviceData();
+  const ParticleData* getDeviceData() const;
   void copyToHost(ParticleData& h_particles) const;
 
   void saveState(const std::string& filename) const;
   void loadState(const std::string& filename);
   SimulationState getState() const;
+  // Accepts plain state and precision-form state (low parts present, direct method).
   void setState(const SimulationState& state);
 
   float computeKineticEnergy() const;
@@ -61,24 +74,8 @@ class ParticleSystem {
   float computeTotalEnergy() const;
 
  private:
-  ParticleData d_particles_;
-  ParticleData h_particles_;
-  size_t particle_count_;
-  std::unique_ptr<ForceCalculator> force_calculator_;
-  std::unique_ptr<Integrator> integrator_;
-  float dt_;
-  float G_;
-  float softening_;
-  float simulation_time_;
-  ForceMethod force_method_;
-  bool is_paused_;
-  bool is_initialized_;
-  SimulationConfig config_;
-
-  void allocateMemory(size_t count);
-  void freeMemory();
-  void createForceCalculator();
-  void finishInitialization();
+  struct Run;
+  std::unique_ptr<Run> run_;
 };
 
 }  // namespace nbody

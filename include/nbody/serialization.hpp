@@ -1,7 +1,10 @@
-// The .nbody v1 checkpoint format, byte-compatible with the reference's
-// include/nbody/serialization.hpp (build-workbench/nbody-gpu-sim): a 56-byte
-// little-endian header followed by pos_x, pos_y, pos_z, vel_x, vel_y, vel_z,
-// mass as FP32 arrays.
+// The .nbody checkpoint format. Version 1 is byte-compatible with the
+// reference's include/nbody/serialization.hpp (build-workbench/nbody-gpu-sim):
+// a 56-byte little-endian header followed by pos_x, pos_y, pos_z, vel_x, vel_y,
+// vel_z, mass as FP32 arrays. Version 2 (precision-form state) has the same
+// header with version = 2 and the same seven arrays, followed by pos_lo_x,
+// pos_lo_y, pos_lo_z: a version-1 reader, which accepts only version 1, rejects
+// it with "Unsupported file version" instead of misreading it.
 #pragma once
 
 #include <cstdint>
@@ -16,6 +19,7 @@ namespace nbody {
 
 constexpr uint32_t NBODY_MAGIC = 0x4E424F44;
 constexpr uint32_t NBODY_VERSION = 1;
+constexpr uint32_t NBODY_VERSION_DOUBLE_SINGLE = 2;
 constexpr uint64_t MAX_PARTICLE_COUNT = 100'000'000;
 
 struct FileHeader {
@@ -37,6 +41,7 @@ class Serializer {
   static SimulationState load(const std::string& filename);
   static void save(std::ostream& out, const SimulationState& state);
   static SimulationState load(std::istream& in);
+  // True for a readable header of version 1 or 2.
   static bool validateFile(const std::string& filename);
   static bool validateStream(std::istream& in);
 

@@ -13,6 +13,9 @@ struct SimulationState {
   std::vector<float> pos_x, pos_y, pos_z;
   std::vector<float> vel_x, vel_y, vel_z;
   std::vector<float> mass;
+  // Precision form only: low parts of double-single positions (x = pos_x + pos_lo_x).
+  // Empty for plain FP32 state; non-empty state is written as .nbody version 2.
+  std::vector<float> pos_lo_x, pos_lo_y, pos_lo_z;
   size_t particle_count = 0;
   float simulation_time = 0.0f;
   float dt = 0.0f;
@@ -22,7 +25,8 @@ struct SimulationState {
 
   void serialize(std::ostream& out) const;
   static SimulationState deserialize(std::istream& in);
-  // Element-wise within 1e-6 (the reference's operator==).
+  bool hasLowParts() const noexcept { return !pos_lo_x.empty(); }
+  // Element-wise within 1e-6 (the reference's operator==); low parts must match exactly.
   bool operator==(const SimulationState& other) const;
 };
 

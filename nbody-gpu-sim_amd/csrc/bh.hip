@@ -81,6 +81,7 @@ struct BhState {
   int32_t depth_merges;
   int32_t overflow;  // a run too long to order in one workgroup
   int32_t sort_n;    // cells to number (0 when the sequential builder runs)
+  int32_t truncated; // sequential builder: bodies left out when the node array ran out
 };
 
 // ---------------------------------------------------------------- helpers
@@ -165,6 +166,7 @@ __global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int
   st->multi_count = 0;
   st->run_count = 0;
   st->overflow = 0;
+  st->truncated = 0;
 }
 
 // ---------------------------------------------------------------- 2: morton
@@ -1272,7 +1274,10 @@ __global__ void seq_build(BhState* st, const int32_t* __restrict__ sidx, const f
       cur = ch;
       ++depth;
     }
-    if (overflow) break;  // the reference would write out of bounds here
+    if (overflow) {  // the reference writes past its node array here
+      st->truncated = n - r;
+      break;
+    }
     if (!merged && depth >= kMaxDepth) {
       int leaf = cur;
       for (int guard = 0; guard <= kMaxDepth && !nodes[leaf].leaf; ++guard) {
@@ -1984,6 +1989,20 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   return NBX_OK;
 }
 
+// The reference's insertion (force_barnes_hut.cu:368-389) creates a leaf in an
+// empty child without checking its node budget; inputs exist (deep pairs whose
+// chains use up the budget, then bodies in the chains' empty octants) where it
+// writes past its 2 * max_particles nodes. The sequential builder stops there
+// instead, and the synchronous queries report it.
+static int check_truncated(const BhState& h) {
+  if (h.truncated)
+    return fail(NBX_ERR_CAPACITY,
+                "Barnes-Hut node budget exhausted: %d of %d bodies left out of the tree "
+                "(the reference's insertion writes past its node array for this input)",
+                h.truncated, h.n);
+  return NBX_OK;
+}
+
 int nbx_bh_node_count(nbx_bh* t, int32_t* node_count, int32_t* max_depth) {
   if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_node_count: null tree");
   if (!t->built) {
@@ -1994,6 +2013,7 @@ int nbx_bh_node_count(nbx_bh* t, int32_t* node_count, int32_t* max_depth) {
   BhState h;
   NBX_HIP(hipDeviceSynchronize());
   NBX_HIP(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
+  NBX_TRY(check_truncated(h));
   if (node_count) *node_count = h.final_count;
   if (max_depth) *max_depth = h.max_depth;
   return NBX_OK;
@@ -2048,6 +2068,7 @@ int nbx_bh_build_info(nbx_bh* t, int32_t* builder, int32_t* depth32_leaves) {
   if (t->built) {
     NBX_HIP(hipDeviceSynchronize());
     NBX_HIP(hipMemcpy(&h, t->st, sizeof(h), hipMemcpyDeviceToHost));
+    NBX_TRY(check_truncated(h));
   }
   if (builder) *builder = h.use_seq;
   if (depth32_leaves) *depth32_leaves = h.use_seq ? h.depth_merges : h.multi_count;

@@ -17,6 +17,7 @@ BarnesHutTree::~BarnesHutTree() { nbx_bh_destroy(handle_); }
 void BarnesHutTree::build(const ParticleData* d_particles) {
   NBODY_PROFILE_SCOPE("barnes_hut.build");
   NBODY_CHECK(nbx_bh_build(handle_, reinterpret_cast<const nbx_particles*>(d_particles), nullptr));
+  nodes_stale_ = true;
 }
 
 void BarnesHutTree::computeForces(ParticleData* d_particles, float theta, float G, float eps) {
@@ -36,15 +37,24 @@ int BarnesHutTree::getMaxDepth() const {
   return d;
 }
 
-void BarnesHutTree::copyNodesToHost() {
+void BarnesHutTree::refreshNodes() const {
   const int c = getNodeCount();
   h_nodes_.assign((size_t)c, OctreeNode{});
   if (c) NBODY_CHECK(nbx_bh_copy_nodes(handle_, h_nodes_.data(), h_nodes_.size()));
+  nodes_stale_ = false;
+}
+
+void BarnesHutTree::copyNodesToHost() { refreshNodes(); }
+
+const OctreeNode* BarnesHutTree::getNodes() const {
+  if (nodes_stale_) refreshNodes();
+  return h_nodes_.data();
 }
 
 bool BarnesHutTree::verifyMassConservation(const ParticleData* h_particles) const {
   float total = 0.0f;
   for (size_t i = 0; i < h_particles->count; i++) total += h_particles->mass[i];
+  if (nodes_stale_) refreshNodes();
   if (h_nodes_.empty()) return false;
   return std::abs(total - h_nodes_[0].total_mass) < 0.001f * total;
 }

@@ -51,9 +51,9 @@ void validateSimulationConfig(const SimulationConfig& config) {
       throw ValidationException("Spatial hash cutoff must not exceed cell size");
   }
   if (config.cuda_block_size <= 0 || config.cuda_block_size > 1024)
-    throw ValidationException("Block size must be between 1 and 1024");
+    throw ValidationException("CUDA block size must be between 1 and 1024");
   if ((config.cuda_block_size & (config.cuda_block_size - 1)) != 0)
-    throw ValidationException("Block size must be a power of two");
+    throw ValidationException("CUDA block size must be a power of two");
 }
 
 void validateParticleCountRange(size_t count) {

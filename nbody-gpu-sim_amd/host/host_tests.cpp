@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <functional>
 #include <iostream>
 #include <sstream>
@@ -161,7 +162,10 @@ TEST_CPU(serialization_rejects_corrupt_input) {
   bad[0] ^= 0x5a;
   CHECK_THROWS(load(bad), ValidationException);
   bad = g;
-  bad[4] = 2;
+  bad[4] = 3;  // unknown version
+  CHECK_THROWS(load(bad), ValidationException);
+  bad = g;
+  bad[4] = 2;  // a v1 body under a v2 header: the low parts are missing
   CHECK_THROWS(load(bad), ValidationException);
   CHECK_THROWS(load(g.substr(0, 40)), ValidationException);
   CHECK_THROWS(load(g.substr(0, g.size() - 1)), ValidationException);
@@ -177,6 +181,47 @@ TEST_CPU(serialization_rejects_corrupt_input) {
   CHECK(Serializer::validateStream(ok));
   CHECK(!Serializer::validateStream(trunc));
   CHECK_THROWS(Serializer::load(std::string("/nonexistent/dir/file.nbody")), IOException);
+}
+
+// Version 2 (precision-form state): the v1 header and arrays, then the three
+// low-part arrays. A reader of version 1 only (the reference's, which checks
+// version == 1 right after the magic) sees version 2 and rejects the file.
+TEST_CPU(serialization_version2_low_parts) {
+  for (size_t n : {1, 10, 1000}) {
+    SimulationState s = make_state(n, 11 + (unsigned)n);
+    for (auto* v : {&s.pos_lo_x, &s.pos_lo_y, &s.pos_lo_z}) {
+      v->resize(n);
+      for (size_t i = 0; i < n; ++i) (*v)[i] = 1e-9f * (float)(i % 13) - 3e-9f;
+    }
+    std::stringstream buf;
+    Serializer::save(buf, s);
+    const std::string bytes = buf.str();
+    CHECK(bytes.size() == 56 + 10 * 4 * n);
+    uint32_t version;
+    std::memcpy(&version, bytes.data() + 4, 4);
+    CHECK(version == NBODY_VERSION_DOUBLE_SINGLE && version != NBODY_VERSION);
+    // the first seven arrays sit where version 1 puts them
+    SimulationState plain = s;
+    plain.pos_lo_x.clear();
+    plain.pos_lo_y.clear();
+    plain.pos_lo_z.clear();
+    std::stringstream v1;
+    Serializer::save(v1, plain);
+    CHECK(v1.str().size() == 56 + 7 * 4 * n);
+    CHECK(std::memcmp(v1.str().data() + 8, bytes.data() + 8, v1.str().size() - 8) == 0);
+    std::stringstream in(bytes);
+    const SimulationState r = Serializer::load(in);
+    CHECK(r == s && r.hasLowParts());
+    CHECK(std::memcmp(r.pos_lo_y.data(), s.pos_lo_y.data(), 4 * n) == 0);
+    std::stringstream check(bytes);
+    CHECK(Serializer::validateStream(check));
+  }
+  SimulationState bad = make_state(5, 3);
+  bad.pos_lo_x.assign(4, 0.0f);
+  bad.pos_lo_y.assign(5, 0.0f);
+  bad.pos_lo_z.assign(5, 0.0f);
+  std::stringstream out;
+  CHECK_THROWS(Serializer::save(out, bad), ValidationException);
 }
 
 TEST_CPU(cli_parsing) {
@@ -457,6 +502,14 @@ TEST_GPU(barnes_hut_tree_kats) {
   BarnesHutTree t(50);
   t.build(&s.d);
   CHECK(t.getNodeCount() > 0);
+  // getNodes() holds the nodes of the last build without an explicit copy
+  float root_mass = t.getNodes()[0].total_mass;
+  CHECK(std::fabs(root_mass - 50.0f) < 1e-3f);
+  CHECK(t.verifyMassConservation(&s.h));
+  for (size_t i = 0; i < 50; ++i) s.h.mass[i] = 2.0f;
+  s.up();
+  t.build(&s.d);
+  CHECK(std::fabs(t.getNodes()[0].total_mass - 100.0f) < 1e-3f);
   t.copyNodesToHost();
   CHECK(t.verifyMassConservation(&s.h));
   for (int mode = 0; mode < 2; ++mode) {  // coincident, collinear
@@ -638,6 +691,66 @@ TEST_GPU(checkpoint_exact_resume) {
     CHECK(std::fabs(sa.simulation_time - sb.simulation_time) < 1e-6f);
     std::remove(path.c_str());
   }
+}
+
+// The precision form behind ParticleSystem: a double-single run checkpointed as
+// .nbody version 2 resumes bit for bit, low parts included.
+TEST_GPU(precision_checkpoint_exact_resume) {
+  ParticleSystem a;
+  SimulationConfig c;
+  c.particle_count = 3000;
+  c.softening = 0.05f;
+  a.setPrecisionMode(true);
+  a.initialize(c);
+  CHECK(a.getPrecisionMode());
+  for (int k = 0; k < 5; ++k) a.update(0.001f);
+  const std::string path = "/tmp/nbx_resume_ds.nbody";
+  a.saveState(path);
+  {
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    CHECK((size_t)f.tellg() == 56 + 10 * 4 * c.particle_count);
+  }
+  for (int k = 0; k < 7; ++k) a.update(0.001f);
+  ParticleSystem b;
+  b.initialize(c);
+  CHECK(!b.getPrecisionMode());
+  b.loadState(path);
+  CHECK(b.getPrecisionMode());
+  for (int k = 0; k < 7; ++k) b.update(0.001f);
+  const SimulationState sa = a.getState(), sb = b.getState();
+  CHECK(sa.hasLowParts() && sb.hasLowParts());
+  bool any_low = false;
+  for (float v : sa.pos_lo_x) any_low = any_low || v != 0.0f;
+  CHECK(any_low);
+  const size_t bytes = 4 * sa.particle_count;
+  CHECK(std::memcmp(sa.pos_x.data(), sb.pos_x.data(), bytes) == 0);
+  CHECK(std::memcmp(sa.pos_lo_x.data(), sb.pos_lo_x.data(), bytes) == 0);
+  CHECK(std::memcmp(sa.pos_lo_z.data(), sb.pos_lo_z.data(), bytes) == 0);
+  CHECK(std::memcmp(sa.vel_y.data(), sb.vel_y.data(), bytes) == 0);
+  std::remove(path.c_str());
+}
+
+TEST_GPU(precision_mode_rules) {
+  ParticleSystem ps;
+  SimulationConfig c;
+  c.particle_count = 500;
+  ps.initialize(c);
+  const float e_plain = ps.computeTotalEnergy();
+  ps.setPrecisionMode(true);
+  CHECK(std::fabs(ps.computeTotalEnergy() - e_plain) <= 1e-5f * std::fabs(e_plain));
+  CHECK_THROWS(ps.setForceMethod(ForceMethod::BARNES_HUT), ValidationException);
+  for (int k = 0; k < 3; ++k) ps.update(0.001f);
+  const SimulationState ds = ps.getState();
+  ps.setPrecisionMode(false);  // hi + lo rounded to FP32
+  const SimulationState plain = ps.getState();
+  CHECK(!plain.hasLowParts());
+  for (size_t i = 0; i < 500; ++i)
+    CHECK(plain.pos_y[i] == (float)((double)ds.pos_y[i] + (double)ds.pos_lo_y[i]));
+  ps.setForceMethod(ForceMethod::BARNES_HUT);
+  CHECK_THROWS(ps.setPrecisionMode(true), ValidationException);
+  SimulationState bh_ds = ds;
+  bh_ds.force_method = ForceMethod::BARNES_HUT;
+  CHECK_THROWS(ps.setState(bh_ds), ValidationException);
 }
 
 }  // namespace

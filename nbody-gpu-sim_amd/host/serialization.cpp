@@ -1,5 +1,5 @@
-// .nbody v1 reader/writer (reference: src/utils/serialization.cpp:9-145,
-// src/utils/simulation_state.cpp:7-60).
+// .nbody reader/writer: version 1 as the reference (src/utils/serialization.cpp:9-145,
+// src/utils/simulation_state.cpp:7-60), version 2 for precision-form state.
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -24,14 +24,16 @@ SimulationState Serializer::load(const std::string& filename) {
 
 void Serializer::save(std::ostream& out, const SimulationState& state) {
   NBODY_PROFILE_SCOPE("serialization.save");
+  if (state.hasLowParts() &&
+      (state.pos_lo_x.size() != state.particle_count || state.pos_lo_y.size() != state.particle_count ||
+       state.pos_lo_z.size() != state.particle_count))
+    throw ValidationException("SimulationState low-part vectors do not match particle_count");
   writeHeader(out, state);
-  writeFloatArray(out, state.pos_x);
-  writeFloatArray(out, state.pos_y);
-  writeFloatArray(out, state.pos_z);
-  writeFloatArray(out, state.vel_x);
-  writeFloatArray(out, state.vel_y);
-  writeFloatArray(out, state.vel_z);
-  writeFloatArray(out, state.mass);
+  for (const auto* a : {&state.pos_x, &state.pos_y, &state.pos_z, &state.vel_x, &state.vel_y,
+                        &state.vel_z, &state.mass})
+    writeFloatArray(out, *a);
+  if (state.hasLowParts())
+    for (const auto* a : {&state.pos_lo_x, &state.pos_lo_y, &state.pos_lo_z}) writeFloatArray(out, *a);
 }
 
 SimulationState Serializer::load(std::istream& in) {
@@ -58,6 +60,11 @@ SimulationState Serializer::load(std::istream& in) {
   st.vel_y = readFloatArray(in, st.particle_count);
   st.vel_z = readFloatArray(in, st.particle_count);
   st.mass = readFloatArray(in, st.particle_count);
+  if (header.version == NBODY_VERSION_DOUBLE_SINGLE) {
+    st.pos_lo_x = readFloatArray(in, st.particle_count);
+    st.pos_lo_y = readFloatArray(in, st.particle_count);
+    st.pos_lo_z = readFloatArray(in, st.particle_count);
+  }
   return st;
 }
 
@@ -70,7 +77,8 @@ bool Serializer::validateFile(const std::string& filename) {
 bool Serializer::validateStream(std::istream& in) {
   try {
     const FileHeader h = readHeader(in);
-    return h.magic == NBODY_MAGIC && h.version == NBODY_VERSION;
+    return h.magic == NBODY_MAGIC &&
+           (h.version == NBODY_VERSION || h.version == NBODY_VERSION_DOUBLE_SINGLE);
   } catch (const std::exception&) {
     return false;
   }
@@ -80,7 +88,7 @@ void Serializer::writeHeader(std::ostream& out, const SimulationState& state) {
   FileHeader h;
   std::memset(&h, 0, sizeof(h));  // deterministic bytes, padding included
   h.magic = NBODY_MAGIC;
-  h.version = NBODY_VERSION;
+  h.version = state.hasLowParts() ? NBODY_VERSION_DOUBLE_SINGLE : NBODY_VERSION;
   h.particle_count = state.particle_count;
   h.simulation_time = state.simulation_time;
   h.dt = state.dt;
@@ -96,7 +104,8 @@ FileHeader Serializer::readHeader(std::istream& in) {
   if (!in || in.gcount() != sizeof(h))
     throw ValidationException("Failed to read file header: file may be truncated or corrupted");
   if (h.magic != NBODY_MAGIC) throw ValidationException("Invalid file format: wrong magic number");
-  if (h.version != NBODY_VERSION) throw ValidationException("Unsupported file version");
+  if (h.version != NBODY_VERSION && h.version != NBODY_VERSION_DOUBLE_SINGLE)
+    throw ValidationException("Unsupported file version");
   return h;
 }
 
@@ -127,7 +136,7 @@ bool SimulationState::operator==(const SimulationState& o) const {
   for (int k = 0; k < 7; ++k)
     for (size_t i = 0; i < particle_count; ++i)
       if (!near((*a[k])[i], (*b[k])[i])) return false;
-  return true;
+  return pos_lo_x == o.pos_lo_x && pos_lo_y == o.pos_lo_y && pos_lo_z == o.pos_lo_z;
 }
 
 void SimulationState::serialize(std::ostream& out) const { Serializer::save(out, *this); }

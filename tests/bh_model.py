@@ -305,3 +305,34 @@ def shared_prefix_runs(pos, sorted_idx, bmin, bmax, digits):
     for p in paths(pos, sorted_idx, bmin, bmax):
         groups[p[:digits]] = groups.get(p[:digits], 0) + 1
     return sorted(groups.values(), reverse=True)
+
+
+def budget_overrun_input(pairs=5, levels=(4, 5, 6), per_level=6):
+    """An input on which the reference's insertion runs out of its 2N nodes.
+
+    Two anchors fix the root cube to [-1, 1]^3. Each pair sits at the low corner
+    of its own depth-3 cell, 2^-27 apart, so its insertion splits a chain of
+    cells down to depth ~27 (octant 0 at every level, so everything else in the
+    cell sorts after it in Morton order). Single bodies at the centres of the
+    chain cells' empty sibling octants (depths 4-6) then each create one leaf in
+    an empty child of an internal cell, where the insertion makes no budget check
+    (force_barnes_hut.cu:368-389 via :478-486). With 5 pairs and 6 siblings per
+    level (102 bodies, budget 204) the node count passes the budget.
+    """
+    pts = [(-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)]
+    c0, h0 = root_cell([-1, -1, -1], [1, 1, 1])
+    body, singles = [], []
+    for k in range(pairs):
+        qc, qh = c0, h0
+        for o in (7 - k, k, 3):
+            qc, qh = child_center(qc, qh, o)
+        corner = [float(qc[a] - qh) for a in range(3)]
+        c = [v + 2.0 ** -27 for v in corner]
+        body += [tuple(c), (c[0] + 1.5 * 2.0 ** -27, c[1] + 2.0 ** -27, c[2] + 0.5 * 2.0 ** -27)]
+        cc, hh = qc, qh
+        for d in range(4, max(levels) + 1):
+            if d in levels:
+                for s in range(1, 1 + per_level):
+                    singles.append(tuple(child_center(cc, hh, s)[0]))
+            cc, hh = child_center(cc, hh, 0)
+    return pts + body + singles

@@ -318,3 +318,19 @@ def test_two_plummer_1m_exact_and_forces(nbx, orc):
     print(f"1M two-Plummer: nodes {ref['node_count']}, depth {ref['max_depth']}, "
           f"force rel err median {np.median(err):.2e} max {err.max():.2e}")
     assert np.median(err) < 1e-6 and err.max() < 1e-4
+
+
+def test_node_budget_exhausted_is_reported(nbx, orc):
+    # the input on which the reference's insertion writes past its 2N nodes
+    # (tests/bh_model.py budget_overrun_input): the sequential builder stops at the
+    # budget and every synchronous query reports NBX_ERR_CAPACITY
+    from test_bh_oracle import overrun_host
+    h = overrun_host(nbx)
+    n = len(h["mass"])
+    d, t = gpu_build(nbx, h)
+    for query in (t.node_count, t.build_info, t.nodes):
+        with pytest.raises(nbx.NbxError) as e:
+            query()
+        assert e.value.status == 4 and "node budget" in str(e.value)
+    # with room for four times the bodies the same input builds bit-exact
+    check_exact(nbx, orc, h, capacity=4 * n)

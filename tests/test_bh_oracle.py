@@ -239,3 +239,32 @@ def test_approximation_convergence_kat(orc, nbx):
 
     assert err(0.3) <= 1.1 * err(0.8)
     assert err(0.3) < err(0.8)
+
+
+def overrun_host(nbx):
+    pts = np.array(M.budget_overrun_input(), np.float32)
+    h = nbx.empty_host(len(pts))
+    h["pos_x"][:], h["pos_y"][:], h["pos_z"][:] = pts[:, 0], pts[:, 1], pts[:, 2]
+    h["mass"][:] = np.float32(1.0 / len(pts))
+    return h
+
+
+def test_node_budget_overrun_detected(orc, nbx):
+    # The reference's insertion makes no budget check when it creates a leaf in an
+    # empty child; on this input it needs more than its 2N nodes. The independent
+    # Python restatement (no bound on its node list) shows the count past 2N, and
+    # the oracle refuses the build instead of writing past its array.
+    h = overrun_host(nbx)
+    n = len(h["mass"])
+    x, y, z = h["pos_x"], h["pos_y"], h["pos_z"]
+    lo, hi = orc.bbox(x, y, z)
+    codes, idx = orc.morton_codes(x, y, z, lo, hi)
+    _, sidx = orc.stable_sort(codes, idx)
+    pos = np.stack([x, y, z], 1)
+    nodes, _ = M.insert_build(pos, h["mass"], sidx, lo, hi, 2 * n)
+    assert len(nodes) > 2 * n
+    with pytest.raises(RuntimeError, match="overrun"):
+        orc.bh_pipeline(h)
+    # with room for every node (capacity for more bodies) the same input builds
+    t = orc.bh_pipeline(h, max_particles=4 * n)
+    assert t["node_count"] == len(M.insert_build(pos, h["mass"], sidx, lo, hi, 8 * n)[0])
