@@ -271,6 +271,9 @@ def run_cpu_path(nbx, args):
 
 # algorithmic bytes per particle of the HBM-bound build phases
 BH_BYTES = {"bbox": 12.0, "morton": 20.0, "paths": 4.0 + 12.0 + 16.0}
+# bytes of one traversal record (centre of mass and G m, centre and half size,
+# skip and thresholds, mass); the walk visits one per step
+BH_RECORD_BYTES = 64
 
 
 def run_bh(nbx, args):
@@ -306,13 +309,46 @@ def run_bh(nbx, args):
     t.set_timing(False)
     nodes, depth = t.node_count()
     gbs = {k: BH_BYTES[k] * n / (phases[k] * 1e-3) / 1e9 for k in BH_BYTES if phases[k] > 0}
+    # the traversal alone on the current tree: HIP events around R launches on s,
+    # and the walk's own counts (nbx_bh_walk_census: the same tasks and decisions)
+    R = 5
+    f0, f1 = nbx.Event(), nbx.Event()
+    f0.record(s)
+    for _ in range(R):
+        t.forces(d, theta, G, eps, s)
+    f1.record(s)
+    s.synchronize()
+    trav_ms = f0.elapsed_ms(f1) / R
+    census = t.walk_census(d, theta, G, eps)
+    rec_bytes = census["wave_steps"] * BH_RECORD_BYTES
+    flops = census["interactions"] * FLOP_PER_INTERACTION
+    trav_gbs = rec_bytes / (trav_ms * 1e-3) / 1e9
+    trav_tf = flops / (trav_ms * 1e-3) / 1e12
+    waves = -(-n // 64)
+    roofline = {
+        "kernel": "bh_traverse", "bound": "hbm", "achieved": trav_gbs, "peak": PEAK_HBM_GBS,
+        "unit": "GB/s", "frac": trav_gbs / PEAK_HBM_GBS, "traffic": None,
+        "launch_ms_mean": trav_ms,
+        "bytes_per_launch": rec_bytes,
+        "bytes_rule": f"{BH_RECORD_BYTES} B per record a wave's walk visits (one per step)",
+        "fp32": {"achieved_TFLOPs": trav_tf, "peak_TFLOPs": PEAK_FP32_TFLOPS,
+                 "frac": trav_tf / PEAK_FP32_TFLOPS,
+                 "rule": f"{FLOP_PER_INTERACTION:.0f} flop per particle-node force evaluation"},
+        "census": census, "waves": waves,
+        "steps_per_wave": census["wave_steps"] / waves,
+        "lane_utilisation": census["active_lane_steps"] / (64.0 * census["wave_steps"]),
+        "note": "neither roof binds: each step needs the record the previous step chose, so "
+                "the walk is bound by the latency of dependent record loads"}
+    log(f"bh_traverse: {trav_ms:.3f} ms, {census['wave_steps']} wave-steps "
+        f"({census['wave_steps'] / waves:.0f} per wave), {census['interactions']} interactions; "
+        f"{trav_gbs:.0f} GB/s of records, {trav_tf:.2f} TFLOP/s")
     log(f"barnes-hut N={n}: {ms:.3f} ms/step, phases {phases}, nodes {nodes}, depth {depth}")
     out = {"n": n, "theta": theta, "eps": eps, "dt": dt, "steps": K, "ms_per_step": ms,
            "workload": "two Plummer spheres (a=1, M=1 each, truncated at 99.9% mass) at x=+-5, "
                        "relative velocity 0.5 along x",
            "phases_ms": phases, "nodes": nodes, "max_depth": depth,
            "builder": t.build_info()["builder"],
-           "achieved_GBps": gbs, "hbm_peak_GBps": PEAK_HBM_GBS}
+           "achieved_GBps": gbs, "hbm_peak_GBps": PEAK_HBM_GBS, "roofline": roofline}
     if not args.no_cpu_baseline:
         orc = oracle()
         hn = d.download()

@@ -205,6 +205,13 @@ int nbx_bh_set_timing(nbx_bh* t, int enable);
 /* Builder selection for cross-checks: 0 = automatic (parallel unless the node
  * budget could fire), 1 = always the exact sequential builder. */
 int nbx_bh_set_builder(nbx_bh* t, int mode);
+/* Diagnostics (no reference counterpart): the walk nbx_bh_forces makes for
+ * (theta, G, eps) on the last build, counted instead of summed. counts[0] the
+ * records the waves visit (one per step of a 64-particle wave's walk),
+ * [1] particle-node force evaluations, [2] particle-node openings, [3] active
+ * lane-steps. The same tasks and decisions as the force walk; synchronous. */
+int nbx_bh_walk_census(nbx_bh* t, const nbx_particles* d, float theta, float G, float eps,
+                       int64_t counts[4]);
 /* phases: bbox, morton, sort, paths, tree, com, export, forces (8 values, ms). */
 int nbx_bh_phase_ms(nbx_bh* t, double ms[8]);
 

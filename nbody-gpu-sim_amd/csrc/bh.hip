@@ -1520,6 +1520,73 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_waitcnt lgkmcnt(0)\n"                                                                  \
   "s_mov_b64 %[M], s[56:57]\n"
 
+// The decisions of one walk step at record v for the wave's lanes M (the C
+// form of the asm's common step, covering every case): the lanes that open the
+// node (om, descend), the lanes that take its force (im), the force operands
+// and the position after the node's subtree.
+struct WalkStep {
+  uint64_t om, im;
+  float dx, dy, dz, d2, gm;
+  int after;
+};
+
+__device__ __forceinline__ WalkStep walk_decide(const float4* __restrict__ rec, int v, uint64_t M,
+                                                float xi, float yi, float zi, float mi,
+                                                uint64_t me, float theta2, float G, float eps2) {
+  const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 64u;
+  const float4 c4 = *reinterpret_cast<const float4*>(rp);
+  const float4 b4 = *reinterpret_cast<const float4*>(rp + 16);
+  const float4 a4 = *reinterpret_cast<const float4*>(rp + 32);
+  const float mass = reinterpret_cast<const float4*>(rp + 48)->x;
+  const int after = max(__float_as_int(a4.x), v + 1);
+  const float size2 = a4.y, s_open = a4.z, s_acc = a4.w;
+  const float h = fabsf(b4.w);
+  const float far = fmaxf(fmaxf(fabsf(xi - b4.x), fabsf(yi - b4.y)), fabsf(zi - b4.z));
+  const uint64_t cm = __ballot(far <= h) & M;  // :180-182, particle inside the cell
+  float dx = sub_rn(c4.x, xi), dy = sub_rn(c4.y, yi), dz = sub_rn(c4.z, zi);
+  float gm = c4.w;
+  uint64_t o = 0;     // lanes that open the node
+  uint64_t gone = 0;  // lanes whose own leaf holds no other mass
+  if (cm) {
+    if (__builtin_signbit(b4.w)) {  // :190-200, the particle's own leaf
+      const bool own = (cm & me) != 0;
+      const float meff = own ? sub_rn(mass, mi) : mass;
+      const bool fix = own && meff > 0.0f;
+      if (__ballot(fix)) {  // remove the particle from a leaf that holds others
+        const float cx = div_rn(sub_rn(mul_rn(c4.x, mass), mul_rn(xi, mi)), meff);
+        const float cy = div_rn(sub_rn(mul_rn(c4.y, mass), mul_rn(yi, mi)), meff);
+        const float cz = div_rn(sub_rn(mul_rn(c4.z, mass), mul_rn(zi, mi)), meff);
+        dx = fix ? sub_rn(cx, xi) : dx;
+        dy = fix ? sub_rn(cy, yi) : dy;
+        dz = fix ? sub_rn(cz, zi) : dz;
+      }
+      gm = own ? mul_rn(G, meff) : gm;
+      gone = __ballot(meff <= 0.0f) & cm;
+    } else {
+      o = cm;  // :222, an internal node holding the particle is always opened
+    }
+  }
+  const float d2 = add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
+  // :218-224, open when contains || !(size2 / d2 < theta2): the thresholds
+  // decide outside a narrow band (trav_prep); inside it the IEEE division.
+  o |= __ballot(d2 < s_open);
+  const uint64_t am = M & ~(o | __ballot(d2 > s_acc));
+  if (am) {
+    const bool mine = (am & me) != 0;
+    o |= __ballot(mine && !(div_rn(size2, d2) < theta2)) & am;
+  }
+  WalkStep w;
+  w.om = o & M;
+  w.im = M & ~o & ~gone;  // lanes that take the node's force
+  w.dx = dx;
+  w.dy = dy;
+  w.dz = dz;
+  w.d2 = d2;
+  w.gm = gm;
+  w.after = after;
+  return w;
+}
+
 // One task: the 64 particles [64 blk, 64 blk + 64) of the order.
 __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __restrict__ rec,
                                           const BhState* __restrict__ st,
@@ -1570,50 +1637,10 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)M);
     if (v >= nn) break;
     // A step the asm leaves to C: record v (v != top here).
-    const char* rp = reinterpret_cast<const char*>(rec) + (uint32_t)v * 64u;
-    const float4 c4 = *reinterpret_cast<const float4*>(rp);
-    const float4 b4 = *reinterpret_cast<const float4*>(rp + 16);
-    const float4 a4 = *reinterpret_cast<const float4*>(rp + 32);
-    const float mass = reinterpret_cast<const float4*>(rp + 48)->x;
-    const int after = max(__float_as_int(a4.x), v + 1);
-    const float size2 = a4.y, s_open = a4.z, s_acc = a4.w;
-    const float h = fabsf(b4.w);
-    const float far = fmaxf(fmaxf(fabsf(xi - b4.x), fabsf(yi - b4.y)), fabsf(zi - b4.z));
-    const uint64_t cm = __ballot(far <= h) & M;  // :180-182, particle inside the cell
-    float dx = sub_rn(c4.x, xi), dy = sub_rn(c4.y, yi), dz = sub_rn(c4.z, zi);
-    float gm = c4.w;
-    uint64_t o = 0;     // lanes that open the node
-    uint64_t gone = 0;  // lanes whose own leaf holds no other mass
-    if (cm) {
-      if (__builtin_signbit(b4.w)) {  // :190-200, the particle's own leaf
-        const bool own = (cm & me) != 0;
-        const float meff = own ? sub_rn(mass, mi) : mass;
-        const bool fix = own && meff > 0.0f;
-        if (__ballot(fix)) {  // remove the particle from a leaf that holds others
-          const float cx = div_rn(sub_rn(mul_rn(c4.x, mass), mul_rn(xi, mi)), meff);
-          const float cy = div_rn(sub_rn(mul_rn(c4.y, mass), mul_rn(yi, mi)), meff);
-          const float cz = div_rn(sub_rn(mul_rn(c4.z, mass), mul_rn(zi, mi)), meff);
-          dx = fix ? sub_rn(cx, xi) : dx;
-          dy = fix ? sub_rn(cy, yi) : dy;
-          dz = fix ? sub_rn(cz, zi) : dz;
-        }
-        gm = own ? mul_rn(G, meff) : gm;
-        gone = __ballot(meff <= 0.0f) & cm;
-      } else {
-        o = cm;  // :222, an internal node holding the particle is always opened
-      }
-    }
-    const float d2 = add_rn(add_rn(add_rn(mul_rn(dx, dx), mul_rn(dy, dy)), mul_rn(dz, dz)), eps2);
-    // :218-224, open when contains || !(size2 / d2 < theta2): the thresholds
-    // decide outside a narrow band (trav_prep); inside it the IEEE division.
-    o |= __ballot(d2 < s_open);
-    const uint64_t am = M & ~(o | __ballot(d2 > s_acc));
-    if (am) {
-      const bool mine = (am & me) != 0;
-      o |= __ballot(mine && !(div_rn(size2, d2) < theta2)) & am;
-    }
-    const uint64_t om = o & M;
-    const uint64_t im = M & ~o & ~gone;  // lanes that take the node's force
+    const WalkStep w = walk_decide(rec, v, M, xi, yi, zi, mi, me, theta2, G, eps2);
+    const int after = w.after;
+    const uint64_t om = w.om, im = w.im;
+    const float dx = w.dx, dy = w.dy, dz = w.dz, d2 = w.d2, gm = w.gm;
     const float inv = __builtin_amdgcn_rsqf(d2);
     const float fi = gm * (inv * inv * inv);
     const float f = ((im >> lane) & 1) ? fi : 0.0f;
@@ -1710,6 +1737,66 @@ __global__ void trav_queue_reset(int* queue) {
   if (threadIdx.x < 8) queue[16 * threadIdx.x] = 0;
 }
 
+// Diagnostics: the walk of bh_traverse in C (pops included) over the same tasks
+// with the same decisions (walk_decide), counted instead of summed: records the
+// waves visit (one per step), particle-node force evaluations, openings and
+// active lane-steps, added to counts[0..3]. One wave per workgroup; the stack
+// of remembered (position, mask) entries sits in LDS, written by every lane
+// with the same values.
+__global__ __launch_bounds__(64) void bh_walk_census(
+    const float4* __restrict__ rec, const BhState* __restrict__ st,
+    const int32_t* __restrict__ morton_order, const int32_t* __restrict__ path_order, int ntasks,
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    const float* __restrict__ m, int n, float theta2, float G, float eps2,
+    unsigned long long* counts) {
+  __shared__ int at[kMaxDepth + 8];
+  __shared__ unsigned long long mask[kMaxDepth + 8];
+  const int lane = threadIdx.x;
+  const int32_t* order = st->use_seq ? morton_order : path_order;
+  const int nn = st->final_count;
+  const uint64_t me = 1ull << lane;
+  unsigned long long steps = 0, forces = 0, opens = 0, active = 0;
+  for (int blk = blockIdx.x; blk < ntasks; blk += gridDim.x) {
+    const int r = blk * 64 + lane;
+    const bool valid = r < n;
+    const int ip = valid ? order[r] : 0;
+    const float xi = x[ip], yi = y[ip], zi = z[ip], mi = m[ip];
+    uint64_t M = __ballot(valid);
+    int depth = 0, v = 0;
+    while (v < nn) {
+      if (depth > 0 && v == at[depth - 1]) {
+        --depth;
+        M = mask[depth];
+        continue;
+      }
+      const WalkStep w = walk_decide(rec, v, M, xi, yi, zi, mi, me, theta2, G, eps2);
+      ++steps;
+      forces += __popcll(w.im);
+      opens += __popcll(w.om);
+      active += __popcll(M);
+      if (w.om && depth < kMaxDepth + 8) {
+        at[depth] = w.after;
+        mask[depth] = M;
+        ++depth;
+        M = w.om;
+        v = v + 1;
+      } else {
+        v = w.after;
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&counts[0], steps);
+    atomicAdd(&counts[1], forces);
+    atomicAdd(&counts[2], opens);
+    atomicAdd(&counts[3], active);
+  }
+}
+
+__global__ void census_clear(unsigned long long* counts) {
+  if (threadIdx.x < 4) counts[threadIdx.x] = 0;
+}
+
 }  // namespace
 
 }  // namespace nbx
@@ -1742,6 +1829,7 @@ struct nbx_bh {
   nbx::TravParams prep{0.0f, 0.0f, 0};
   bool nodes_pending = false;  // the reference node array is emitted on demand
   int* trav_queue = nullptr;     // 8 task counters, 64 bytes apart
+  unsigned long long* census = nullptr;  // nbx_bh_walk_census counters
   int trav_waves = 0;            // resident traversal waves on this device
   int force_seq = 0;
   bool timing = false;
@@ -1855,6 +1943,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->nodes, NN);
   A(&t->trav_rec, (NN + 1) * 4);  // + the pad record the walk's prefetch may read
   A(&t->trav_queue, 8 * 16);
+  A(&t->census, 4);
   if (rc == NBX_OK) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -1962,6 +2051,19 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   return NBX_OK;
 }
 
+// The per-call fields of the traversal records (thresholds for theta, G * m),
+// rewritten only when (theta, G) differ from the last call's.
+static int ensure_prep(nbx_bh* t, float theta2, float G, hipStream_t s) {
+  const nbx::TravParams want{1.0f / theta2, G, 1};
+  if (!t->prep.valid || std::memcmp(&want, &t->prep, sizeof(want)) != 0) {
+    hipLaunchKernelGGL(trav_prep, dim3(grid_for((size_t)t->max_nodes, 2048)), dim3(kThreads), 0,
+                       s, t->trav_rec, t->st, want.inv_theta2, want.G);
+    NBX_LAUNCHED();
+    t->prep = want;
+  }
+  return NBX_OK;
+}
+
 int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, nbx_stream stream) {
   if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_forces: null tree");
   NBX_TRY(check_particles(d, "nbx_bh_forces"));
@@ -1971,13 +2073,7 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   const int n = (int)d->count;
   mark(t, 8, s);
   const float theta2 = theta * theta;
-  const nbx::TravParams want{1.0f / theta2, G, 1};
-  if (!t->prep.valid || std::memcmp(&want, &t->prep, sizeof(want)) != 0) {
-    hipLaunchKernelGGL(trav_prep, dim3(grid_for((size_t)t->max_nodes, 2048)), dim3(kThreads), 0,
-                       s, t->trav_rec, t->st, want.inv_theta2, want.G);
-    NBX_LAUNCHED();
-    t->prep = want;
-  }
+  NBX_TRY(ensure_prep(t, theta2, G, s));
   const int ntasks = (int)div_up(n, 64);
   const int waves = std::max(8, std::min(ntasks, t->trav_waves) & ~7);
   hipLaunchKernelGGL(trav_queue_reset, dim3(1), dim3(64), 0, s, t->trav_queue);
@@ -2072,6 +2168,26 @@ int nbx_bh_build_info(nbx_bh* t, int32_t* builder, int32_t* depth32_leaves) {
   }
   if (builder) *builder = h.use_seq;
   if (depth32_leaves) *depth32_leaves = h.use_seq ? h.depth_merges : h.multi_count;
+  return NBX_OK;
+}
+
+int nbx_bh_walk_census(nbx_bh* t, const nbx_particles* d, float theta, float G, float eps,
+                       int64_t counts[4]) {
+  if (!t || !counts) return fail(NBX_ERR_INVALID, "nbx_bh_walk_census: null argument");
+  NBX_TRY(check_particles(d, "nbx_bh_walk_census"));
+  if (!t->built || t->built_n != d->count)
+    return fail(NBX_ERR_STATE, "nbx_bh_walk_census: build the tree for these particles first");
+  const int n = (int)d->count;
+  NBX_TRY(ensure_prep(t, theta * theta, G, nullptr));
+  const int ntasks = (int)div_up(n, 64);
+  hipLaunchKernelGGL(census_clear, dim3(1), dim3(64), 0, nullptr, t->census);
+  hipLaunchKernelGGL(bh_walk_census, dim3(std::min(ntasks, 8192)), dim3(64), 0, nullptr,
+                     t->trav_rec, t->st, t->idx, t->O.ppart, ntasks, d->pos_x, d->pos_y,
+                     d->pos_z, d->mass, n, theta * theta, G, eps * eps, t->census);
+  NBX_LAUNCHED();
+  unsigned long long h[4];
+  NBX_HIP(hipMemcpy(h, t->census, sizeof(h), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 4; ++k) counts[k] = (int64_t)h[k];
   return NBX_OK;
 }
 

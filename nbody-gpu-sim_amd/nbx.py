@@ -117,6 +117,7 @@ _sig("nbx_bh_build_info", C.c_int, VP, I32P, I32P)
 _sig("nbx_bh_set_timing", C.c_int, VP, C.c_int)
 _sig("nbx_bh_set_builder", C.c_int, VP, C.c_int)
 _sig("nbx_bh_phase_ms", C.c_int, VP, F64P)
+_sig("nbx_bh_walk_census", C.c_int, VP, PP, C.c_float, C.c_float, C.c_float, I64P)
 _sig("nbx_grid_create", C.c_int, C.c_size_t, C.c_float, C.POINTER(VP))
 _sig("nbx_grid_destroy", C.c_int, VP)
 _sig("nbx_grid_build", C.c_int, VP, PP, VP)
@@ -165,6 +166,7 @@ EXPORTED = [
     "nbx_verlet_drift", "nbx_energy", "nbx_bh_create", "nbx_bh_destroy", "nbx_bh_build",
     "nbx_bh_forces", "nbx_bh_node_count", "nbx_bh_copy_nodes", "nbx_bh_copy_sort",
     "nbx_bh_build_info", "nbx_bh_set_timing", "nbx_bh_set_builder", "nbx_bh_phase_ms",
+    "nbx_bh_walk_census",
     "nbx_grid_create", "nbx_grid_destroy", "nbx_grid_build", "nbx_grid_forces", "nbx_grid_info",
     "nbx_grid_copy_cells", "nbx_grid_set_walk",
     "nbx_ds_alloc", "nbx_ds_free", "nbx_ds_forces", "nbx_ds_set_kernel", "nbx_ds_drift",
@@ -461,6 +463,12 @@ class BHTree:
 
     def set_builder(self, mode: int) -> None:
         check(lib.nbx_bh_set_builder(self.h, mode))
+
+    def walk_census(self, d: DeviceParticles, theta: float, G: float, eps: float) -> dict:
+        c = np.zeros(4, np.int64)
+        check(lib.nbx_bh_walk_census(self.h, d.ptr, theta, G, eps, c.ctypes.data_as(I64P)))
+        return {"wave_steps": int(c[0]), "interactions": int(c[1]), "openings": int(c[2]),
+                "active_lane_steps": int(c[3])}
 
     def phase_ms(self) -> dict:
         ms = (C.c_double * 8)()

@@ -334,3 +334,33 @@ def test_node_budget_exhausted_is_reported(nbx, orc):
         assert e.value.status == 4 and "node budget" in str(e.value)
     # with room for four times the bodies the same input builds bit-exact
     check_exact(nbx, orc, h, capacity=4 * n)
+
+
+def test_walk_census_closed_form_at_theta_zero(nbx, orc):
+    # theta = 0 opens every cell with mass, so each wave visits every record:
+    # wave steps = tasks * nodes, openings = n * internal cells, force evaluations
+    # = n * leaves - n (a body's own single-body leaf gives no force), active
+    # lane-steps = n * nodes. 3,001 bodies: a last wave with 57 lanes.
+    h = nbx.gen_plummer(3001, seed=5)
+    d, t = gpu_build(nbx, h)
+    nodes = t.nodes()
+    n, nn = 3001, len(nodes)
+    leaves = int(nodes["is_leaf"].sum())
+    assert np.all(nodes["particle_count"][nodes["is_leaf"] == 1] == 1)
+    c = t.walk_census(d, 0.0, 1.0, 0.01)
+    assert c["wave_steps"] == -(-n // 64) * nn
+    assert c["openings"] == n * (nn - leaves)
+    assert c["interactions"] == n * leaves - n
+    assert c["active_lane_steps"] == n * nn
+    # at theta = 0.5 the walk does less, and the census agrees with itself
+    c5 = t.walk_census(d, 0.5, 1.0, 0.01)
+    assert 0 < c5["wave_steps"] < c["wave_steps"] and c5["interactions"] < c["interactions"]
+    # each active lane at a step opens the node, takes its force or skips its own leaf
+    assert c5["active_lane_steps"] >= c5["interactions"] + c5["openings"]
+    assert c5["active_lane_steps"] - c5["interactions"] - c5["openings"] <= n * 40
+    # forces after a census are the walk's own (the census shares its prepared records)
+    t.forces(d, 0.5, 1.0, 0.01)
+    nbx.synchronize()
+    ref = orc.bh_forces(nodes, h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 0.5, 1.0, 0.01)
+    e = np.linalg.norm(d.acc().astype(np.float64) - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert np.median(e) < 1e-6
