@@ -79,7 +79,7 @@ struct BhState {
   int32_t max_depth;    // max_depth_ of the reference
   int32_t budget_merges;
   int32_t depth_merges;
-  int32_t overflow;  // a run too long to order in one workgroup
+  int32_t long_runs;  // runs longer than kRefineMax (ordered by path_refine_long)
   int32_t sort_n;    // cells to number (0 when the sequential builder runs)
   int32_t truncated; // sequential builder: bodies left out when the node array ran out
 };
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int
   st->deepest = 0;
   st->multi_count = 0;
   st->run_count = 0;
-  st->overflow = 0;
+  st->long_runs = 0;
   st->truncated = 0;
 }
 
@@ -303,9 +303,10 @@ __device__ __forceinline__ void write_trav(float4* rec, int p, int size, float4 
 //
 // The path sort is a stable LSD radix sort over digits 0..19 (two 30-bit
 // keys). Particles whose first 20 digits agree, rare outside exact or
-// near-coincident duplicates, form runs that one workgroup per run orders by
-// the remaining 12 digits and rank. A run longer than kRefineMax sends the
-// build to the exact sequential builder.
+// near-coincident duplicates and dense clusters, form runs that one workgroup
+// per run orders by the remaining 12 digits and rank: in LDS up to kRefineMax
+// positions (path_refine), by a stable radix sort through global scratch
+// beyond that (path_refine_long).
 
 // Cells ("tmp nodes"), indexed by preorder position.
 struct TmpNodes {
@@ -335,6 +336,9 @@ struct PathOrder {
   int32_t* bsum;      // per block of kPosBlock positions: cells, then their offsets
   int32_t* lvl;       // straddling cells by (hierarchy level, unit, depth), or -1
   int32_t* runs;      // starts of runs sharing 20 digits
+  int2* lrun;         // runs longer than kRefineMax: (start, length)
+  unsigned long long* lkey[2];  // path_refine_long: keys, ping-pong, by position
+  int32_t* lval[2];             // path_refine_long: ranks, ping-pong, by position
   int8_t* lmin1;      // min lcp per chunk of 64 positions
   int8_t* lmin2;      // min lcp per block of kPosBlock positions
   int8_t* lmin3;      // min lcp per kSuperBlocks blocks
@@ -447,8 +451,8 @@ __global__ __launch_bounds__(kThreads) void path_refine(BhState* st,
       __syncthreads();
     }
     const int m = sh_end - s;
-    if (m > kRefineMax) {
-      if (threadIdx.x == 0) st->overflow = 1;
+    if (m > kRefineMax) {  // left to path_refine_long
+      if (threadIdx.x == 0) O.lrun[atomicAdd(&st->long_runs, 1)] = make_int2(s, m);
       __syncthreads();
       continue;
     }
@@ -487,6 +491,104 @@ __global__ __launch_bounds__(kThreads) void path_refine(BhState* st,
     }
     __syncthreads();
     for (int i = 1 + (int)threadIdx.x; i < m; i += kThreads)
+      O.lcp[s + i] = (uint8_t)path_lcp(O.spath[s + i - 1], O.spath[s + i]);
+    __syncthreads();
+  }
+}
+
+// Digits 20..31 of a path as one 36-bit key, octants compared as 7 - o.
+__device__ __forceinline__ unsigned long long low_digits_key(uint4 p) {
+  const uint32_t lo = ((~p.w & 7u) << 3) | ((~p.w >> 3) & 7u);
+  return ((unsigned long long)order_key10(p.z) << 6) | lo;
+}
+
+// Runs longer than kRefineMax, one workgroup per run: a stable LSD radix sort of
+// the run by its remaining 12 digits (36 bits) in nine 4-bit counting passes
+// through global scratch, each pass ranking kLongThreads positions at a time in
+// LDS (a lane's rank among the earlier lanes of its digit from ballots, the
+// waves' counts in LDS). The path sort left every run in rank order, so
+// stability keeps equal paths in rank order: the order a full stable sort of
+// the 32-digit paths gives. Then the run's ranks, paths and lcp values are
+// rewritten as path_refine does. Costs one idle launch when no run is long.
+constexpr int kLongThreads = 1024;
+constexpr int kLongWaves = kLongThreads / 64;
+__global__ __launch_bounds__(kLongThreads) void path_refine_long(BhState* st,
+                                                                 const uint4* __restrict__ path,
+                                                                 PathOrder O) {
+  __shared__ int hist[16];
+  __shared__ int base[16];
+  __shared__ int wcnt[kLongWaves][16];
+  const int nl = st->long_runs;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t below_me = (1ull << lane) - 1;
+  for (int g = blockIdx.x; g < nl; g += gridDim.x) {
+    const int s = O.lrun[g].x, m = O.lrun[g].y;
+    unsigned long long* key[2] = {O.lkey[0] + s, O.lkey[1] + s};
+    int32_t* val[2] = {O.lval[0] + s, O.lval[1] + s};
+    for (int i = threadIdx.x; i < m; i += kLongThreads) {
+      const int r = O.rank[s + i];
+      key[0][i] = low_digits_key(path[r]);
+      val[0][i] = r;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 9; ++pass) {
+      const int shift = 4 * pass;
+      const unsigned long long* sk = key[pass & 1];
+      const int32_t* sv = val[pass & 1];
+      unsigned long long* dk = key[(pass + 1) & 1];
+      int32_t* dv = val[(pass + 1) & 1];
+      if (threadIdx.x < 16) hist[threadIdx.x] = 0;
+      __syncthreads();
+      for (int i = threadIdx.x; i < m; i += kLongThreads) atomicAdd(&hist[(sk[i] >> shift) & 15], 1);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int b = 0; b < 16; ++b) {
+          base[b] = acc;
+          acc += hist[b];
+        }
+      }
+      __syncthreads();
+      for (int r0 = 0; r0 < m; r0 += kLongThreads) {
+        const int i = r0 + (int)threadIdx.x;
+        const bool valid = i < m;
+        const unsigned long long k = valid ? sk[i] : 0ull;
+        const int v = valid ? sv[i] : 0;
+        const int d = (int)((k >> shift) & 15u);
+        if (threadIdx.x < kLongWaves * 16) (&wcnt[0][0])[threadIdx.x] = 0;
+        __syncthreads();
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const uint64_t set = __ballot((d >> b) & 1);
+          match &= ((d >> b) & 1) ? set : ~set;
+        }
+        const int below = __popcll(match & below_me);
+        if (valid && below == 0) wcnt[wave][d] = __popcll(match);
+        __syncthreads();
+        if (valid) {
+          int off = base[d] + below;
+          for (int w = 0; w < wave; ++w) off += wcnt[w][d];
+          dk[off] = k;
+          dv[off] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 16) {
+          int t = 0;
+          for (int w = 0; w < kLongWaves; ++w) t += wcnt[w][threadIdx.x];
+          base[threadIdx.x] += t;
+        }
+        __syncthreads();
+      }
+    }
+    // nine passes: the sorted run is in key[1] / val[1]
+    for (int i = threadIdx.x; i < m; i += kLongThreads) {
+      const int r = val[1][i];
+      O.rank[s + i] = r;
+      O.spath[s + i] = path[r];
+    }
+    __syncthreads();
+    for (int i = 1 + (int)threadIdx.x; i < m; i += kLongThreads)
       O.lcp[s + i] = (uint8_t)path_lcp(O.spath[s + i - 1], O.spath[s + i]);
     __syncthreads();
   }
@@ -604,7 +706,7 @@ __global__ __launch_bounds__(1024) void cell_offsets(BhState* st, PathOrder O, T
   }
   if (threadIdx.x != 0) return;
   const int cells = 1 + sh_carry;
-  const int seq = (st->overflow || cells + kMaxDepth + 1 >= max_nodes) ? 1 : 0;  // :449
+  const int seq = (cells + kMaxDepth + 1 >= max_nodes) ? 1 : 0;  // :449
   st->node_count = cells;
   st->final_count = cells;
   st->use_seq = seq;
@@ -1935,6 +2037,11 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->O.bsum, (size_t)t->pos_blocks);
   A(&t->O.lvl, (size_t)lvl_base((int)(P / kLocalBlock + 1), kMaxLevel + 1) + 64);
   A(&t->O.runs, P);
+  A(&t->O.lrun, P / (kRefineMax + 1) + 1);
+  A(&t->O.lkey[0], P);
+  A(&t->O.lkey[1], P);
+  A(&t->O.lval[0], P);
+  A(&t->O.lval[1], P);
   A(&t->O.bstat, (size_t)t->pos_blocks * 3);
   A(&t->O.ppart, P);
   A(&t->nkeys, NN);
@@ -2009,6 +2116,7 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
                      t->st, t->path, t->O);
   hipLaunchKernelGGL(path_refine, dim3(grid_for((size_t)n * 64, 1024)), dim3(kThreads), 0, s,
                      t->st, t->path, t->O);
+  hipLaunchKernelGGL(path_refine_long, dim3(64), dim3(kLongThreads), 0, s, t->st, t->path, t->O);
   const int pb = (int)div_up(n, kPosBlock);
   hipLaunchKernelGGL(cell_count, dim3(n / kPosBlock + 1), dim3(kThreads), 0, s, t->st, t->O);
   hipLaunchKernelGGL(cell_offsets, dim3(1), dim3(1024), 0, s, t->st, t->O, t->T, t->max_nodes);

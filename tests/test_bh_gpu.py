@@ -102,13 +102,52 @@ def test_runs_sharing_sorted_digits(nbx, orc):
     assert t.build_info()["builder"] == "parallel"
 
 
-def test_run_longer_than_a_workgroup_uses_sequential_builder(nbx, orc):
-    # 1,500 distinct bodies share their first 20 digits: more than one workgroup
-    # orders, so the exact sequential builder takes the build; the result is the same
-    h = clustered(nbx, 3000, 1500, 1e-5, seed=8)
+@pytest.mark.parametrize("n_cl,clusters,side,seed", [
+    (1025, 1, 1e-5, 10),  # one more than LDS holds
+    (1025, 1, 1e-7, 9),   # mostly exact duplicates: depth-32 leaves, ties kept in rank order
+    (1500, 1, 1e-5, 8), (5000, 1, 1e-5, 8),
+    (1800, 3, 1e-6, 8),   # three long runs in one build
+])
+def test_runs_longer_than_a_workgroup_stay_parallel(nbx, orc, n_cl, clusters, side, seed):
+    # more bodies share their first 20 digits than one workgroup orders in LDS:
+    # path_refine_long sorts each such run (one workgroup per run, through
+    # global scratch) and the parallel builder keeps the build, bit-exact
+    h = clustered(nbx, 3000, n_cl * clusters, side, seed=seed)
+    for c in range(1, clusters):  # further clusters elsewhere in the sphere
+        sl = slice(3000 + c * n_cl, 3000 + (c + 1) * n_cl)
+        h["pos_x"][sl] = (h["pos_x"][sl] + np.float32(0.4 * c)).astype(np.float32)
     _, t, ref = check_exact(nbx, orc, h)
-    assert prefix_runs(h, ref)[0] > 1024 and ref["node_count"] + 33 < 2 * len(h["mass"])
-    assert t.build_info()["builder"] == "sequential"
+    runs = prefix_runs(h, ref)
+    assert sum(r > 1024 for r in runs) >= clusters, runs[:8]
+    assert t.build_info()["builder"] == "parallel"
+
+
+def test_one_million_with_dense_cluster_stays_parallel(nbx, orc):
+    # the 1,048,576-body collision workload with 50,000 of its bodies moved into
+    # a cube of side 1e-5 (runs of tens of thousands sharing 20 digits, leaves to
+    # depth ~30): parallel build, node array bit-exact, forces vs the oracle
+    n, n_cl = 1 << 20, 50000
+    h = nbx.gen_two_plummer(n, seed=42)
+    rng = np.random.default_rng(3)
+    for a, k in enumerate(("pos_x", "pos_y", "pos_z")):
+        h[k][-n_cl:] = (np.float32((0.3, -0.2, 0.1)[a]) +
+                        rng.uniform(0, 1e-5, n_cl).astype(np.float32)).astype(np.float32)
+    d, t, ref = check_exact(nbx, orc, h)
+    assert t.build_info()["builder"] == "parallel"
+    cl = np.stack([h["pos_x"][-n_cl:], h["pos_y"][-n_cl:], h["pos_z"][-n_cl:]], 1)
+    runs = M.shared_prefix_runs(cl, np.arange(n_cl), ref["bbox_min"], ref["bbox_max"], 20)
+    assert runs[0] > 1024 and ref["max_depth"] > 25, (runs[:4], ref["max_depth"])
+    t.forces(d, 0.5, 1.0, 0.01)
+    nbx.synchronize()
+    tg = np.sort(np.concatenate([np.random.default_rng(4).choice(n - n_cl, 512, replace=False),
+                                 n - n_cl + np.arange(0, n_cl, n_cl // 512)]))
+    got = d.acc().astype(np.float64)[tg]
+    a = orc.bh_forces(ref["nodes"], h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 0.5, 1.0, 0.01,
+                      targets=tg).astype(np.float64)
+    err = np.linalg.norm(got - a, axis=1) / np.linalg.norm(a, axis=1)
+    print(f"1M with a 50,000-body cluster: depth {ref['max_depth']}, nodes {ref['node_count']}, "
+          f"force rel err median {np.median(err):.2e} max {err.max():.2e}", flush=True)
+    assert np.median(err) < 1e-6 and err.max() < 1e-4
 
 
 def test_block_with_more_cells_than_lds_holds(nbx, orc):
