@@ -5,9 +5,12 @@
 //    drift x += v dt + a dt^2/2 does not lose the low bits of small
 //    increments over long runs;
 //  * pair forces stay FP32 (v_rsq_f32); r_ij = (x_j.hi - x_i.hi) + (x_j.lo - x_i.lo);
-//  * each target's sum is Kahan-compensated in FP32 over its whole j chunk;
-//  * FP64 appears only in the cross-wave reduction of the compensated sums, the
-//    hi/lo renormalisation of the drift, and the energy/momentum reductions.
+//  * each target's sum is Kahan-compensated in FP32 over its whole j chunk, and
+//    the compensated sums are combined (across waves, chunks, tiles and rounds)
+//    as double-single FP32 pairs (ds_add: TwoSum keeps every addition's
+//    rounding error);
+//  * FP64 appears only in the hi/lo renormalisation of the drift and the
+//    energy/momentum reductions.
 #include <algorithm>
 
 #include "common.hpp"
@@ -21,6 +24,21 @@ namespace nbx {
 namespace {
 
 constexpr int kDsBatch = 8;  // 7 scalar loads per body: keep a batch within the SGPR budget
+
+// hi + lo += x in FP32, the addition's rounding error kept in lo (Knuth's TwoSum).
+__device__ __forceinline__ void ds_add(float& hi, float& lo, float x) {
+  const float s = add_rn(hi, x);
+  const float bp = sub_rn(s, hi);
+  lo = add_rn(lo, add_rn(sub_rn(hi, sub_rn(s, bp)), sub_rn(x, bp)));
+  hi = s;
+}
+
+// (hi + lo) * k as one FP32 value: the product's rounding error of hi * k is
+// recovered with an FMA and added back with lo * k.
+__device__ __forceinline__ float ds_scale(float hi, float lo, float k) {
+  const float p = mul_rn(hi, k);
+  return add_rn(p, add_rn(__builtin_fmaf(hi, k, -p), mul_rn(lo, k)));
+}
 
 __device__ __forceinline__ double dadd_rn(double a, double b) {
 #pragma clang fp contract(off)
@@ -59,9 +77,9 @@ template <int IPL, int WS, bool CHECKED, bool KICK>
 __global__ __launch_bounds__(64 * WS) void ds_forces_kernel(
     const float* __restrict__ xh, const float* __restrict__ yh, const float* __restrict__ zh,
     const float* __restrict__ xl, const float* __restrict__ yl, const float* __restrict__ zl,
-    const float* __restrict__ m, int n, double G, float eps2, float* ax, float* ay, float* az,
+    const float* __restrict__ m, int n, float G, float eps2, float* ax, float* ay, float* az,
     KickArgs kick) {
-  __shared__ double red[WS * 3 * 64 * IPL];
+  __shared__ float2 red[WS * 3 * 64 * IPL];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ibase = (int)blockIdx.x * 64 * IPL;
@@ -111,17 +129,22 @@ __global__ __launch_bounds__(64 * WS) void ds_forces_kernel(
 #pragma unroll
   for (int q = 0; q < IPL; ++q) {
     const int sl = q * 64 + lane;
-    for (int a = 0; a < 3; ++a) red[(wave * 3 + a) * T + sl] = (double)s[q][a] - (double)c[q][a];
+    for (int a = 0; a < 3; ++a) red[(wave * 3 + a) * T + sl] = make_float2(s[q][a], -c[q][a]);
   }
   __syncthreads();
   for (int sl = threadIdx.x; sl < T; sl += 64 * WS) {
     const int i = ibase + sl;
     if (i >= n) continue;
-    double t[3] = {0.0, 0.0, 0.0};
+    float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int w = 0; w < WS; ++w)
-      for (int a = 0; a < 3; ++a) t[a] += red[(w * 3 + a) * T + sl];
-    const float a0 = (float)(G * t[0]), a1 = (float)(G * t[1]), a2 = (float)(G * t[2]);
+      for (int a = 0; a < 3; ++a) {
+        const float2 r = red[(w * 3 + a) * T + sl];
+        ds_add(hi[a], lo[a], r.x);
+        lo[a] = add_rn(lo[a], r.y);
+      }
+    const float a0 = ds_scale(hi[0], lo[0], G), a1 = ds_scale(hi[1], lo[1], G),
+                a2 = ds_scale(hi[2], lo[2], G);
     ax[i] = a0;
     ay[i] = a1;
     az[i] = a2;
@@ -229,7 +252,7 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
                                             const float* __restrict__ m, const SymGeom& g, int A,
                                             int C, int dd, float mref, float eps2,
                                             const float (&ih)[IPL][3], const float (&il)[IPL][3],
-                                            const float (&mi)[IPL], double (*red)[W][3][64]) {
+                                            const float (&mi)[IPL], float2 (*red)[W][3][64]) {
   constexpr int B = 64 * IPL;
   constexpr int S = W * B;
   constexpr int kChunks = S / 64;
@@ -237,11 +260,11 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int jbase = C * S;
   const int src_addr = ((lane + 63) & 63) * 4;
-  double sx[IPL][3];
+  float sh[IPL][3], sl[IPL][3];  // the targets' sums over chunks, double-single
 #pragma unroll
   for (int q = 0; q < IPL; ++q)
 #pragma unroll
-    for (int a = 0; a < 3; ++a) sx[q][a] = 0.0;
+    for (int a = 0; a < 3; ++a) sh[q][a] = sl[q][a] = 0.0f;
   float nb[7];
   load_ds_body(xh, yh, zh, xl, yl, zl, m, jbase + lane, g.n, nb);
   for (int ch = 0; ch < kChunks; ++ch) {
@@ -263,22 +286,25 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
 #pragma unroll
     for (int q = 0; q < IPL; ++q)
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double v = (double)s[q][a] - (double)c[q][a];
-        sx[q][a] += UNI ? v * (double)mref : v;
+      for (int a = 0; a < 3; ++a) {  // the chunk's Kahan sum is s - c
+        ds_add(sh[q][a], sl[q][a], s[q][a]);
+        sl[q][a] = sub_rn(sl[q][a], c[q][a]);
       }
     if (JSIDE) {
-      double(*buf)[3][64] = red[ch & 1];
+      float2(*buf)[3][64] = red[ch & 1];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) buf[wave][a][lane] = (double)gs[a] - (double)gc[a];
+      for (int a = 0; a < 3; ++a) buf[wave][a][lane] = make_float2(gs[a], -gc[a]);
       __syncthreads();
       for (int r = threadIdx.x; r < 192; r += 64 * W) {
         const int k = r >> 6, l = r & 63;
-        double t = 0.0;
+        float th = 0.0f, tl = 0.0f;
 #pragma unroll
-        for (int w = 0; w < W; ++w) t += buf[w][k][l];
-        if (UNI) t *= (double)mref;
-        g.pj[((long)(dd - g.pj_lo()) * 3 + k) * g.npad + jbase + ch * 64 + l] = (float)t;
+        for (int w = 0; w < W; ++w) {
+          ds_add(th, tl, buf[w][k][l].x);
+          tl = add_rn(tl, buf[w][k][l].y);
+        }
+        g.pj[((long)(dd - g.pj_lo()) * 3 + k) * g.npad + jbase + ch * 64 + l] =
+            UNI ? ds_scale(th, tl, mref) : add_rn(th, tl);
       }
     }
   }
@@ -287,7 +313,8 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
   for (int q = 0; q < IPL; ++q)
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      g.pi[((long)(dd - g.d_lo) * 3 + a) * g.npad + ibase + q * 64 + lane] = (float)sx[q][a];
+      g.pi[((long)(dd - g.d_lo) * 3 + a) * g.npad + ibase + q * 64 + lane] =
+          UNI ? ds_scale(sh[q][a], sl[q][a], mref) : add_rn(sh[q][a], sl[q][a]);
 }
 
 template <int IPL, int W, bool CHECKED>
@@ -297,7 +324,7 @@ __global__ __launch_bounds__(64 * W) void ds_sym_kernel(
     const float* __restrict__ m, SymGeom g, float eps2) {
   constexpr int B = 64 * IPL;
   constexpr int S = W * B;
-  __shared__ double red[2][W][3][64];
+  __shared__ float2 red[2][W][3][64];
   int A, dd;
   if (!sym_tile_of(g, (int)blockIdx.x, A, dd)) return;  // whole workgroup: the other row's tile
   const int lane = threadIdx.x & 63;
@@ -335,13 +362,83 @@ __global__ __launch_bounds__(64 * W) void ds_sym_kernel(
 #undef NBX_DS_TILE
 }
 
+// Body b's partials of one round added to (hi, lo) in the canonical order of
+// sym_round_add (for d ascending, the i-side partial of distance d, then the
+// j-side one), as double-single FP32 sums; the carry between rounds holds the
+// pair exactly, so every round size gives the same bits.
+__device__ __forceinline__ void ds_round_add(const SymGeom& g, int b, float (&hi)[3],
+                                             float (&lo)[3]) {
+  const int C = b / g.S;
+  int d = g.d_lo;
+  if (d == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ds_add(hi[k], lo[k], g.pi[(long)k * g.npad + b]);
+    d = 1;
+  }
+  // both sides exist for every row at d <= Dmax - 1; four distances' loads are
+  // issued before their adds, which keep the canonical order
+  const int dsafe = min(g.d_hi, g.Dmax - 1);
+  for (; d + 3 <= dsafe; d += 4) {
+    float vi[4][3], vj[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        vi[u][k] = g.pi[((long)(d + u - g.d_lo) * 3 + k) * g.npad + b];
+        vj[u][k] = g.pj[((long)(d + u - g.pj_lo()) * 3 + k) * g.npad + b];
+      }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        ds_add(hi[k], lo[k], vi[u][k]);
+        ds_add(hi[k], lo[k], vj[u][k]);
+      }
+  }
+  for (; d <= g.d_hi; ++d) {
+    const bool own = d <= sym_rowlen(C, g.K);
+    const bool other = d <= sym_rowlen((C - d + g.K) % g.K, g.K);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (own) ds_add(hi[k], lo[k], g.pi[((long)(d - g.d_lo) * 3 + k) * g.npad + b]);
+      if (other) ds_add(hi[k], lo[k], g.pj[((long)(d - g.pj_lo()) * 3 + k) * g.npad + b]);
+    }
+  }
+}
+
+// One round's reduction of the double-single partial sums: (hi, lo) starts at
+// zero (first round) or from the carry (3 float pairs per padded body); after
+// the last round a = G (hi + lo) and the optional kick, as sym_reduce_body.
 template <bool KICK>
-__global__ __launch_bounds__(256) void ds_sym_reduce(SymGeom g, double G, float* ax, float* ay,
-                                                     float* az, KickArgs kick, double* carry,
+__global__ __launch_bounds__(256) void ds_sym_reduce(SymGeom g, float G, float* ax, float* ay,
+                                                     float* az, KickArgs kick, float2* carry,
                                                      int last) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= g.n) return;
-  sym_reduce_body<KICK>(g, b, G, ax, ay, az, kick, carry, last != 0);
+  float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
+  if (g.d_lo > 0)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float2 c = carry[(long)k * g.npad + b];
+      hi[k] = c.x;
+      lo[k] = c.y;
+    }
+  ds_round_add(g, b, hi, lo);
+  if (!last) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) carry[(long)k * g.npad + b] = make_float2(hi[k], lo[k]);
+    return;
+  }
+  const float a0 = ds_scale(hi[0], lo[0], G), a1 = ds_scale(hi[1], lo[1], G),
+              a2 = ds_scale(hi[2], lo[2], G);
+  ax[b] = a0;
+  ay[b] = a1;
+  az[b] = a2;
+  if (KICK) {
+    kick.vx[b] = add_rn(kick.vx[b], mul_rn(add_rn(kick.aox[b], a0), kick.dt_half));
+    kick.vy[b] = add_rn(kick.vy[b], mul_rn(add_rn(kick.aoy[b], a1), kick.dt_half));
+    kick.vz[b] = add_rn(kick.vz[b], mul_rn(add_rn(kick.aoz[b], a2), kick.dt_half));
+  }
 }
 
 // acc_old <- acc; (hi, lo) += v dt + a dt^2/2 evaluated and renormalised in FP64.
@@ -410,12 +507,13 @@ int ds_sym_launch(nbx_ds_particles* d, float G, float eps2, const KickArgs* kick
                          0, s, p.pos_x, p.pos_y, p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z,
                          p.mass, gr, eps2);
     NBX_LAUNCHED();
+    float2* carry = reinterpret_cast<float2*>(w.carry);  // 8 bytes per component either way
     if (kick)
-      hipLaunchKernelGGL(ds_sym_reduce<true>, dim3(rb), dim3(256), 0, s, gr, (double)G, p.acc_x,
-                         p.acc_y, p.acc_z, k, w.carry, last);
+      hipLaunchKernelGGL(ds_sym_reduce<true>, dim3(rb), dim3(256), 0, s, gr, G, p.acc_x, p.acc_y,
+                         p.acc_z, k, carry, last);
     else
-      hipLaunchKernelGGL(ds_sym_reduce<false>, dim3(rb), dim3(256), 0, s, gr, (double)G, p.acc_x,
-                         p.acc_y, p.acc_z, k, w.carry, last);
+      hipLaunchKernelGGL(ds_sym_reduce<false>, dim3(rb), dim3(256), 0, s, gr, G, p.acc_x, p.acc_y,
+                         p.acc_z, k, carry, last);
     NBX_LAUNCHED();
     if (last) break;
     lo = hi + 1;
@@ -439,7 +537,7 @@ int ds_forces_launch(nbx_direct* op, nbx_ds_particles* d, float G, float eps2,
   const bool checked = eps2 < kUncheckedMinEps2;
 #define NBX_DS_LAUNCH(C, K)                                                                    \
   hipLaunchKernelGGL((ds_forces_kernel<kIpl, kWs, C, K>), grid, block, 0, s, p.pos_x, p.pos_y, \
-                     p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z, p.mass, n, (double)G,    \
+                     p.pos_z, d->pos_lo_x, d->pos_lo_y, d->pos_lo_z, p.mass, n, G,            \
                      eps2, p.acc_x, p.acc_y, p.acc_z, k)
   if (checked) {
     if (kick) NBX_DS_LAUNCH(true, true);
