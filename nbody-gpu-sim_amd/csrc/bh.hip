@@ -256,8 +256,8 @@ __device__ __forceinline__ void trav_fields(float size2, float mass, float inv_t
                                             float* gm, float* s_open, float* s_acc) {
   const float sc = mul_rn(size2, inv_theta2);
   *gm = mul_rn(G, mass);
-  *s_open = mul_rn(sc, 1.0f - 0x1p-20f);
-  *s_acc = mul_rn(sc, 1.0f + 0x1p-20f);
+  *s_open = mul_rn(sc, 1.0f - 0x1p-18f);
+  *s_acc = mul_rn(sc, 1.0f + 0x1p-18f);
 }
 
 __device__ __forceinline__ void write_trav(float4* rec, int p, int size, float4 com, float4 box,
@@ -1462,18 +1462,77 @@ __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __re
   }
 }
 
+// ---------------------------------------------------------------- 6b: task order
+
+// Skilling's Hilbert index (AIP Conf. Proc. 707, 381 (2004)) of a cell of the
+// 1024^3 grid: the transpose form of the three 10-bit coordinates, then its
+// bits interleaved, the first axis most significant.
+__device__ __forceinline__ uint32_t hilbert_index(uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t X[3] = {x, y, z};
+  for (uint32_t q = 1u << 9; q > 1; q >>= 1) {
+    const uint32_t p = q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & q) {
+        X[0] ^= p;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & p;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  uint32_t t = 0;
+  for (uint32_t q = 1u << 9; q > 1; q >>= 1)
+    if (X[2] & q) t ^= q - 1;
+  return (expand_bits(X[0] ^ t) << 2) | (expand_bits(X[1] ^ t) << 1) | expand_bits(X[2] ^ t);
+}
+
+// The order traversal tasks take the particles in (64 consecutive per task):
+// the tree's own order (path order; Morton order after the sequential
+// builder), sorted stably by the Hilbert index of each particle's Morton cell
+// (the same 1024^3 grid as its Morton code). Morton and path order jump
+// between distant cells at octant boundaries, so some 64-particle tasks span
+// two far-apart groups and their walk covers both groups' trees; consecutive
+// Hilbert cells are always adjacent, so tasks stay compact (7 % fewer walk
+// steps on the two-Plummer million). Any order gives the same forces: each
+// lane sums its interactions in the records' order whatever task it is in.
+__global__ __launch_bounds__(kThreads) void task_keys(const BhState* __restrict__ st,
+                                                      const float* __restrict__ x,
+                                                      const float* __restrict__ y,
+                                                      const float* __restrict__ z,
+                                                      const int32_t* __restrict__ morton_order,
+                                                      const int32_t* __restrict__ path_order,
+                                                      uint32_t* key, int32_t* order) {
+  const int r = blockIdx.x * kThreads + threadIdx.x;
+  if (r >= st->n) return;
+  const int p = st->use_seq ? morton_order[r] : path_order[r];
+  const uint32_t qx = quantise(mul_rn(sub_rn(x[p], st->bmin[0]), st->inv[0]));
+  const uint32_t qy = quantise(mul_rn(sub_rn(y[p], st->bmin[1]), st->inv[1]));
+  const uint32_t qz = quantise(mul_rn(sub_rn(z[p], st->bmin[2]), st->inv[2]));
+  key[r] = hilbert_index(qx, qy, qz);
+  order[r] = p;
+}
+
 // ---------------------------------------------------------------- 7: traversal
 
 // Per force call: each record's opening thresholds for this theta and its
 // G * mass. rec[4p+2].zw = (S_open, S_acc) with S = size2 * (1 / theta2),
-// S_open = S (1 - 2^-20) and S_acc = S (1 + 2^-20), each product rounded once;
+// S_open = S (1 - 2^-18) and S_acc = S (1 + 2^-18), each product rounded once;
 // rec[4p].w = G * mass rounded once, as the traversal's force term rounds it.
-// S_open and S_acc are within 2^-22 (relative) of size2 / theta2 (1 -+ 2^-20),
-// so d2 < S_open implies size2 / d2 > theta2 (1 + 2^-21) and d2 > S_acc implies
-// size2 / d2 < theta2 (1 - 2^-21): either way the rounded division takes the
-// same decision (:218-224). In between, the walk divides. Leaves and zero-mass
-// cells (size2 = -1) get negative thresholds and are never opened by the test;
-// theta = 0 gives S = +inf, so every cell with mass is opened.
+// S_open and S_acc are within 2^-22 (relative) of size2 / theta2 (1 -+ 2^-18).
+// The walk's common step tests them against a fused
+// d2f = fma(dz, dz, fma(dy, dy, fma(dx, dx, eps2))), three roundings of
+// nonnegative terms against the four of the reference's
+// d2 = ((dx dx + dy dy) + dz dz) + eps2 (:221), so d2 <= d2f (1 + 7 2^-24)
+// and d2 >= d2f (1 - 7 2^-24). Hence d2f < S_open implies d2 < S (1 - 2^-19),
+// i.e. size2 / d2 > theta2 (1 + 2^-20), and d2f > S_acc implies
+// size2 / d2 < theta2 (1 - 2^-20): either way the rounded division takes the
+// same decision (:224). In between, the walk computes the reference's d2
+// and divides. Leaves and zero-mass cells (size2 = -1) get negative thresholds
+// and are never opened by the test; theta = 0 gives S = +inf, so every cell
+// with mass is opened.
 __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState* __restrict__ st,
                                                       float inv_theta2, float G) {
   const int nn = st->final_count;
@@ -1499,19 +1558,21 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 // (<= 33), kept one per lane in VGPRs (lane k holds entry k). A lane's
 // interactions are summed in the reference's order.
 //
-// Each step waits on one dependent record load, which mostly misses the
-// scalar cache; that latency, not instruction issue, bounds the walk. So the
-// common step is written out by hand (NBX_WALK_FAST) and loads ahead: while
+// Each step needs the record the previous step chose, and that load mostly
+// misses the scalar cache. So the common step is written out by hand
+// (NBX_WALK_FAST) and loads ahead: while
 // it works on record v it already fetches both records the next step can
 // need, v + 1 (descend) and skip[v] (continue). Three SGPR sets of 12 dwords
 // rotate: the step on set X loads v + 1 into Y and skip[v] into Z, and
 // continues with Y or Z (A -> B/C, B -> C/A, C -> A/B). Pops change only the
 // mask, never v, so the next record is always one of the two. Prefetch
 // addresses are clamped to the pad record after the last one (v + 1 <= nn
-// inside a step), and every exit waits for the loads in flight. The common step is: no lane's particle inside a leaf and
-// every lane's theta decision clear of the band where the division decides.
-// Any other step leaves the asm and runs as C code below (the own-leaf
-// correction, the IEEE division); the walk then re-enters the asm.
+// inside a step), and every exit waits for the loads in flight. The common
+// step is: no lane's particle inside a leaf and every lane's theta decision
+// clear of the band where the division decides (tested with the fused d2, see
+// trav_prep). Any other step leaves the asm and runs as C code below (the
+// own-leaf correction, the reference's d2 and the IEEE division); the walk then
+// re-enters the asm.
 //
 // Registers: s[20:31], s[32:43], s[44:55] the record sets (dwords 0..11 of a
 // record), s[56:57] M, s[58:59] the lanes inside the cell (then the undecided
@@ -1549,12 +1610,9 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "v_sub_f32 %[dx], " c0 ", %[xi]\n"                                             \
   "v_sub_f32 %[dy], " c1 ", %[yi]\n"                                             \
   "v_sub_f32 %[dz], " c2 ", %[zi]\n"                                             \
-  "v_mul_f32 %[d2], %[dx], %[dx]\n"                                              \
-  "v_mul_f32 %[t1], %[dy], %[dy]\n"                                              \
-  "v_add_f32 %[d2], %[d2], %[t1]\n"                                              \
-  "v_mul_f32 %[t1], %[dz], %[dz]\n"                                              \
-  "v_add_f32 %[d2], %[d2], %[t1]\n"                                              \
-  "v_add_f32 %[d2], %[eps2], %[d2]\n"                                            \
+  "v_fma_f32 %[d2], %[dx], %[dx], %[eps2]\n"                                     \
+  "v_fmac_f32 %[d2], %[dy], %[dy]\n"                                             \
+  "v_fmac_f32 %[d2], %[dz], %[dz]\n"                                             \
   "v_rsq_f32 %[inv], %[d2]\n"                                                    \
   "v_cmp_gt_f32_e64 s[60:61], " so ", %[d2]\n"                                   \
   "v_cmp_lt_f32_e64 s[62:63], " sa ", %[d2]\n"                                   \
@@ -1625,10 +1683,12 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 // The decisions of one walk step at record v for the wave's lanes M (the C
 // form of the asm's common step, covering every case): the lanes that open the
 // node (om, descend), the lanes that take its force (im), the force operands
-// and the position after the node's subtree.
+// and the position after the node's subtree. The decisions use the reference's
+// d2; the force uses the fused d2 of the asm step, so a lane's sum does not
+// depend on which of its steps ran here (nor on which task it was in).
 struct WalkStep {
   uint64_t om, im;
-  float dx, dy, dz, d2, gm;
+  float dx, dy, dz, d2f, gm;  // d2f: the fused d2 the force uses in every step
   int after;
 };
 
@@ -1683,7 +1743,7 @@ __device__ __forceinline__ WalkStep walk_decide(const float4* __restrict__ rec, 
   w.dx = dx;
   w.dy = dy;
   w.dz = dz;
-  w.d2 = d2;
+  w.d2f = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
   w.gm = gm;
   w.after = after;
   return w;
@@ -1742,8 +1802,8 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
     const WalkStep w = walk_decide(rec, v, M, xi, yi, zi, mi, me, theta2, G, eps2);
     const int after = w.after;
     const uint64_t om = w.om, im = w.im;
-    const float dx = w.dx, dy = w.dy, dz = w.dz, d2 = w.d2, gm = w.gm;
-    const float inv = __builtin_amdgcn_rsqf(d2);
+    const float dx = w.dx, dy = w.dy, dz = w.dz, gm = w.gm;
+    const float inv = __builtin_amdgcn_rsqf(w.d2f);
     const float fi = gm * (inv * inv * inv);
     const float f = ((im >> lane) & 1) ? fi : 0.0f;
     sx = __builtin_fmaf(f, dx, sx);
@@ -1780,9 +1840,9 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
 }
 
 // Persistent waves, one per workgroup, one grid of resident waves. Tasks are
-// 64 consecutive particles in the tree's own order (path order; Morton order
-// after the sequential builder): bodies of one cell share a wave, however many
-// share a Morton code. Workgroups are dealt to the 8 XCDs round-robin; each
+// 64 consecutive particles of the task order (task_keys: the tree's own order
+// re-sorted by Hilbert index), so bodies of one cell share a wave however many
+// share a Morton code, and a task's bodies are close together. Workgroups are dealt to the 8 XCDs round-robin; each
 // XCD owns one contiguous eighth of the tasks, so that its L2 holds the part
 // of the tree its waves walk, and its waves take that eighth's tasks in order
 // from a counter. A walk is bound by load latency, so a slow task cannot be
@@ -1792,8 +1852,7 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
 // slower: those walks miss the L2 that holds their part of the tree.)
 __global__ __launch_bounds__(64) void bh_traverse(
     const float4* __restrict__ rec, const BhState* __restrict__ st,
-    const int32_t* __restrict__ morton_order, const int32_t* __restrict__ path_order,
-    int* queue, int ntasks, const float* __restrict__ x, const float* __restrict__ y,
+    const int32_t* __restrict__ task_order, int* queue, int ntasks, const float* __restrict__ x, const float* __restrict__ y,
     const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
     float eps2, float* ax, float* ay, float* az) {
   const int lane = threadIdx.x;
@@ -1810,7 +1869,7 @@ __global__ __launch_bounds__(64) void bh_traverse(
     return v;
   };
   const int32_t* order =
-      reinterpret_cast<const int32_t*>(vreg(st->use_seq ? morton_order : path_order));
+      reinterpret_cast<const int32_t*>(vreg(task_order));
   const float* vx = reinterpret_cast<const float*>(vreg(x));
   const float* vy = reinterpret_cast<const float*>(vreg(y));
   const float* vz = reinterpret_cast<const float*>(vreg(z));
@@ -1847,14 +1906,13 @@ __global__ void trav_queue_reset(int* queue) {
 // with the same values.
 __global__ __launch_bounds__(64) void bh_walk_census(
     const float4* __restrict__ rec, const BhState* __restrict__ st,
-    const int32_t* __restrict__ morton_order, const int32_t* __restrict__ path_order, int ntasks,
+    const int32_t* __restrict__ order, int ntasks,
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     const float* __restrict__ m, int n, float theta2, float G, float eps2,
     unsigned long long* counts) {
   __shared__ int at[kMaxDepth + 8];
   __shared__ unsigned long long mask[kMaxDepth + 8];
   const int lane = threadIdx.x;
-  const int32_t* order = st->use_seq ? morton_order : path_order;
   const int nn = st->final_count;
   const uint64_t me = 1ull << lane;
   unsigned long long steps = 0, forces = 0, opens = 0, active = 0;
@@ -1923,6 +1981,8 @@ struct nbx_bh {
   int32_t* final_of = nullptr;
   nbx::RefNode* nodes = nullptr;
   float4* trav_rec = nullptr;  // 64 bytes per cell, traversal order
+  uint32_t* task_key = nullptr;   // Hilbert index of each particle's Morton cell
+  int32_t* task_order = nullptr;  // particles in the order traversal tasks take them
   nbx::RadixSort sort_particles;
   nbx::RadixSort sort_nodes;
   bool built = false;
@@ -2049,6 +2109,8 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->final_of, NN);
   A(&t->nodes, NN);
   A(&t->trav_rec, (NN + 1) * 4);  // + the pad record the walk's prefetch may read
+  A(&t->task_key, P);
+  A(&t->task_order, P);
   A(&t->trav_queue, 8 * 16);
   A(&t->census, 4);
   if (rc == NBX_OK) {
@@ -2152,7 +2214,10 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
                      d->pos_z, d->mass, t->nodes, t->max_nodes);
   hipLaunchKernelGGL(seq_preorder, dim3(1), dim3(64), 0, s, t->st, t->nodes, t->T.pre, t->T.size,
                      t->T.k, t->trav_rec, t->prep);
+  hipLaunchKernelGGL(task_keys, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->st, d->pos_x,
+                     d->pos_y, d->pos_z, t->idx, t->O.ppart, t->task_key, t->task_order);
   NBX_LAUNCHED();
+  NBX_TRY(t->sort_particles.sort(t->task_key, t->task_order, &t->st->n, 30, s));
   mark(t, 7, s);
   t->built = true;
   t->built_n = d->count;
@@ -2185,8 +2250,8 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   const int ntasks = (int)div_up(n, 64);
   const int waves = std::max(8, std::min(ntasks, t->trav_waves) & ~7);
   hipLaunchKernelGGL(trav_queue_reset, dim3(1), dim3(64), 0, s, t->trav_queue);
-  hipLaunchKernelGGL(bh_traverse, dim3(waves), dim3(64), 0, s, t->trav_rec, t->st, t->idx,
-                     t->O.ppart, t->trav_queue, ntasks, d->pos_x, d->pos_y, d->pos_z, d->mass, n,
+  hipLaunchKernelGGL(bh_traverse, dim3(waves), dim3(64), 0, s, t->trav_rec, t->st,
+                     t->task_order, t->trav_queue, ntasks, d->pos_x, d->pos_y, d->pos_z, d->mass, n,
                      theta2, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   mark(t, 9, s);
@@ -2290,7 +2355,7 @@ int nbx_bh_walk_census(nbx_bh* t, const nbx_particles* d, float theta, float G, 
   const int ntasks = (int)div_up(n, 64);
   hipLaunchKernelGGL(census_clear, dim3(1), dim3(64), 0, nullptr, t->census);
   hipLaunchKernelGGL(bh_walk_census, dim3(std::min(ntasks, 8192)), dim3(64), 0, nullptr,
-                     t->trav_rec, t->st, t->idx, t->O.ppart, ntasks, d->pos_x, d->pos_y,
+                     t->trav_rec, t->st, t->task_order, ntasks, d->pos_x, d->pos_y,
                      d->pos_z, d->mass, n, theta * theta, G, eps * eps, t->census);
   NBX_LAUNCHED();
   unsigned long long h[4];
