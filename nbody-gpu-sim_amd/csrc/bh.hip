@@ -256,7 +256,10 @@ __device__ __forceinline__ void trav_fields(float size2, float mass, float inv_t
                                             float* gm, float* s_open, float* s_acc) {
   const float sc = mul_rn(size2, inv_theta2);
   *gm = mul_rn(G, mass);
-  *s_open = mul_rn(sc, 1.0f - 0x1p-18f);
+  // Leaves and zero-mass cells (size2 < 0) are never opened by the test, so
+  // their S_open holds -mass (<= 0, still never above d2): the walk's common
+  // step compares a lane's own mass with a leaf's from it.
+  *s_open = size2 < 0.0f ? -mass : mul_rn(sc, 1.0f - 0x1p-18f);
   *s_acc = mul_rn(sc, 1.0f + 0x1p-18f);
 }
 
@@ -1568,11 +1571,15 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 // mask, never v, so the next record is always one of the two. Prefetch
 // addresses are clamped to the pad record after the last one (v + 1 <= nn
 // inside a step), and every exit waits for the loads in flight. The common
-// step is: no lane's particle inside a leaf and every lane's theta decision
-// clear of the band where the division decides (tested with the fused d2, see
-// trav_prep). Any other step leaves the asm and runs as C code below (the
-// own-leaf correction, the reference's d2 and the IEEE division); the walk then
-// re-enters the asm.
+// step is: every lane's theta decision clear of the band where the division
+// decides (tested with the fused d2, see trav_prep), and every lane whose
+// particle lies inside a leaf is at least as heavy as the leaf (m_eff <= 0 at
+// :194-196, the lane's own single-body leaf: no force). Such lanes are handled
+// as if they opened the leaf: they take no force, and the walk steps to v + 1
+// with them, where the remembered entry (the leaf's end, v + 1) restores the
+// mask at once. Any other step leaves the asm and runs as C code below (the
+// own-leaf correction of a heavier leaf, the reference's d2 and the IEEE
+// division); the walk then re-enters the asm.
 //
 // Registers: s[20:31], s[32:43], s[44:55] the record sets (dwords 0..11 of a
 // record), s[56:57] M, s[58:59] the lanes inside the cell (then the undecided
@@ -1646,7 +1653,11 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_branch L_next_" Y "_%=\n"                                                   \
   "L_inside_" X "_%=:\n"                                                         \
   "s_cmp_lt_i32 " hh ", 0\n"                                                     \
+  "s_cbranch_scc0 L_open_" X "_%=\n"                                             \
+  "v_cmp_ge_f32_e64 s[66:67], %[mi], -" so "\n"                                  \
+  "s_andn2_b64 s[66:67], s[58:59], s[66:67]\n"                                   \
   "s_cbranch_scc1 L_exit_%=\n"                                                   \
+  "L_open_" X "_%=:\n"                                                           \
   "s_or_b64 s[60:61], s[60:61], s[58:59]\n"                                      \
   "s_branch L_back_" X "_%=\n"                                                   \
   "L_pop_" X "_%=:\n"                                                            \
@@ -1786,7 +1797,7 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
                    [t2] "=&v"(t2), [dx] "=&v"(dxa), [dy] "=&v"(dya), [dz] "=&v"(dza),
                    [d2] "=&v"(d2a), [inv] "=&v"(inva)
                  : [rb] "s"(rec), [nn] "s"(nn), [eps2] "v"(eps2), [xi] "v"(xi), [yi] "v"(yi),
-                   [zi] "v"(zi)
+                   [zi] "v"(zi), [mi] "v"(mi)
                  : "s20", "s21", "s22", "s23", "s24", "s25", "s26", "s27", "s28", "s29", "s30",
                    "s31", "s32", "s33", "s34", "s35", "s36", "s37", "s38", "s39", "s40", "s41",
                    "s42", "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52",
