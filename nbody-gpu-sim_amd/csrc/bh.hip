@@ -1592,19 +1592,24 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 // scalar instructions (interlocked) and from scalar instructions to v_cndmask
 // (no hazard); readlane takes its lane from a scalar instruction and writelane
 // from m0, written one instruction before.
-#define NBX_WALK_STEP(X, Y, Z, c0, c1, c2, gm, b0, b1, b2, hh, sk, so, sa, y8, y4, z8, z4) \
+// The common path of one step on record set X (the record at v): the loads of
+// the two possible next records into Y (v + 1) and Z (skip), the decisions,
+// the force; it ends by continuing at the skip (set Z), which falls through to
+// the next block when Z is laid out after X (TAIL empty), or descending, or
+// leaving for C. The pop test also ends the walk: with no remembered entry,
+// top holds nn. The skip is clamped to [v + 1, nn] and a step runs only at
+// v < top <= nn, so both loads stay within the records and the pad record
+// after them (at nn), and the walk ends whatever the records hold.
+#define NBX_WALK_MAIN(X, Y, Z, c0, c1, c2, gm, b0, b1, b2, hh, sk, so, sa, y8, y4, z8, z4, TAIL) \
   "L_next_" X "_%=:\n"                                                           \
-  "s_cmp_ge_i32 %[v], %[nn]\n"                                                   \
-  "s_cbranch_scc1 L_exit_%=\n"                                                   \
-  "L_popchk_" X "_%=:\n"                                                         \
-  "s_cmp_eq_u32 %[v], %[top]\n"                                                  \
+  "s_cmp_ge_u32 %[v], %[top]\n"                                                  \
   "s_cbranch_scc1 L_pop_" X "_%=\n"                                              \
   "s_waitcnt lgkmcnt(0)\n"                                                       \
   "s_add_i32 s70, %[v], 1\n"                                                     \
   "s_max_i32 s71, " sk ", s70\n"                                                 \
   "s_lshl_b32 s66, s70, 6\n"                                                     \
-  "s_min_i32 s67, s71, %[nn]\n"                                                  \
-  "s_lshl_b32 s67, s67, 6\n"                                                     \
+  "s_min_i32 s71, s71, %[nn]\n"                                                  \
+  "s_lshl_b32 s67, s71, 6\n"                                                     \
   "s_load_dwordx8 " y8 ", %[rb], s66\n"                                          \
   "s_load_dwordx4 " y4 ", %[rb], s66 offset:32\n"                                \
   "s_load_dwordx8 " z8 ", %[rb], s67\n"                                          \
@@ -1639,8 +1644,14 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "v_fmac_f32 %[sy], %[t1], %[dy]\n"                                             \
   "v_fmac_f32 %[sz], %[t1], %[dz]\n"                                             \
   "s_cbranch_scc1 L_desc_" X "_%=\n"                                             \
-  "s_mov_b32 %[v], s71\n"                                                        \
-  "s_branch L_next_" Z "_%=\n"                                                   \
+  "s_mov_b32 %[v], s71\n" TAIL
+
+// The rarer paths of a step on set X, laid out after the three common paths:
+// descend (remember the skip and the mask, continue at v + 1 in set Y), a lane
+// inside the cell (open an internal cell; a leaf no heavier than the lane's
+// particle, or leave for C), and the pops (restore the remembered mask; with
+// none left the walk is over).
+#define NBX_WALK_OOL(X, Y, hh, so)                                               \
   "L_desc_" X "_%=:\n"                                                           \
   "s_mov_b32 m0, %[depth]\n"                                                     \
   "s_add_i32 %[depth], %[depth], 1\n"                                            \
@@ -1661,6 +1672,8 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_or_b64 s[60:61], s[60:61], s[58:59]\n"                                      \
   "s_branch L_back_" X "_%=\n"                                                   \
   "L_pop_" X "_%=:\n"                                                            \
+  "s_cmp_eq_u32 %[depth], 0\n"                                                   \
+  "s_cbranch_scc1 L_exit_%=\n"                                                   \
   "s_add_i32 %[depth], %[depth], -1\n"                                           \
   "v_readlane_b32 s56, %[klo], %[depth]\n"                                       \
   "v_readlane_b32 s57, %[khi], %[depth]\n"                                       \
@@ -1668,11 +1681,12 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_cmp_lt_i32 s58, 0\n"                                                        \
   "s_cbranch_scc1 L_root_" X "_%=\n"                                             \
   "v_readlane_b32 %[top], %[kat], s58\n"                                         \
-  "s_branch L_popchk_" X "_%=\n"                                                 \
+  "s_branch L_next_" X "_%=\n"                                                   \
   "L_root_" X "_%=:\n"                                                           \
-  "s_brev_b32 %[top], -2\n"                                                      \
-  "s_branch L_popchk_" X "_%=\n"
+  "s_mov_b32 %[top], %[nn]\n"                                                    \
+  "s_branch L_next_" X "_%=\n"
 
+// Layout A, C, B: the continue path of A (to C) and of C (to B) falls through.
 #define NBX_WALK_FAST                                                                       \
   "s_mov_b64 s[56:57], %[M]\n"                                                              \
   "s_min_i32 s66, %[v], %[nn]\n"                                                            \
@@ -1680,13 +1694,16 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_lshl_b32 s66, s66, 6\n"                                                                \
   "s_load_dwordx8 s[20:27], %[rb], s66\n"                                                   \
   "s_load_dwordx4 s[28:31], %[rb], s66 offset:32\n"                                         \
-  "s_branch L_next_A_%=\n"                                                                  \
-  NBX_WALK_STEP("A", "B", "C", "s20", "s21", "s22", "s23", "s24", "s25", "s26", "s27", "s28", \
-                "s30", "s31", "s[32:39]", "s[40:43]", "s[44:51]", "s[52:55]")               \
-  NBX_WALK_STEP("B", "C", "A", "s32", "s33", "s34", "s35", "s36", "s37", "s38", "s39", "s40", \
-                "s42", "s43", "s[44:51]", "s[52:55]", "s[20:27]", "s[28:31]")               \
-  NBX_WALK_STEP("C", "A", "B", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", \
-                "s54", "s55", "s[20:27]", "s[28:31]", "s[32:39]", "s[40:43]")               \
+  NBX_WALK_MAIN("A", "B", "C", "s20", "s21", "s22", "s23", "s24", "s25", "s26", "s27", "s28", \
+                "s30", "s31", "s[32:39]", "s[40:43]", "s[44:51]", "s[52:55]", "")           \
+  NBX_WALK_MAIN("C", "A", "B", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", \
+                "s54", "s55", "s[20:27]", "s[28:31]", "s[32:39]", "s[40:43]", "")           \
+  NBX_WALK_MAIN("B", "C", "A", "s32", "s33", "s34", "s35", "s36", "s37", "s38", "s39", "s40", \
+                "s42", "s43", "s[44:51]", "s[52:55]", "s[20:27]", "s[28:31]",             \
+                "s_branch L_next_A_%=\n")                                                   \
+  NBX_WALK_OOL("A", "B", "s27", "s30")                                                      \
+  NBX_WALK_OOL("C", "A", "s51", "s54")                                                      \
+  NBX_WALK_OOL("B", "C", "s39", "s42")                                                      \
   "L_exit_%=:\n"                                                                            \
   "s_waitcnt lgkmcnt(0)\n"                                                                  \
   "s_mov_b64 %[M], s[56:57]\n"
@@ -1779,7 +1796,7 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
   int keep_at = 0;              // lane k: position of remembered entry k
   uint32_t keep_lo = 0, keep_hi = 0;  // lane k: its mask
   int depth = 0;
-  int top = INT_MAX;  // position of the innermost remembered entry (INT_MAX: none)
+  int top = nn;  // position of the innermost remembered entry (nn: none)
   int v = 0;
   for (;;) {
     // The compiler cannot prove the walk state uniform after a C step, nor the
@@ -1836,7 +1853,7 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
             [d] "s"(__builtin_amdgcn_readfirstlane(depth))
           : "m0");
       ++depth;
-      top = after;
+      top = min(after, nn);
       M = om;
       v = v + 1;
     } else {
