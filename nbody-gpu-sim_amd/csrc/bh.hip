@@ -1867,23 +1867,30 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
   }
 }
 
-// Persistent waves, one per workgroup, one grid of resident waves. Tasks are
-// 64 consecutive particles of the task order (task_keys: the tree's own order
-// re-sorted by Hilbert index), so bodies of one cell share a wave however many
-// share a Morton code, and a task's bodies are close together. Workgroups are dealt to the 8 XCDs round-robin; each
-// XCD owns one contiguous eighth of the tasks, so that its L2 holds the part
-// of the tree its waves walk, and its waves take that eighth's tasks in order
-// from a counter. A walk is bound by load latency, so a slow task cannot be
-// sped up once it runs alone: taking tasks as slots free up (not two fixed
-// tasks per slot) keeps the tail to about one task. Every wave exits once its
+// Persistent workgroups of kTravWaves waves, one grid of resident waves.
+// Tasks are 64 consecutive particles of the task order (task_keys: the tree's
+// own order re-sorted by Hilbert index), so bodies of one cell share a wave
+// however many share a Morton code, and a task's bodies are close together.
+// Workgroups are dealt to the 8 XCDs round-robin; each XCD owns one contiguous
+// eighth of the tasks, so that its L2 holds the part of the tree its waves
+// walk, and its workgroups take that eighth's tasks in order from a counter,
+// kTravWaves adjacent tasks per round, one per wave. The waves of a workgroup
+// share a CU and its scalar cache, and adjacent tasks walk mostly the same
+// records at about the same time: 8-wave rounds raise the scalar-cache hits
+// from 8 to 12 % and save 2-3 % of the traversal, more than the waves lose
+// waiting at the end of a round (1-wave workgroups taking one task at a time
+// keep the tail shortest but share nothing). Every workgroup exits once its
 // XCD's counter is past the range. (Taking the other XCDs' leftovers was
 // slower: those walks miss the L2 that holds their part of the tree.)
-__global__ __launch_bounds__(64) void bh_traverse(
+constexpr int kTravWaves = 8;
+__global__ __launch_bounds__(64 * kTravWaves) void bh_traverse(
     const float4* __restrict__ rec, const BhState* __restrict__ st,
     const int32_t* __restrict__ task_order, int* queue, int ntasks, const float* __restrict__ x, const float* __restrict__ y,
     const float* __restrict__ z, const float* __restrict__ m, int n, float theta2, float G,
     float eps2, float* ax, float* ay, float* az) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  __shared__ int round_base;
   // The walk's asm clobbers s20-s71, which keeps the kernel at 78 SGPRs: on
   // gfx950, 8 waves per SIMD stay resident up to 78 SGPRs and only 7 from 82
   // on (measured with spinning waves), although the compiler's occupancy and
@@ -1913,12 +1920,15 @@ __global__ __launch_bounds__(64) void bh_traverse(
   int* ctr = reinterpret_cast<int*>(vreg(queue + 16 * xq));
   asm volatile("" : "+v"(theta2), "+v"(G), "+v"(eps2));  // operands of vector instructions only
   for (;;) {
-    int t = 0;
-    if (lane == 0) t = atomicAdd(ctr, 1);
-    t = __builtin_amdgcn_readfirstlane(t);
-    if (t >= __builtin_amdgcn_readfirstlane(len)) break;
-    walk_task(__builtin_amdgcn_readfirstlane(b) + t, lane, rec, st, order, vx, vy, vz, vm, n,
-              theta2, G, eps2, vax, vay, vaz);
+    if (threadIdx.x == 0) round_base = atomicAdd(ctr, kTravWaves);
+    __syncthreads();
+    const int c = __builtin_amdgcn_readfirstlane(round_base);
+    __syncthreads();  // every wave has read round_base before the next round writes it
+    if (c >= __builtin_amdgcn_readfirstlane(len)) break;  // the same for the whole workgroup
+    const int t = c + wave;
+    if (t < __builtin_amdgcn_readfirstlane(len))
+      walk_task(__builtin_amdgcn_readfirstlane(b) + t, lane, rec, st, order, vx, vy, vz, vm, n,
+                theta2, G, eps2, vax, vay, vaz);
   }
 }
 
@@ -2020,7 +2030,7 @@ struct nbx_bh {
   bool nodes_pending = false;  // the reference node array is emitted on demand
   int* trav_queue = nullptr;     // 8 task counters, 64 bytes apart
   unsigned long long* census = nullptr;  // nbx_bh_walk_census counters
-  int trav_waves = 0;            // resident traversal waves on this device
+  int trav_groups = 0;           // resident traversal workgroups on this device
   int force_seq = 0;
   bool timing = false;
   hipEvent_t ev[10] = {};
@@ -2145,10 +2155,10 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(bh_traverse), 64, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(bh_traverse), 64 * kTravWaves, 0) !=
             hipSuccess)
       rc = fail(NBX_ERR_HIP, "Barnes-Hut: occupancy query failed");
-    t->trav_waves = std::max(8, cus * per_cu);
+    t->trav_groups = std::max(8, cus * per_cu);
   }
   if (rc == NBX_OK) rc = t->sort_particles.init((int)P);
   if (rc == NBX_OK) rc = t->sort_nodes.init((int)NN);
@@ -2276,9 +2286,10 @@ int nbx_bh_forces(nbx_bh* t, nbx_particles* d, float theta, float G, float eps, 
   const float theta2 = theta * theta;
   NBX_TRY(ensure_prep(t, theta2, G, s));
   const int ntasks = (int)div_up(n, 64);
-  const int waves = std::max(8, std::min(ntasks, t->trav_waves) & ~7);
+  const int groups =
+      std::max(8, std::min((ntasks + kTravWaves - 1) / kTravWaves, t->trav_groups) & ~7);
   hipLaunchKernelGGL(trav_queue_reset, dim3(1), dim3(64), 0, s, t->trav_queue);
-  hipLaunchKernelGGL(bh_traverse, dim3(waves), dim3(64), 0, s, t->trav_rec, t->st,
+  hipLaunchKernelGGL(bh_traverse, dim3(groups), dim3(64 * kTravWaves), 0, s, t->trav_rec, t->st,
                      t->task_order, t->trav_queue, ntasks, d->pos_x, d->pos_y, d->pos_z, d->mass, n,
                      theta2, G, eps * eps, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
