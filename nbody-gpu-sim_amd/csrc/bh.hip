@@ -173,13 +173,38 @@ __global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int
 
 // Also packs each particle as (x, y, z, m), so that the paths pass gathers one
 // 16-byte record per particle through the sort permutation instead of four.
+// Skilling's Hilbert index (AIP Conf. Proc. 707, 381 (2004)) of a cell of the
+// 1024^3 grid: the transpose form of the three 10-bit coordinates, then its
+// bits interleaved, the first axis most significant.
+__device__ __forceinline__ uint32_t hilbert_index(uint32_t x, uint32_t y, uint32_t z) {
+  uint32_t X[3] = {x, y, z};
+  for (uint32_t q = 1u << 9; q > 1; q >>= 1) {
+    const uint32_t p = q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & q) {
+        X[0] ^= p;
+      } else {
+        const uint32_t t = (X[0] ^ X[i]) & p;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  uint32_t t = 0;
+  for (uint32_t q = 1u << 9; q > 1; q >>= 1)
+    if (X[2] & q) t ^= q - 1;
+  return (expand_bits(X[0] ^ t) << 2) | (expand_bits(X[1] ^ t) << 1) | expand_bits(X[2] ^ t);
+}
+
 __global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restrict__ x,
                                                           const float* __restrict__ y,
                                                           const float* __restrict__ z,
                                                           const float* __restrict__ m, int n,
                                                           const BhState* __restrict__ st,
                                                           uint32_t* codes, int32_t* idx,
-                                                          float4* packed) {
+                                                          float4* packed, uint32_t* hkey) {
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   const float px = x[i], py = y[i], pz = z[i];
@@ -187,9 +212,10 @@ __global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restric
   const float nx = mul_rn(sub_rn(px, st->bmin[0]), st->inv[0]);
   const float ny = mul_rn(sub_rn(py, st->bmin[1]), st->inv[1]);
   const float nz = mul_rn(sub_rn(pz, st->bmin[2]), st->inv[2]);
-  codes[i] = (expand_bits(quantise(nx)) << 2) | (expand_bits(quantise(ny)) << 1) |
-             expand_bits(quantise(nz));
+  const uint32_t qx = quantise(nx), qy = quantise(ny), qz = quantise(nz);
+  codes[i] = (expand_bits(qx) << 2) | (expand_bits(qy) << 1) | expand_bits(qz);
   idx[i] = i;
+  hkey[i] = hilbert_index(qx, qy, qz);  // the traversal's task order (task_keys)
 }
 
 // ---------------------------------------------------------------- 4: paths
@@ -1467,54 +1493,24 @@ __global__ void seq_preorder(const BhState* __restrict__ st, const RefNode* __re
 
 // ---------------------------------------------------------------- 6b: task order
 
-// Skilling's Hilbert index (AIP Conf. Proc. 707, 381 (2004)) of a cell of the
-// 1024^3 grid: the transpose form of the three 10-bit coordinates, then its
-// bits interleaved, the first axis most significant.
-__device__ __forceinline__ uint32_t hilbert_index(uint32_t x, uint32_t y, uint32_t z) {
-  uint32_t X[3] = {x, y, z};
-  for (uint32_t q = 1u << 9; q > 1; q >>= 1) {
-    const uint32_t p = q - 1;
-    for (int i = 0; i < 3; ++i) {
-      if (X[i] & q) {
-        X[0] ^= p;
-      } else {
-        const uint32_t t = (X[0] ^ X[i]) & p;
-        X[0] ^= t;
-        X[i] ^= t;
-      }
-    }
-  }
-  X[1] ^= X[0];
-  X[2] ^= X[1];
-  uint32_t t = 0;
-  for (uint32_t q = 1u << 9; q > 1; q >>= 1)
-    if (X[2] & q) t ^= q - 1;
-  return (expand_bits(X[0] ^ t) << 2) | (expand_bits(X[1] ^ t) << 1) | expand_bits(X[2] ^ t);
-}
-
 // The order traversal tasks take the particles in (64 consecutive per task):
 // the tree's own order (path order; Morton order after the sequential
 // builder), sorted stably by the Hilbert index of each particle's Morton cell
-// (the same 1024^3 grid as its Morton code). Morton and path order jump
+// (the same 1024^3 grid as its Morton code; morton_kernel computes both). Morton and path order jump
 // between distant cells at octant boundaries, so some 64-particle tasks span
 // two far-apart groups and their walk covers both groups' trees; consecutive
 // Hilbert cells are always adjacent, so tasks stay compact (7 % fewer walk
 // steps on the two-Plummer million). Any order gives the same forces: each
 // lane sums its interactions in the records' order whatever task it is in.
 __global__ __launch_bounds__(kThreads) void task_keys(const BhState* __restrict__ st,
-                                                      const float* __restrict__ x,
-                                                      const float* __restrict__ y,
-                                                      const float* __restrict__ z,
+                                                      const uint32_t* __restrict__ hkey,
                                                       const int32_t* __restrict__ morton_order,
                                                       const int32_t* __restrict__ path_order,
                                                       uint32_t* key, int32_t* order) {
   const int r = blockIdx.x * kThreads + threadIdx.x;
   if (r >= st->n) return;
   const int p = st->use_seq ? morton_order[r] : path_order[r];
-  const uint32_t qx = quantise(mul_rn(sub_rn(x[p], st->bmin[0]), st->inv[0]));
-  const uint32_t qy = quantise(mul_rn(sub_rn(y[p], st->bmin[1]), st->inv[1]));
-  const uint32_t qz = quantise(mul_rn(sub_rn(z[p], st->bmin[2]), st->inv[2]));
-  key[r] = hilbert_index(qx, qy, qz);
+  key[r] = hkey[p];
   order[r] = p;
 }
 
@@ -2019,7 +2015,8 @@ struct nbx_bh {
   int32_t* final_of = nullptr;
   nbx::RefNode* nodes = nullptr;
   float4* trav_rec = nullptr;  // 64 bytes per cell, traversal order
-  uint32_t* task_key = nullptr;   // Hilbert index of each particle's Morton cell
+  uint32_t* hkey = nullptr;       // Hilbert index of each particle's Morton cell
+  uint32_t* task_key = nullptr;   // the same, by position of the task order (sort keys)
   int32_t* task_order = nullptr;  // particles in the order traversal tasks take them
   nbx::RadixSort sort_particles;
   nbx::RadixSort sort_nodes;
@@ -2147,6 +2144,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->final_of, NN);
   A(&t->nodes, NN);
   A(&t->trav_rec, (NN + 1) * 4);  // + the pad record the walk's prefetch may read
+  A(&t->hkey, P);
   A(&t->task_key, P);
   A(&t->task_order, P);
   A(&t->trav_queue, 8 * 16);
@@ -2198,7 +2196,7 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   // the packed particles borrow the path-order buffer, which path_lcp_kernel fills later
   float4* packed = reinterpret_cast<float4*>(t->O.spath);
   hipLaunchKernelGGL(morton_kernel, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, d->pos_x,
-                     d->pos_y, d->pos_z, d->mass, n, t->st, t->codes, t->idx, packed);
+                     d->pos_y, d->pos_z, d->mass, n, t->st, t->codes, t->idx, packed, t->hkey);
   NBX_LAUNCHED();
   mark(t, 2, s);
   NBX_TRY(t->sort_particles.sort(t->codes, t->idx, &t->st->n, 30, s));
@@ -2252,8 +2250,8 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
                      d->pos_z, d->mass, t->nodes, t->max_nodes);
   hipLaunchKernelGGL(seq_preorder, dim3(1), dim3(64), 0, s, t->st, t->nodes, t->T.pre, t->T.size,
                      t->T.k, t->trav_rec, t->prep);
-  hipLaunchKernelGGL(task_keys, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->st, d->pos_x,
-                     d->pos_y, d->pos_z, t->idx, t->O.ppart, t->task_key, t->task_order);
+  hipLaunchKernelGGL(task_keys, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->st, t->hkey,
+                     t->idx, t->O.ppart, t->task_key, t->task_order);
   NBX_LAUNCHED();
   NBX_TRY(t->sort_particles.sort(t->task_key, t->task_order, &t->st->n, 30, s));
   mark(t, 7, s);
