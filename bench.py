@@ -337,9 +337,10 @@ def run_bh(nbx, args):
         "census": census, "waves": waves,
         "steps_per_wave": census["wave_steps"] / waves,
         "lane_utilisation": census["active_lane_steps"] / (64.0 * census["wave_steps"]),
-        "note": "neither roof binds: each wave step issues about 25 VALU and 15 SALU "
-                "instructions for 64 lanes, of which about 61 % are active, and needs the record "
-                "the previous step chose; counters put the SIMDs' vector units about 86 % busy"}
+        "note": "neither roof binds: each wave step issues about 22 VALU and 14 SALU "
+                "instructions for 64 lanes, of which about 65 % are active, and needs the record "
+                "the previous step chose; wave wait-state counters put 44 % of the waves' cycles "
+                "on those loads and 32 % on dependencies between their own instructions"}
     log(f"bh_traverse: {trav_ms:.3f} ms, {census['wave_steps']} wave-steps "
         f"({census['wave_steps'] / waves:.0f} per wave), {census['interactions']} interactions; "
         f"{trav_gbs:.0f} GB/s of records, {trav_tf:.2f} TFLOP/s")
