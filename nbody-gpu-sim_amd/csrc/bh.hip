@@ -1581,10 +1581,15 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 // record), s[56:57] M, s[58:59] the lanes inside the cell (then the undecided
 // lanes), s[60:61] the lanes that open it, s[62:63] the lanes that clearly
 // accept it, s[66:69] the lanes that take the force and those that descend,
-// s70 = v + 1, s71 = skip clamped to at least v + 1. s66/s67 hold the two load
+// s70 = v + 1, s71 = skip clamped to [v + 1, nn]. s66/s67 hold the two load
 // offsets until the loads issue (scalar loads read their offsets at issue),
-// s58 the pop lane. s64/s65 stay free, so that the kernel needs no more than
-// 78 SGPRs. Ordering: the result of v_rsq is first read two instructions later; lane sets pass from vector compares to
+// s58 the pop lane. The record after the skip target (the next step's descend
+// target if this step continues) is pulled into the L2 a step ahead by a vector
+// load into one scratch VGPR: it counts on the vector memory counter, so the
+// step's wait for its scalar loads does not wait for it, and the next step's
+// scalar load of it hits the L2 (3 % faster; also loading v + 2 ahead gave only
+// 1.3 %, and the record two continues away, from a skip-of-skip field, was 5 %
+// slower). s64/s65 stay free: the kernel needs no more than 80 SGPRs. Ordering: the result of v_rsq is first read two instructions later; lane sets pass from vector compares to
 // scalar instructions (interlocked) and from scalar instructions to v_cndmask
 // (no hazard); readlane takes its lane from a scalar instruction and writelane
 // from m0, written one instruction before.
@@ -1610,6 +1615,8 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   "s_load_dwordx4 " y4 ", %[rb], s66 offset:32\n"                                \
   "s_load_dwordx8 " z8 ", %[rb], s67\n"                                          \
   "s_load_dwordx4 " z4 ", %[rb], s67 offset:32\n"                                \
+  "v_mov_b32 %[po], s67\n"                                                       \
+  "global_load_dword %[pf], %[po], %[rb] offset:64\n"                            \
   "v_subrev_f32 %[t0], " b0 ", %[xi]\n"                                          \
   "v_subrev_f32 %[t1], " b1 ", %[yi]\n"                                          \
   "v_subrev_f32 %[t2], " b2 ", %[zi]\n"                                          \
@@ -1701,7 +1708,7 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
   NBX_WALK_OOL("C", "A", "s51", "s54")                                                      \
   NBX_WALK_OOL("B", "C", "s39", "s42")                                                      \
   "L_exit_%=:\n"                                                                            \
-  "s_waitcnt lgkmcnt(0)\n"                                                                  \
+  "s_waitcnt vmcnt(0) lgkmcnt(0)\n"                                                         \
   "s_mov_b64 %[M], s[56:57]\n"
 
 // The decisions of one walk step at record v for the wave's lanes M (the C
@@ -1793,6 +1800,7 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
   uint32_t keep_lo = 0, keep_hi = 0;  // lane k: its mask
   int depth = 0;
   int top = nn;  // position of the innermost remembered entry (nn: none)
+  uint32_t pf = 0;  // the asm's L2 prefetches land here (drained before the asm exits)
   int v = 0;
   for (;;) {
     // The compiler cannot prove the walk state uniform after a C step, nor the
@@ -1803,12 +1811,13 @@ __device__ __forceinline__ void walk_task(int blk, int lane, const float4* __res
     M = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(M >> 32)) << 32) |
         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)M);
     float t0, t1, t2, dxa, dya, dza, d2a, inva;
+    uint32_t po;
     asm volatile(NBX_WALK_FAST
                  : [v] "+s"(v), [top] "+s"(top), [depth] "+s"(depth), [M] "+s"(M),
                    [sx] "+v"(sx), [sy] "+v"(sy), [sz] "+v"(sz), [kat] "+v"(keep_at),
                    [klo] "+v"(keep_lo), [khi] "+v"(keep_hi), [t0] "=&v"(t0), [t1] "=&v"(t1),
                    [t2] "=&v"(t2), [dx] "=&v"(dxa), [dy] "=&v"(dya), [dz] "=&v"(dza),
-                   [d2] "=&v"(d2a), [inv] "=&v"(inva)
+                   [d2] "=&v"(d2a), [inv] "=&v"(inva), [pf] "+v"(pf), [po] "=&v"(po)
                  : [rb] "s"(rec), [nn] "s"(nn), [eps2] "v"(eps2), [xi] "v"(xi), [yi] "v"(yi),
                    [zi] "v"(zi), [mi] "v"(mi)
                  : "s20", "s21", "s22", "s23", "s24", "s25", "s26", "s27", "s28", "s29", "s30",
@@ -2143,7 +2152,7 @@ int nbx_bh_create(size_t max_particles, nbx_bh** out) {
   A(&t->nvals, NN);
   A(&t->final_of, NN);
   A(&t->nodes, NN);
-  A(&t->trav_rec, (NN + 1) * 4);  // + the pad record the walk's prefetch may read
+  A(&t->trav_rec, (NN + 2) * 4);  // + two pad records the walk's loads ahead may read
   A(&t->hkey, P);
   A(&t->task_key, P);
   A(&t->task_order, P);
