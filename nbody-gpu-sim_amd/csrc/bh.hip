@@ -1545,10 +1545,11 @@ __global__ __launch_bounds__(kThreads) void trav_prep(float4* rec, const BhState
 }
 
 // Wave-coherent traversal over the preorder layout. A wave holds 64
-// consecutive particles in Morton order and walks the union of their trees in
-// the reference's visiting order (:160-241). State: the current record v and
-// the mask M of lanes that need it. Each lane takes exactly the reference's
-// accept/open decisions in the same IEEE arithmetic. If some lanes open v, the
+// consecutive particles of the task order (task_keys) and walks the union of
+// their trees in the reference's visiting order (:160-241). State: the current
+// record v and the mask M of lanes that need it. Each lane takes exactly the
+// reference's accept/open decisions (in the reference's IEEE arithmetic where a
+// decision is close; see trav_prep). If some lanes open v, the
 // walk descends to v + 1 (its first child in this order) with those lanes,
 // remembering (skip[v], M); otherwise it continues at skip[v]. Arriving at a
 // remembered position restores its mask. Positions only grow (the skip is
