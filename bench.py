@@ -269,8 +269,18 @@ def run_cpu_path(nbx, args):
 
 # ------------------------------------------------------------------ Barnes-Hut
 
-# algorithmic bytes per particle of the HBM-bound build phases
-BH_BYTES = {"bbox": 12.0, "morton": 20.0, "paths": 4.0 + 12.0 + 16.0}
+# algorithmic bytes per particle of the HBM-bound build phases:
+#   bbox    reads x, y, z
+#   morton  reads x, y, z, m; writes the Morton code, the index, the Hilbert
+#           index and the packed body (x, y, z, m)
+#   sort    stable radix sort of (code, index): 4 passes of 8 bits, each reading
+#           the keys to count, then reading and writing keys and values
+#   paths   reads the sorted index and the packed body at it; writes the body in
+#           sorted order, the octant path (16 B), both path-sort keys and the rank
+#   export  the traversal's task order: gathers an order index and a Hilbert
+#           index, writes both, and sorts them as above
+BH_BYTES = {"bbox": 12.0, "morton": 16.0 + 28.0, "sort": 4 * 20.0,
+            "paths": 4.0 + 16.0 + 16.0 + 16.0 + 12.0, "export": 16.0 + 4 * 20.0}
 # bytes of one traversal record (centre of mass and G m, centre and half size,
 # skip and thresholds, mass); the walk visits one per step
 BH_RECORD_BYTES = 64
@@ -350,7 +360,8 @@ def run_bh(nbx, args):
                        "relative velocity 0.5 along x",
            "phases_ms": phases, "nodes": nodes, "max_depth": depth,
            "builder": t.build_info()["builder"],
-           "achieved_GBps": gbs, "hbm_peak_GBps": PEAK_HBM_GBS, "roofline": roofline}
+           "achieved_GBps": gbs, "build_bytes_per_particle": BH_BYTES,
+           "hbm_peak_GBps": PEAK_HBM_GBS, "roofline": roofline}
     if not args.no_cpu_baseline:
         orc = oracle()
         hn = d.download()
