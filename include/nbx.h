@@ -205,6 +205,12 @@ int nbx_bh_set_timing(nbx_bh* t, int enable);
 /* Builder selection for cross-checks: 0 = automatic (parallel unless the node
  * budget could fire), 1 = always the exact sequential builder. */
 int nbx_bh_set_builder(nbx_bh* t, int mode);
+/* Task order of the traversal for cross-checks (no reference counterpart;
+ * applies from the next build): 0 = 64-particle tasks in Hilbert order (the
+ * default), 1 = in the tree's own order (path order; Morton order after the
+ * sequential builder). Forces are the same bit for bit either way: only the
+ * number of records the waves visit changes (nbx_bh_walk_census). */
+int nbx_bh_set_task_order(nbx_bh* t, int mode);
 /* Diagnostics (no reference counterpart): the walk nbx_bh_forces makes for
  * (theta, G, eps) on the last build, counted instead of summed. counts[0] the
  * records the waves visit (one per step of a 64-particle wave's walk),

@@ -2039,6 +2039,7 @@ struct nbx_bh {
   unsigned long long* census = nullptr;  // nbx_bh_walk_census counters
   int trav_groups = 0;           // resident traversal workgroups on this device
   int force_seq = 0;
+  int task_tree_order = 0;  // 1: tasks in the tree's own order (no Hilbert sort)
   bool timing = false;
   hipEvent_t ev[10] = {};
   double phase_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -2263,7 +2264,7 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   hipLaunchKernelGGL(task_keys, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->st, t->hkey,
                      t->idx, t->O.ppart, t->task_key, t->task_order);
   NBX_LAUNCHED();
-  NBX_TRY(t->sort_particles.sort(t->task_key, t->task_order, &t->st->n, 30, s));
+  if (!t->task_tree_order) NBX_TRY(t->sort_particles.sort(t->task_key, t->task_order, &t->st->n, 30, s));
   mark(t, 7, s);
   t->built = true;
   t->built_n = d->count;
@@ -2414,6 +2415,13 @@ int nbx_bh_walk_census(nbx_bh* t, const nbx_particles* d, float theta, float G, 
 int nbx_bh_set_timing(nbx_bh* t, int enable) {
   if (!t) return fail(NBX_ERR_INVALID, "nbx_bh_set_timing: null tree");
   t->timing = enable != 0;
+  return NBX_OK;
+}
+
+int nbx_bh_set_task_order(nbx_bh* t, int mode) {
+  if (!t || mode < 0 || mode > 1)
+    return fail(NBX_ERR_INVALID, "nbx_bh_set_task_order: bad argument");
+  t->task_tree_order = mode;
   return NBX_OK;
 }
 

@@ -116,6 +116,7 @@ _sig("nbx_bh_copy_sort", C.c_int, VP, F32P, U32P, I32P, C.c_size_t)
 _sig("nbx_bh_build_info", C.c_int, VP, I32P, I32P)
 _sig("nbx_bh_set_timing", C.c_int, VP, C.c_int)
 _sig("nbx_bh_set_builder", C.c_int, VP, C.c_int)
+_sig("nbx_bh_set_task_order", C.c_int, VP, C.c_int)
 _sig("nbx_bh_phase_ms", C.c_int, VP, F64P)
 _sig("nbx_bh_walk_census", C.c_int, VP, PP, C.c_float, C.c_float, C.c_float, I64P)
 _sig("nbx_grid_create", C.c_int, C.c_size_t, C.c_float, C.POINTER(VP))
@@ -165,7 +166,7 @@ EXPORTED = [
     "nbx_store_old_accelerations", "nbx_update_positions", "nbx_update_velocities",
     "nbx_verlet_drift", "nbx_energy", "nbx_bh_create", "nbx_bh_destroy", "nbx_bh_build",
     "nbx_bh_forces", "nbx_bh_node_count", "nbx_bh_copy_nodes", "nbx_bh_copy_sort",
-    "nbx_bh_build_info", "nbx_bh_set_timing", "nbx_bh_set_builder", "nbx_bh_phase_ms",
+    "nbx_bh_build_info", "nbx_bh_set_timing", "nbx_bh_set_builder", "nbx_bh_set_task_order", "nbx_bh_phase_ms",
     "nbx_bh_walk_census",
     "nbx_grid_create", "nbx_grid_destroy", "nbx_grid_build", "nbx_grid_forces", "nbx_grid_info",
     "nbx_grid_copy_cells", "nbx_grid_set_walk",
@@ -463,6 +464,9 @@ class BHTree:
 
     def set_builder(self, mode: int) -> None:
         check(lib.nbx_bh_set_builder(self.h, mode))
+
+    def set_task_order(self, mode: int) -> None:
+        check(lib.nbx_bh_set_task_order(self.h, mode))
 
     def walk_census(self, d: DeviceParticles, theta: float, G: float, eps: float) -> dict:
         c = np.zeros(4, np.int64)

@@ -403,3 +403,34 @@ def test_walk_census_closed_form_at_theta_zero(nbx, orc):
     ref = orc.bh_forces(nodes, h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 0.5, 1.0, 0.01)
     e = np.linalg.norm(d.acc().astype(np.float64) - ref, axis=1) / np.linalg.norm(ref, axis=1)
     assert np.median(e) < 1e-6
+
+
+def test_forces_do_not_depend_on_task_order(nbx, orc):
+    # Tasks of 64 particles in Hilbert order (the default) or in the tree's own
+    # order: every lane sums its interactions in the records' order and every force
+    # term uses the same d2, so the accelerations agree bit for bit; the Hilbert
+    # order's tasks are more compact, so its waves visit fewer records.
+    h = nbx.gen_two_plummer(1 << 17, seed=21)
+    d = nbx.DeviceParticles.from_host(h)
+    t = nbx.BHTree(1 << 17)
+    res = {}
+    for mode in (1, 0):
+        t.set_task_order(mode)
+        t.build(d)
+        t.forces(d, 0.5, 1.0, 0.01)
+        nbx.synchronize()
+        res[mode] = (d.acc().copy(), t.walk_census(d, 0.5, 1.0, 0.01))
+    assert np.array_equal(res[0][0], res[1][0])
+    c0, c1 = res[0][1], res[1][1]
+    assert c0["interactions"] == c1["interactions"] and c0["openings"] == c1["openings"]
+    assert c0["active_lane_steps"] == c1["active_lane_steps"]
+    assert c0["wave_steps"] < c1["wave_steps"]
+    ref = orc.bh_pipeline(h)
+    sel = np.random.default_rng(3).choice(1 << 17, 512, replace=False)
+    sel = np.sort(sel)
+    a = orc.bh_forces(ref["nodes"], h["pos_x"], h["pos_y"], h["pos_z"], h["mass"], 0.5, 1.0, 0.01,
+                      targets=sel)
+    e = np.linalg.norm(res[0][0][sel].astype(np.float64) - a, axis=1) / np.linalg.norm(a, axis=1)
+    assert np.median(e) < 1e-6
+    with pytest.raises(nbx.NbxError):
+        t.set_task_order(2)
