@@ -111,7 +111,10 @@ static int one_sided_default(size_t n) {
 }
 
 static int default_variant(size_t n) {
-  if (n >= (size_t(1) << 16)) return kNumOneSided + 1;  // sym_ipl4_ws4
+  // sym_ipl6_ws4: 1,536-body super-blocks, 4 waves per SIMD (6 targets per lane
+  // take 1.5x the pairs of 4 per rotation step; 8 need 163 VGPRs, 3 waves);
+  // 1.5-6 % faster than sym_ipl4_ws4 from 2^16 to 2^20 bodies
+  if (n >= (size_t(1) << 16)) return kNumOneSided + 8;
   return one_sided_default(n);
 }
 
@@ -140,13 +143,19 @@ int direct_launch(const nbx_particles* d, size_t i0, size_t ni, float G, float e
 
 int direct_sym_block_of(int variant) { return kVariants[variant].sym_block; }
 
-// Workspace sizes for up to max_particles bodies: the FP64 carry covers every
-// super-block size (all divide kSymMaxBlock); the partials default to the
-// largest one-round need over the symmetric shapes, capped at kDefaultPartialCap,
-// and are at least the three slots of a one-distance round.
+// Workspace sizes for up to max_particles bodies: the FP64 carry covers the
+// padded length of every symmetric shape (n rounded up to its super-block);
+// the partials default to the largest one-round need over the symmetric
+// shapes, capped at kDefaultPartialCap, and are at least the three slots of a
+// one-distance round.
 constexpr size_t kDefaultPartialCap = size_t(1) << 30;
 
-static size_t padded(size_t n) { return (n + kSymMaxBlock - 1) / kSymMaxBlock * kSymMaxBlock; }
+static size_t padded(size_t n) {
+  size_t p = n;
+  for (const DirectVariant& V : kVariants)
+    if (V.sym_block) p = std::max(p, (n + V.sym_block - 1) / V.sym_block * V.sym_block);
+  return p;
+}
 static size_t min_partial_bytes(size_t n) { return 3 * 3 * padded(n) * sizeof(float); }
 
 int direct_ws_create(size_t max_particles, size_t scratch_bytes, nbx_direct** out) {

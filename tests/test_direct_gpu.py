@@ -309,7 +309,9 @@ def test_sym_rounds_bit_identical(nbx, blocks):
             continue
         h = nbx.gen_uniform_sphere(n, seed=v)
         whole = run_forces(nbx, h, 1.0, 0.01, variant=v)
-        npad = -(-n // 2048) * 2048
+        # the workspace's padded length: n rounded up to each symmetric shape's
+        # super-block, the largest of them
+        npad = max(-(-n // sym_block(nm)) * sym_block(nm) for _, nm in sym_variants(nbx))
         for slots in (3, 5, 8):
             ws = nbx.DirectWorkspace(n, slots * 3 * npad * 4)
             nbx.check(nbx.lib.nbx_direct_set_variant(v))
