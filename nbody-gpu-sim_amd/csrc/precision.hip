@@ -317,8 +317,10 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
           UNI ? ds_scale(sh[q][a], sl[q][a], mref) : add_rn(sh[q][a], sl[q][a]);
 }
 
+// At least 4 waves per SIMD: unbounded, the unchecked form takes 129 VGPRs (3
+// waves); held to 128 it spills 2 outside its inner loop and runs 2.7 % faster.
 template <int IPL, int W, bool CHECKED>
-__global__ __launch_bounds__(64 * W) void ds_sym_kernel(
+__global__ __launch_bounds__(64 * W, 4) void ds_sym_kernel(
     const float* __restrict__ xh, const float* __restrict__ yh, const float* __restrict__ zh,
     const float* __restrict__ xl, const float* __restrict__ yl, const float* __restrict__ zl,
     const float* __restrict__ m, SymGeom g, float eps2) {
