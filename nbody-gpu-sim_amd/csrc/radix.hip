@@ -7,21 +7,53 @@ namespace {
 
 constexpr int kBins = 256;
 constexpr int kThreads = RadixSort::kWaves * 64;
+static_assert(kThreads >= kBins, "one thread per digit in the digit-wise steps");
 
 __global__ __launch_bounds__(kThreads) void radix_count(const uint32_t* __restrict__ keys,
                                                         const int32_t* __restrict__ n_dev,
                                                         int shift, uint32_t* table, int tiles) {
+  constexpr int PER = RadixSort::kTile / kThreads;
   __shared__ uint32_t hist[kBins];
   const int n = *n_dev;
-  hist[threadIdx.x] = 0;
-  __syncthreads();
+  if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
   const int base = blockIdx.x * RadixSort::kTile;
-  for (int k = threadIdx.x; k < RadixSort::kTile; k += kThreads) {
-    const int i = base + k;
-    if (i < n) atomicAdd(&hist[(keys[i] >> shift) & 0xFFu], 1u);
+  uint32_t k[PER];
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = base + r * kThreads + threadIdx.x;
+    k[r] = i < n ? keys[i] : 0u;
   }
   __syncthreads();
-  table[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+  // the wave's keys that share the first valid lane's digit are counted by one
+  // lane (skewed digit distributions would serialise on one LDS address)
+#pragma unroll
+  for (int r = 0; r < PER; ++r) {
+    const int i = base + r * kThreads + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t dg = (k[r] >> shift) & (kBins - 1);
+    const uint64_t vm = __ballot(valid);
+    if (vm) {
+      const int fl = (int)__builtin_ctzll(vm);
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readlane((int)dg, fl);
+      const uint64_t same = __ballot(valid && dg == d0);
+      if ((threadIdx.x & 63) == fl) atomicAdd(&hist[d0], (uint32_t)__popcll(same));
+      else if (valid && dg != d0) atomicAdd(&hist[dg], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kBins) table[threadIdx.x * tiles + blockIdx.x] = hist[threadIdx.x];
+}
+
+// Inclusive scan over the wave by DPP: row shifts 1, 2, 4, 8 within each row
+// of 16 lanes, then row broadcasts of lanes 15 and 31.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+  return x;
 }
 
 // Exclusive scan of each digit's row of the digit-major table over the tiles,
@@ -36,11 +68,7 @@ __global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* table, int tile
   for (int base = 0; base < tiles; base += 256) {
     const int i = base + threadIdx.x;
     const uint32_t v = i < tiles ? row[i] : 0u;
-    uint32_t incl = v;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint32_t t = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += t;
-    }
+    const uint32_t incl = wave_incl_scan(v);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     uint32_t before = carry, tot = 0;
@@ -59,11 +87,7 @@ __global__ __launch_bounds__(256) void radix_scan_rows(uint32_t* table, int tile
 __device__ int block_scan_u32(uint32_t v, int* total) {
   __shared__ uint32_t ws[kThreads / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t incl = v;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t t = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += t;
-  }
+  const uint32_t incl = wave_incl_scan(v);
   if (lane == 63) ws[w] = incl;
   __syncthreads();
   uint32_t before = 0, tot = 0;
@@ -105,6 +129,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter(
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   for (int k = threadIdx.x; k < W * kBins; k += kThreads) (&wave_hist[0][0])[k] = 0;
+  // this tile's digit offsets and the digit totals, loaded before the ranking
+  const bool isd = threadIdx.x < kBins;
+  const uint32_t my_row_total = isd ? row_total[threadIdx.x] : 0u;
+  const uint32_t my_table = isd ? table[threadIdx.x * tiles + blockIdx.x] : 0u;
 
   uint32_t key[R];
   int32_t val[R];
@@ -118,50 +146,55 @@ __global__ __launch_bounds__(kThreads) void radix_scatter(
   const uint32_t dmask = (1u << nbits) - 1u;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __syncthreads();
-  // pass 1: digit counts of this wave's sub-tile
+  // pass 1: digit counts of this wave's sub-tile; pre[r]: keys of the same
+  // digit before this one in the wave's sub-tile
+  uint32_t pre[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const bool valid = wbase + r * 64 + lane < n;
     const uint32_t dg = (key[r] >> shift) & dmask;
     const uint64_t m = match_digit(dg, valid, nbits);
-    if (valid && (m & lt) == 0) wave_hist[w][dg] += (uint32_t)__popcll(m);
+    const uint32_t old = valid ? wave_hist[w][dg] : 0u;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (m & lt) == 0) wave_hist[w][dg] = old + (uint32_t)__popcll(m);
+    __builtin_amdgcn_wave_barrier();
+    pre[r] = old + (uint32_t)__popcll(m & lt);
   }
   __syncthreads();
   // tile-local digit starts (exclusive scan of the tile's digit counts), each
   // wave's first slot per digit, and the tile's output position per digit
   {
-    const int d = threadIdx.x;  // kThreads == kBins
+    const int d = threadIdx.x;
     uint32_t tot = 0;
-    for (int q = 0; q < W; ++q) tot += wave_hist[q][d];
+    if (isd)
+      for (int q = 0; q < W; ++q) tot += wave_hist[q][d];
     int total;
     const uint32_t start = (uint32_t)block_scan_u32(tot, &total);
     __syncthreads();  // block_scan_u32's LDS is reused below
-    local_start[d] = start;
     // digit bases: exclusive scan of the row totals (every workgroup does it)
-    const uint32_t dbase = (uint32_t)block_scan_u32(row_total[d], &total);
-    gstart[d] = dbase + table[d * tiles + blockIdx.x];
-    uint32_t run = start;
-    for (int q = 0; q < W; ++q) {
-      const uint32_t c = wave_hist[q][d];
-      wave_hist[q][d] = run;
-      run += c;
+    const uint32_t dbase = (uint32_t)block_scan_u32(my_row_total, &total);
+    if (isd) {
+      local_start[d] = start;
+      gstart[d] = dbase + my_table;
+      uint32_t run = start;
+      for (int q = 0; q < W; ++q) {
+        const uint32_t c = wave_hist[q][d];
+        wave_hist[q][d] = run;
+        run += c;
+      }
     }
   }
   __syncthreads();
-  // pass 2: stable rank of every key in the tile's sorted order, into LDS
+  // pass 2: each key to its slot in the tile's sorted order, in LDS
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const bool valid = wbase + r * 64 + lane < n;
     const uint32_t dg = (key[r] >> shift) & dmask;
-    const uint64_t m = match_digit(dg, valid, nbits);
-    const uint32_t pos = (valid ? wave_hist[w][dg] : 0u) + (uint32_t)__popcll(m & lt);
-    __builtin_amdgcn_wave_barrier();
     if (valid) {
+      const uint32_t pos = wave_hist[w][dg] + pre[r];
       skey[pos] = key[r];
       sval[pos] = val[r];
-      if ((m & lt) == 0) wave_hist[w][dg] += (uint32_t)__popcll(m);
     }
-    __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
   // pass 3: the tile in sorted order, each digit's run to its output range
