@@ -1009,7 +1009,7 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
   __shared__ int s_t1[kLocalCap], s_t2[kLocalCap], s_skip[kLocalCap];
   __shared__ uint32_t s_info[kLocalCap];
   __shared__ int8_t smin[kLocalBlock + 1];  // smin[x - a] = min(lcp[x..b]), -1 at n
-  __shared__ int seg[kThreads];
+  __shared__ int seg[kThreads / 64];  // per-wave minima
   __shared__ uint16_t lvl_list[kLocalCap];  // local internal cells grouped by depth
   __shared__ int lvl_cnt[kMaxDepth + 1], lvl_start[kMaxDepth + 1], lvl_fill[kMaxDepth + 1];
   __shared__ int sh_top;
@@ -1028,7 +1028,16 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
   uint32_t* info = fits ? s_info : T.info + P0;
   if (threadIdx.x <= kMaxDepth) span[threadIdx.x] = -1;
   if (threadIdx.x == 0) sh_top = 0;
-  if (fits)
+  // the boxes of this thread's cells, loaded now and used at the end (written by
+  // cell_emit; a load in the final loop would wait a full memory latency per cell)
+  constexpr int kBoxRounds = kLocalCap / kThreads;
+  float4 box_r[kBoxRounds];
+  if (fits) {
+#pragma unroll
+    for (int k = 0; k < kBoxRounds; ++k) {
+      const int i = threadIdx.x + k * kThreads;
+      box_r[k] = i < ncell ? T.box[P0 + i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     for (int i = threadIdx.x; i < ncell; i += kThreads) {
       const int p = P0 + i;
       const uint32_t u = T.info[p];
@@ -1040,9 +1049,11 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
         s_t2[i] = T.t2[p];
       }
     }
+  }
   // suffix minima of lcp over (a, b], with lcp[b] = -1 when b == n
   {
     const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
     constexpr int kSeg = kLocalBlock / kThreads;
     int v[kSeg];
     int m = 127;
@@ -1053,10 +1064,17 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
       m = min(m, l);
       v[i] = m;
     }
-    seg[t] = m;
+    // minimum over this lane's segment and the later lanes' of the wave
+    int sfx = m;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_down(sfx, off, 64);
+      if (lane + off < 64) sfx = min(sfx, o);
+    }
+    if (lane == 0) seg[w] = sfx;
+    int after = __shfl_down(sfx, 1, 64);
+    if (lane == 63) after = 127;
     __syncthreads();
-    int after = 127;
-    for (int q = t + 1; q < kThreads; ++q) after = min(after, seg[q]);
+    for (int q = w + 1; q < kThreads / 64; ++q) after = min(after, seg[q]);
     for (int i = 0; i < kSeg; ++i) smin[1 + t * kSeg + i] = (int8_t)min(v[i], after);
   }
   __syncthreads();
@@ -1094,12 +1112,15 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
       if (!(u & (kLeafBit | kSpanBit))) atomicAdd(&lvl_cnt[u & 0xFFu], 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int run = 0;
-      for (int d = 0; d <= kMaxDepth; ++d) {
-        lvl_start[d] = run;
-        run += lvl_cnt[d];
+    if (threadIdx.x < 64) {  // exclusive scan of the depth counts (kMaxDepth < 64)
+      const int lane = threadIdx.x;
+      const int c = lane <= kMaxDepth ? lvl_cnt[lane] : 0;
+      int incl = c;
+      for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
       }
+      if (lane <= kMaxDepth) lvl_start[lane] = incl - c;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < ncell; i += kThreads) {
@@ -1127,9 +1148,9 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
   // results of the block's internal cells (not the straddling ones) to global
   // memory: centre of mass, ranks, child table, the children's creation steps,
   // the traversal record
-  for (int i = threadIdx.x; i < ncell; i += kThreads) {
+  auto finish = [&](int i, float4 box) {
     const uint32_t u = info[i];
-    if (u & (kLeafBit | kSpanBit)) continue;
+    if (u & (kLeafBit | kSpanBit)) return;
     const int p = P0 + i;
     const float4 res = com[i];
     const int r2 = t2[i];
@@ -1141,7 +1162,16 @@ __global__ __launch_bounds__(kThreads) void cell_local(BhState* st, PathOrder O,
     int ch[8];
     const int nc = local_children(i, (int)(u & 0xFFu), ncell, P0, info, skip, ch);
     for (int q = 0; q < nc; ++q) T.k[P0 + ch[q]] = max(r2, t1[ch[q]]);
-    write_trav(rec, p, skip[i] - p, res, T.box[p], false, tp);
+    write_trav(rec, p, skip[i] - p, res, box, false, tp);
+  };
+  if (fits) {
+#pragma unroll
+    for (int k = 0; k < kBoxRounds; ++k) {
+      const int i = threadIdx.x + k * kThreads;
+      if (i < ncell) finish(i, box_r[k]);
+    }
+  } else {
+    for (int i = threadIdx.x; i < ncell; i += kThreads) finish(i, T.box[P0 + i]);
   }
   if (!fits) {  // the span marks went to the global info array: clear them
     __syncthreads();
