@@ -149,6 +149,11 @@ __device__ __forceinline__ void sym_tile(const float* __restrict__ x,
   for (int q = 0; q < IPL; ++q) sx[q] = sy[q] = sz[q] = 0.0;
   float nx, ny, nz, nm;
   load_body(x, y, z, m, jbase + lane, g.n, nx, ny, nz, nm);
+  // eps2 as a VGPR operand: on gfx950 an FP32 VALU instruction that reads an
+  // SGPR issues at half the rate of one whose operands are all VGPRs
+  // (measured: v_fmac/v_sub 3.8e13 against 5.3-7.2e13 lane-ops/s).
+  float eps2v = eps2;
+  asm volatile("" : "+v"(eps2v));
   for (int c = 0; c < kChunks; ++c) {
     float xj = nx, yj = ny, zj = nz, mj = nm;
     if (c + 1 < kChunks) load_body(x, y, z, m, jbase + (c + 1) * 64 + lane, g.n, nx, ny, nz, nm);
@@ -161,7 +166,7 @@ __device__ __forceinline__ void sym_tile(const float* __restrict__ x,
       const int q = c - wave * IPL;
       self_q = (q >= 0 && q < IPL) ? q : -1;
     }
-    sym_chunk<IPL, UNI, JSIDE, CHECKED>(src_addr, xj, yj, zj, mj, xi, yi, zi, mi, eps2, self_q,
+    sym_chunk<IPL, UNI, JSIDE, CHECKED>(src_addr, xj, yj, zj, mj, xi, yi, zi, mi, eps2v, self_q,
                                         fx, fy, fz, gx, gy, gz);
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {

@@ -9,12 +9,13 @@
 //           in FP32 (:250-252) and refuses grids above 1e8 cells (:254-260).
 //   assign  cell of each particle: (int)floorf((p - min) / cell) per axis,
 //           clamped to the grid, linear cx + cy*dx + cz*dx*dy (:36-49).
-//   sort    stable radix sort of (key, particle index), key = cell << 6 | the
-//           Morton index of the particle's 4 x 4 x 4 sub-cell (grids above 2^26
-//           cells: the cell alone). Within a cell the order is therefore
-//           spatially compact groups, then index. The reference's atomic
-//           scatter (:72-81) leaves that order to the scheduler; this is one of
-//           its outcomes, and it makes every later sum deterministic.
+//   sort    stable radix sort of (key, particle index), key = cell << 12 | the
+//           Morton index of the particle's sub-cell in a 16 x 16 x 16 grid of
+//           the cell (above 2^20 cells: cell << 6 | a 4 x 4 x 4 one; above 2^26:
+//           the cell alone). Within a cell the order is therefore spatially
+//           compact runs, then index. The reference's atomic scatter (:72-81)
+//           leaves that order to the scheduler; this is one of its outcomes, and
+//           it makes every later sum deterministic.
 //   ranges  cell_start / cell_end of every cell, the exclusive scan of the
 //           per-cell counts (:287-300), from one merge of the sorted keys with
 //           the cell numbers: each thread takes a fixed slice of the merged
@@ -28,14 +29,15 @@
 // linear numbering, so they are one contiguous range of the sorted bodies.
 //   * bodies are gathered once per call into sorted order as float4
 //     (x, y, z, G*m), G*m being the reference's first product (:144), with
-//     the bounding box of every aligned group of 8.
+//     the bounding box of every aligned group of 4.
 //   * a wave takes 64 consecutive sorted targets. When they all lie in one
 //     (cy, cz) row of cells (the common case once cells hold dozens of
 //     bodies), the wave walks the union of the lanes' ranges with
 //     wave-uniform addresses, skipping the groups whose box is provably
 //     beyond the cutoff from the box of its targets (walk_culled): the bodies
-//     arrive through scalar loads and reach every lane as SGPR operands, and a
-//     lane drops the bodies outside its own three cells with a mask.
+//     of two groups at a time arrive through scalar loads and reach every lane
+//     as SGPR operands, and a lane drops the bodies outside its own three cells
+//     with a mask.
 //     Otherwise every lane walks its own ranges with vector loads.
 //   * r^2 is fma(rz, rz, fma(rx, rx, ry*ry)) on both sides (here and in the
 //     oracle), so the set of pairs kept by the cutoff test is exactly the
@@ -59,16 +61,20 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kMergeSpan = 32;             // merged elements per thread in grid_ranges
 constexpr long long kMaxCells = 100000000;  // :257-260
-constexpr int kNB = 8;                      // bodies per batch (and per culling box)
-constexpr int kSubBits = 6;                 // 4 x 4 x 4 sub-cells, Morton order
-constexpr long long kSubMaxCells = 1LL << (32 - kSubBits);  // larger grids: plain cell order
+constexpr int kNB = 8;                      // bodies per batch
+constexpr int kGroup = 4;                   // bodies per culling box
+constexpr int kPer = kNB / kGroup;          // culling groups per batch
+// Order within a cell: the Morton index of a 16 x 16 x 16 sub-cell grid (12 key
+// bits) up to 2^20 cells, of a 4 x 4 x 4 one (6 bits) up to 2^26, none beyond.
+constexpr long long kSub16MaxCells = 1LL << 20;
+constexpr long long kSub4MaxCells = 1LL << 26;
 
 struct GridGeom {
   float lo[3];  // padded lower corner
   float cell;
   int dims[3];
   int total;
-  int shift;  // sort key = cell << shift | sub-cell (shift 0: cell only)
+  int shift;  // sort key = cell << shift | sub-cell (12: 16^3 sub-cells, 6: 4^3, 0: cell only)
 };
 
 // :36-49 and :103-108 (the force kernel repeats the arithmetic and the clamp).
@@ -77,17 +83,18 @@ __device__ __forceinline__ int grid_axis(float p, float lo, float cell, int dim)
   return max(0, min(c, dim - 1));
 }
 
-// Cell and 2-bit sub-cell coordinate on one axis: u = (p - lo) / cell as above,
-// sub = floor(4u) - 4c clamped to 0..3 (4u is exact).
-__device__ __forceinline__ int grid_axis_sub(float p, float lo, float cell, int dim, int& sub) {
+// Cell and sub-cell coordinate on one axis: u = (p - lo) / cell as above,
+// sub = floor(ns u) - ns c clamped to 0..ns-1, ns = 16 or 4 (ns u is exact).
+__device__ __forceinline__ int grid_axis_sub(float p, float lo, float cell, int dim, int ns,
+                                             int& sub) {
   const float u = div_rn(sub_rn(p, lo), cell);
   const int c = max(0, min((int)floorf(u), dim - 1));
-  sub = max(0, min((int)floorf(u * 4.0f) - 4 * c, 3));
+  sub = max(0, min((int)floorf(u * (float)ns) - ns * c, ns - 1));
   return c;
 }
 
-__device__ __forceinline__ uint32_t spread2(int v) {  // bits 1,0 -> 3,0
-  return (uint32_t)((v & 1) | ((v & 2) << 2));
+__device__ __forceinline__ uint32_t spread4(int v) {  // bits 3..0 -> 9, 6, 3, 0
+  return (uint32_t)((v & 1) | ((v & 2) << 2) | ((v & 4) << 4) | ((v & 8) << 6));
 }
 
 __global__ __launch_bounds__(kThreads) void grid_bbox_partial(const float* __restrict__ x,
@@ -107,10 +114,10 @@ __global__ __launch_bounds__(kThreads) void grid_bbox_final(const float* partial
   }
 }
 
-// Cell of each particle (:36-49) and its sort key: the cell, then (shift 6) the
-// Morton index of its 4 x 4 x 4 sub-cell, so that a cell's bodies are ordered in
-// compact spatial groups. The reference's scatter leaves the order within a
-// cell open (:72-81); this order is one of its outcomes.
+// Cell of each particle (:36-49) and its sort key: the cell, then (shift 12 or 6)
+// the Morton index of its sub-cell (16 or 4 per axis), so that a cell's bodies
+// are ordered in compact spatial groups. The reference's scatter leaves the order
+// within a cell open (:72-81); this order is one of its outcomes.
 __global__ __launch_bounds__(kThreads) void grid_assign(const float* __restrict__ x,
                                                         const float* __restrict__ y,
                                                         const float* __restrict__ z, int n,
@@ -121,12 +128,13 @@ __global__ __launch_bounds__(kThreads) void grid_assign(const float* __restrict_
   const int i = blockIdx.x * kThreads + threadIdx.x;
   if (i == 0) *n_dev = n;
   if (i >= n) return;
+  const int ns = g.shift == 12 ? 16 : 4;
   int sx, sy, sz;
-  const int cx = grid_axis_sub(x[i], g.lo[0], g.cell, g.dims[0], sx);
-  const int cy = grid_axis_sub(y[i], g.lo[1], g.cell, g.dims[1], sy);
-  const int cz = grid_axis_sub(z[i], g.lo[2], g.cell, g.dims[2], sz);
+  const int cx = grid_axis_sub(x[i], g.lo[0], g.cell, g.dims[0], ns, sx);
+  const int cy = grid_axis_sub(y[i], g.lo[1], g.cell, g.dims[1], ns, sy);
+  const int cz = grid_axis_sub(z[i], g.lo[2], g.cell, g.dims[2], ns, sz);
   const int c = cx + cy * g.dims[0] + cz * g.dims[0] * g.dims[1];
-  const uint32_t sub = (spread2(sx) << 2) | (spread2(sy) << 1) | spread2(sz);
+  const uint32_t sub = (spread4(sx) << 2) | (spread4(sy) << 1) | spread4(sz);
   pcell[i] = c;
   keys[i] = g.shift ? ((uint32_t)c << g.shift) | sub : (uint32_t)c;
   idx[i] = i;
@@ -168,9 +176,10 @@ __global__ __launch_bounds__(kThreads) void grid_ranges(const uint32_t* __restri
 }
 
 // Bodies in sorted order with G*m (:144 forms G * mass[j] first), padded with
-// zeros to a multiple of kNB, and the bounding box of every aligned group of kNB
-// bodies (box[2g] = min, box[2g+1] = max over the group's real bodies). The
-// grid covers the padded length; the groups' lanes sit in one wave.
+// zeros to a multiple of kNB, then the pad group (kGroup zero-mass bodies at
+// 1e18, beyond any cutoff), and the bounding box of every aligned group of
+// kGroup bodies (box[2g] = min, box[2g+1] = max over the group's real bodies).
+// The grid covers the padded length; the groups' lanes sit in one wave.
 __global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restrict__ idx,
                                                         const float* __restrict__ x,
                                                         const float* __restrict__ y,
@@ -190,17 +199,18 @@ __global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restric
     lo[2] = hi[2] = b.z;
   }
 #pragma unroll
-  for (int off = 1; off < kNB; off <<= 1)
+  for (int off = 1; off < kGroup; off <<= 1)
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
       hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
     }
+  if (k < kGroup) body[n_pad + k] = make_float4(1.0e18f, 1.0e18f, 1.0e18f, 0.0f);
   if (k >= n_pad) return;
   body[k] = b;
-  if ((k & (kNB - 1)) == 0) {
-    box[2 * (k / kNB)] = make_float4(lo[0], lo[1], lo[2], 0.0f);
-    box[2 * (k / kNB) + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+  if ((k & (kGroup - 1)) == 0) {
+    box[2 * (k / kGroup)] = make_float4(lo[0], lo[1], lo[2], 0.0f);
+    box[2 * (k / kGroup) + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
   }
 }
 
@@ -220,22 +230,26 @@ __device__ __forceinline__ float keep_or_zero(uint64_t keep, float v) {
   return r;
 }
 
-template <int NB>
-__device__ __forceinline__ void load_bodies(const float4* __restrict__ body, int j,
-                                            float4 (&b)[NB]) {
+// The bodies of kPer groups of kGroup: j[0] .. j[0]+kGroup-1, then j[1] ...
+__device__ __forceinline__ void load_groups(const float4* __restrict__ body,
+                                            const int (&j)[kPer], float4 (&b)[kNB]) {
 #pragma unroll
-  for (int q = 0; q < NB; ++q) b[q] = body[j + q];
+  for (int g = 0; g < kPer; ++g)
+#pragma unroll
+    for (int q = 0; q < kGroup; ++q) b[g * kGroup + q] = body[j[g] + q];
 }
 
-// kNB bodies (wave-uniform, in SGPRs) against each lane's target, issued stage
-// by stage over independent pairs: differences, then (once this batch's
-// operands are in) the loads of the next batch nxt_j, r^2 and the masks, the
-// rsq back to back, the force factors, the accumulations. Per pair (:133-149):
-// keep = r^2 < cutoff2 [and j != self] [and j in the lane's own range],
-// f = (G m_j) ((inv inv) inv), inv = rsq(r^2 + eps2), a += f r.
+// kNB bodies (wave-uniform, in SGPRs: the groups at j[0 .. kPer-1]) against
+// each lane's target, issued stage by stage over independent pairs: differences,
+// then (once this batch's operands are in) the loads of the next groups (nj),
+// r^2 and the masks, the rsq back to back, the force factors, the
+// accumulations. Per pair (:133-149): keep = r^2 < cutoff2 [and j != self] [and
+// j in the lane's own range], f = (G m_j) ((inv inv) inv), inv = rsq(r^2 + eps2),
+// a += f r.
 template <bool SELF, bool MEMBER>
-__device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], int j,
-                                           const float4* __restrict__ body, int nxt_j,
+__device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], const int (&jg)[kPer],
+                                           const float4* __restrict__ body,
+                                           const int (&nj)[kPer],
                                            float4 (&nxt)[kNB], int own_lo, int own_hi, int self,
                                            float xi, float yi, float zi, float cutoff2,
                                            float eps2, float& fx, float& fy, float& fz) {
@@ -247,17 +261,18 @@ __device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], int j,
     rz[q] = sub_rn(b[q].z, zi);
   }
   __builtin_amdgcn_sched_barrier(0);
-  load_bodies<kNB>(body, nxt_j, nxt);
+  load_groups(body, nj, nxt);
   __builtin_amdgcn_sched_barrier(0);
   // A dropped pair enters v_rsq as +inf, whose result +0 makes its term an
   // exact +0 (the same value the per-lane walk's mask gives): no lane mask
   // has to live across the batch, which keeps the SGPRs to the bodies.
 #pragma unroll
   for (int q = 0; q < kNB; ++q) {
+    const int j = jg[q / kGroup] + q % kGroup;
     const float d2 = grid_d2(rx[q], ry[q], rz[q]);
     bool k = d2 < cutoff2;
-    if (SELF) k = k && j + q != self;
-    if (MEMBER) k = k && j + q >= own_lo && j + q < own_hi;
+    if (SELF) k = k && j != self;
+    if (MEMBER) k = k && j >= own_lo && j < own_hi;
     w[q] = k ? add_rn(d2, eps2) : __builtin_inff();
   }
   __builtin_amdgcn_sched_barrier(0);
@@ -293,28 +308,29 @@ __device__ __forceinline__ void grid_pair(float4 b, bool keep, float xi, float y
 }
 
 // Walk of [ulo, uhi) with wave-uniform indices (scalar loads, SGPR operands) over
-// the aligned groups of kNB bodies that can hold a kept pair. Each lane tests one
-// group's box against the box of the wave's targets; a group whose per-axis gaps
-// (sx, sy, sz) give fma(sz, sz, fma(sx, sx, sy*sy)) >= cutoff2 is skipped. No pair
-// of a skipped group can be kept: every pair difference is at least the gap on
-// each axis, the gaps are rounded from the same coordinates by the same monotone
-// operations, and r^2 is the same monotone expression. The groups left are
-// evaluated in ascending order with the next one's loads in flight; a skipped
-// group's pairs would only have added exact zeros, so the sums are those of a
-// walk over every body.
+// the aligned groups of kGroup bodies that can hold a kept pair. Each lane tests
+// one group's box against the box of the wave's targets; a group whose per-axis
+// gaps (sx, sy, sz) give fma(sz, sz, fma(sx, sx, sy*sy)) >= cutoff2 is skipped.
+// No pair of a skipped group can be kept: every pair difference is at least the
+// gap on each axis, the gaps are rounded from the same coordinates by the same
+// monotone operations, and r^2 is the same monotone expression. The groups left
+// are evaluated in ascending order, two per batch, with the next two's loads in
+// flight; a skipped group's pairs would only have added exact zeros, so the sums
+// are those of a walk over every body.
 // SELF: the range may hold a lane's own body. multi: the wave's targets lie in
 // more than one cell, so each lane keeps only its own [own_lo, own_hi); edge
 // groups keep [ulo, uhi) the same way.
 template <bool SELF>
 __device__ __forceinline__ void walk_culled(const float4* __restrict__ body,
                                             const float4* __restrict__ box, int ulo, int uhi,
-                                            bool multi, int own_lo, int own_hi, int self,
-                                            const float (&tlo)[3], const float (&thi)[3],
-                                            float xi, float yi, float zi, float cutoff2,
-                                            float eps2, float& fx, float& fy, float& fz) {
+                                            int pad, bool multi, int own_lo, int own_hi,
+                                            int self, const float (&tlo)[3],
+                                            const float (&thi)[3], float xi, float yi, float zi,
+                                            float cutoff2, float eps2, float& fx, float& fy,
+                                            float& fz) {
   if (ulo >= uhi) return;
   const int lane = (int)(threadIdx.x & 63u);
-  const int g_first = ulo / kNB, g_last = (uhi - 1) / kNB;
+  const int g_first = ulo / kGroup, g_last = (uhi - 1) / kGroup;
   const int mlo = multi ? own_lo : ulo, mhi = multi ? own_hi : uhi;
   for (int gt = g_first; gt <= g_last; gt += 64) {
     const int gl = gt + lane;
@@ -327,25 +343,38 @@ __device__ __forceinline__ void walk_culled(const float4* __restrict__ body,
       pass = grid_d2(sx, sy, sz) < cutoff2;
     }
     uint64_t todo = __builtin_amdgcn_ballot_w64(pass);
+    // Groups left, kPer per batch in ascending order; a missing group is the pad
+    // group (zero mass, far beyond any cutoff: exact zeros). Once the groups run
+    // out the walk takes pad as its next batch and stops there.
+    auto next = [&](int (&j)[kPer]) {
+#pragma unroll
+      for (int g = 0; g < kPer; ++g) {
+        j[g] = todo ? (gt + (int)__builtin_ctzll(todo)) * kGroup : pad;
+        todo &= todo - 1;
+      }
+    };
     if (!todo) continue;
-    int gc = gt + (int)__builtin_ctzll(todo);
-    todo &= todo - 1;
-    float4 cur[kNB], nxt[kNB];
-    load_bodies<kNB>(body, gc * kNB, cur);
-    for (;;) {
-      const bool more = todo != 0;
-      const int gn = more ? gt + (int)__builtin_ctzll(todo) : gc;
-      const int j = gc * kNB;
-      if (multi || j < ulo || j + kNB > uhi)
-        grid_batch<SELF, true>(cur, j, body, gn * kNB, nxt, mlo, mhi, self, xi, yi, zi, cutoff2,
-                               eps2, fx, fy, fz);
+    int ja[kPer], jb[kPer];
+    next(ja);
+    float4 ba[kNB], bb[kNB];
+    load_groups(body, ja, ba);
+    for (;;) {  // two batches per turn, so the buffers alternate without copies
+      next(jb);
+      if (multi || ja[0] < ulo || ja[kPer - 1] + kGroup > uhi)
+        grid_batch<SELF, true>(ba, ja, body, jb, bb, mlo, mhi, self, xi, yi, zi, cutoff2, eps2,
+                               fx, fy, fz);
       else
-        grid_batch<SELF, false>(cur, j, body, gn * kNB, nxt, 0, 0, self, xi, yi, zi, cutoff2,
-                                eps2, fx, fy, fz);
-      if (!more) break;
-      todo &= todo - 1;
-      gc = gn;
-      for (int q = 0; q < kNB; ++q) cur[q] = nxt[q];
+        grid_batch<SELF, false>(ba, ja, body, jb, bb, 0, 0, self, xi, yi, zi, cutoff2, eps2, fx,
+                                fy, fz);
+      if (jb[0] == pad) break;
+      next(ja);
+      if (multi || jb[0] < ulo || jb[kPer - 1] + kGroup > uhi)
+        grid_batch<SELF, true>(bb, jb, body, ja, ba, mlo, mhi, self, xi, yi, zi, cutoff2, eps2,
+                               fx, fy, fz);
+      else
+        grid_batch<SELF, false>(bb, jb, body, ja, ba, 0, 0, self, xi, yi, zi, cutoff2, eps2, fx,
+                                fy, fz);
+      if (ja[0] == pad) break;
     }
   }
 }
@@ -391,7 +420,8 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
                                                         const int32_t* __restrict__ cstart,
                                                         const int32_t* __restrict__ cend,
                                                         GridGeom g, float cutoff2, float eps2,
-                                                        int n, int mode, float* __restrict__ ax,
+                                                        int n, int n_pad, int mode,
+                                                        float* __restrict__ ax,
                                                         float* __restrict__ ay,
                                                         float* __restrict__ az) {
   const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
@@ -428,10 +458,10 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
         // the self test is needed only where a lane's own body lies in the range
         // (the centre row, unless positions moved since the build)
         if (__any((k >= ulo) & (k < uhi)))
-          walk_culled<true>(body, box, ulo, uhi, multi, olo, ohi, k, tlo, thi, me.x, me.y, me.z,
+          walk_culled<true>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x, me.y, me.z,
                             cutoff2, eps2, fx, fy, fz);
         else
-          walk_culled<false>(body, box, ulo, uhi, multi, olo, ohi, k, tlo, thi, me.x, me.y,
+          walk_culled<false>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x, me.y,
                              me.z, cutoff2, eps2, fx, fy, fz);
       }
     }
@@ -479,7 +509,7 @@ struct nbx_grid {
   int32_t* idx = nullptr;
   int32_t* n_dev = nullptr;
   float4* body = nullptr;  // sorted bodies, padded to a multiple of kNB
-  float4* group_box = nullptr;  // per group of kNB sorted bodies: min, max
+  float4* group_box = nullptr;  // per group of kGroup sorted bodies: min, max
   int32_t* cstart = nullptr;
   int32_t* cend = nullptr;
   size_t cell_cap = 0;
@@ -553,8 +583,8 @@ int nbx_grid_create(size_t max_particles, float cell_size, nbx_grid** out) {
   A(&g->idx, P);
   A(&g->n_dev, 1);
   const size_t P8 = (P + kNB - 1) / kNB * kNB;
-  A(&g->body, P8);
-  A(&g->group_box, 2 * (P8 / kNB));
+  A(&g->body, P8 + kGroup);  // + the pad group
+  A(&g->group_box, 2 * (P8 / kGroup));
   if (rc == NBX_OK && hipHostMalloc((void**)&g->box_host, 6 * sizeof(float)) != hipSuccess) {
     (void)hipGetLastError();
     g->box_host = nullptr;
@@ -623,7 +653,7 @@ int nbx_grid_build(nbx_grid* g, const nbx_particles* d, nbx_stream stream) {
   }
   geom.cell = g->cell;
   geom.total = (int)total;
-  geom.shift = total <= kSubMaxCells ? kSubBits : 0;
+  geom.shift = total <= kSub16MaxCells ? 12 : (total <= kSub4MaxCells ? 6 : 0);
   if ((size_t)total > g->cell_cap) {  // grow-only cell arrays
     if (g->cstart) (void)hipFree(g->cstart);
     if (g->cend) (void)hipFree(g->cend);
@@ -669,8 +699,8 @@ int nbx_grid_forces(nbx_grid* g, nbx_particles* d, float cutoff, float G, float 
                      g->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, n, n_pad, G, g->body,
                      g->group_box);
   hipLaunchKernelGGL(grid_forces, dim3(grid_blocks(n, INT32_MAX)), dim3(kThreads), 0, s, g->body,
-                     g->group_box, g->idx, g->cstart, g->cend, g->geom, cutoff2, eps2, n, g->mode,
-                     d->acc_x, d->acc_y, d->acc_z);
+                     g->group_box, g->idx, g->cstart, g->cend, g->geom, cutoff2, eps2, n, n_pad,
+                     g->mode, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   return NBX_OK;
 }

@@ -267,6 +267,8 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
     for (int a = 0; a < 3; ++a) sh[q][a] = sl[q][a] = 0.0f;
   float nb[7];
   load_ds_body(xh, yh, zh, xl, yl, zl, m, jbase + lane, g.n, nb);
+  float eps2v = eps2;  // a VGPR operand (an SGPR operand halves the FP32 issue rate)
+  asm volatile("" : "+v"(eps2v));
   for (int ch = 0; ch < kChunks; ++ch) {
     float b[7];
 #pragma unroll
@@ -282,7 +284,7 @@ __device__ __forceinline__ void ds_sym_tile(const float* __restrict__ xh,
       const int q = ch - wave * IPL;
       self_q = (q >= 0 && q < IPL) ? q : -1;
     }
-    ds_sym_chunk<IPL, UNI, JSIDE, CHECKED>(src_addr, b, ih, il, mi, eps2, self_q, s, c, gs, gc);
+    ds_sym_chunk<IPL, UNI, JSIDE, CHECKED>(src_addr, b, ih, il, mi, eps2v, self_q, s, c, gs, gc);
 #pragma unroll
     for (int q = 0; q < IPL; ++q)
 #pragma unroll

@@ -597,25 +597,31 @@ static float grid_d2(float rx, float ry, float rz) {
   return fmaf(rz, rz, fmaf(rx, rx, ry * ry));
 }
 
-/* 2-bit sub-cell coordinate of a position in its (clamped) cell c on one axis:
- * floor(4u) - 4c clamped to 0..3 with u = (p - lo) / cell as for the cell. */
-static int32_t grid_sub(float p, float lo, float cell, int32_t c) {
+/* Sub-cell coordinate of a position in its (clamped) cell c on one axis, ns sub-cells
+ * per axis (16 or 4): floor(ns u) - ns c clamped to 0..ns-1 with u = (p - lo) / cell
+ * as for the cell (ns u is exact). */
+static int32_t grid_sub(float p, float lo, float cell, int32_t c, int32_t ns) {
   const float u = (p - lo) / cell;
-  const int32_t s = (int32_t)floorf(u * 4.0f) - 4 * c;
-  return s < 0 ? 0 : (s > 3 ? 3 : s);
+  const int32_t s = (int32_t)floorf(u * (float)ns) - ns * c;
+  return s < 0 ? 0 : (s > ns - 1 ? ns - 1 : s);
 }
 
-static uint32_t spread2(int32_t v) { return (uint32_t)((v & 1) | ((v & 2) << 2)); }
+/* bits 3..0 of v to bits 9, 6, 3, 0 */
+static uint32_t spread4(int32_t v) {
+  return (uint32_t)((v & 1) | ((v & 2) << 2) | ((v & 4) << 4) | ((v & 8) << 6));
+}
 
 /* force_spatial_hash.cu:278-308. Cell ranges: the exclusive scan of the counts.
  * Order within a cell (open in the reference, whose scatter is atomic): the Morton
- * index of the position's 4 x 4 x 4 sub-cell, then the particle index; grids of
- * more than 2^26 cells keep the particle index alone. */
+ * index of the position's sub-cell, then the particle index. Sub-cells are
+ * 16 x 16 x 16 per cell for grids of at most 2^20 cells, 4 x 4 x 4 up to 2^26;
+ * larger grids keep the particle index alone. */
 void orc_grid_build(size_t n, const float* x, const float* y, const float* z, float cell,
                     const float bbox[6], const int32_t dims[3], int32_t* particle_cell,
                     int32_t* cell_start, int32_t* cell_end, int32_t* sorted) {
   const size_t total = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
-  const int with_sub = total <= ((size_t)1 << 26);
+  const int32_t ns = total <= ((size_t)1 << 20) ? 16 : (total <= ((size_t)1 << 26) ? 4 : 0);
+  const int shift = ns == 16 ? 12 : 6;
   int32_t* count = (int32_t*)calloc(total ? total : 1, sizeof(int32_t));
   uint32_t* key = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
   for (size_t i = 0; i < n; ++i) {
@@ -625,11 +631,11 @@ void orc_grid_build(size_t n, const float* x, const float* y, const float* z, fl
     particle_cell[i] = id;
     count[id] += 1;
     uint32_t k = (uint32_t)id;
-    if (with_sub) {
-      const uint32_t sub = (spread2(grid_sub(x[i], bbox[0], cell, c[0])) << 2) |
-                           (spread2(grid_sub(y[i], bbox[1], cell, c[1])) << 1) |
-                           spread2(grid_sub(z[i], bbox[2], cell, c[2]));
-      k = (k << 6) | sub;
+    if (ns) {
+      const uint32_t sub = (spread4(grid_sub(x[i], bbox[0], cell, c[0], ns)) << 2) |
+                           (spread4(grid_sub(y[i], bbox[1], cell, c[1], ns)) << 1) |
+                           spread4(grid_sub(z[i], bbox[2], cell, c[2], ns));
+      k = (k << shift) | sub;
     }
     key[i] = k;
     sorted[i] = (int32_t)i;

@@ -40,21 +40,29 @@ def numpy_cells(x, y, z, lo, dims, cell):
     return c[0] + c[1] * dims[0] + c[2] * dims[0] * dims[1], c
 
 
+def sub_cells_per_axis(dims):
+    """16 sub-cells per axis up to 2^20 cells, 4 up to 2^26, none beyond."""
+    total = int(np.prod(dims))
+    return 16 if total <= 1 << 20 else (4 if total <= 1 << 26 else 0)
+
+
 def numpy_sort_key(x, y, z, lo, dims, cell):
-    """Order within a cell: Morton index of the 4x4x4 sub-cell (floor(4u) - 4c per axis,
-    u = (p - min) / cell in float32), for grids of at most 2^26 cells."""
+    """Order within a cell: Morton index of the position's sub-cell (floor(ns u) - ns c
+    per axis, u = (p - min) / cell in float32, ns = sub_cells_per_axis(dims))."""
     f = np.float32
     pc, c = numpy_cells(x, y, z, lo, dims, cell)
-    if int(np.prod(dims)) > 1 << 26:
+    ns = sub_cells_per_axis(dims)
+    if ns == 0:
         return pc, None
     s = []
     for a, p in enumerate((x, y, z)):
         u = (p - lo[a]) / f(cell)
         assert u.dtype == np.float32
-        s.append(np.clip(np.floor(u * f(4)).astype(np.int64) - 4 * c[a], 0, 3))
-    spread = lambda v: (v & 1) | ((v & 2) << 2)
+        s.append(np.clip(np.floor(u * f(ns)).astype(np.int64) - ns * c[a], 0, ns - 1))
+    def spread(v):  # bit b of v to bit 3b
+        return sum(((v >> b) & 1) << (3 * b) for b in range(4))
     sub = (spread(s[0]) << 2) | (spread(s[1]) << 1) | spread(s[2])
-    return pc * 64 + sub, s
+    return pc * (ns ** 3) + sub, s
 
 
 def test_cell_index_known_answers(orc):
@@ -96,8 +104,9 @@ def test_grid_too_large_is_refused(orc):
 
 def test_build_matches_numpy(orc):
     x, y, z = cube(5000, 10.0, 3)
-    for cell in (0.9, 2.0, 4.5):
+    for cell in (0.09, 0.9, 2.0, 4.5):  # 0.09: above 2^20 cells, 4 sub-cells per axis
         g = orc.grid_build(x, y, z, cell)
+        assert (g["total"] > 1 << 20) == (cell < 0.1)
         lo = g["bbox"][:3]
         pc, _ = numpy_cells(x, y, z, lo, g["dims"].astype(np.int64), cell)
         np.testing.assert_array_equal(g["particle_cell"], pc)
@@ -133,7 +142,7 @@ def test_every_particle_inside_its_cell(orc):
         for a, p in enumerate((x, y, z)):
             cmin = g["bbox"][a] + coords[a].astype(f) * f(cell)
             assert np.all(p >= cmin) and np.all(p <= cmin + f(cell))
-            q = f(cell) / f(4)
+            q = f(cell) / f(sub_cells_per_axis(dims))
             smin = cmin + subs[a].astype(f) * q
             assert np.all(p >= smin - 1e-5 * f(cell)) and np.all(p <= smin + q + 1e-5 * f(cell))
         # bodies of one cell are listed in ascending sub-cell order
