@@ -82,4 +82,8 @@ __device__ __forceinline__ float div_rn(float a, float b) {
 
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// The self pair contributes exactly zero (dx = 0, finite f) when eps2 keeps
+// rsq(eps2)^3 finite for every mass in use; below this the j == i test runs.
+constexpr float kUncheckedMinEps2 = 1e-12f;
+
 }  // namespace nbx

@@ -347,8 +347,4 @@ __device__ __forceinline__ void direct_tile(const float* __restrict__ x,
   X(1, 4, 4) X(1, 4, 8) X(1, 8, 4) X(1, 8, 8) X(1, 8, 16) X(1, 16, 8) X(2, 4, 8) X(4, 2, 4)
 #endif
 
-// The self pair contributes exactly zero (dx = 0, finite f) when eps2 keeps
-// rsq(eps2)^3 finite for every mass in use; below this the j == i test runs.
-constexpr float kUncheckedMinEps2 = 1e-12f;
-
 }  // namespace nbx

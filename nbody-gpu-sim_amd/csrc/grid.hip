@@ -439,13 +439,19 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   const int row = cy + cz * g.dims[1];
   const bool one_row = mode == 0 && __all(row == __builtin_amdgcn_readfirstlane(row));
+  const bool self_test = eps2 < kUncheckedMinEps2;
   if (one_row) {
     const int cy0 = __builtin_amdgcn_readfirstlane(cy), cz0 = __builtin_amdgcn_readfirstlane(cz);
     const int xmin = wave_min(cx), xmax = wave_max(cx);
     const int ulo_x = max(xmin - 1, 0), uhi_x = min(xmax + 1, g.dims[0] - 1);
     const bool multi = xmin != xmax;
-    const float tlo[3] = {wave_fmin(me.x), wave_fmin(me.y), wave_fmin(me.z)};
-    const float thi[3] = {wave_fmax(me.x), wave_fmax(me.y), wave_fmax(me.z)};
+    float tlo[3] = {wave_fmin(me.x), wave_fmin(me.y), wave_fmin(me.z)};
+    float thi[3] = {wave_fmax(me.x), wave_fmax(me.y), wave_fmax(me.z)};
+    // operands of vector instructions only, kept in VGPRs (an SGPR operand
+    // halves an FP32 instruction's issue rate, and the SGPRs hold the bodies)
+    float cut2v = cutoff2, eps2v = eps2;
+    asm volatile("" : "+v"(tlo[0]), "+v"(tlo[1]), "+v"(tlo[2]), "+v"(thi[0]), "+v"(thi[1]),
+                 "+v"(thi[2]), "+v"(cut2v), "+v"(eps2v));
     for (int dz = -1; dz <= 1; ++dz) {
       const int nz = cz0 + dz;
       if (nz < 0 || nz >= g.dims[2]) continue;
@@ -456,13 +462,14 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
         const int ulo = cstart[base + ulo_x], uhi = cend[base + uhi_x];
         const int olo = multi ? cstart[base + lo_x] : 0, ohi = multi ? cend[base + hi_x] : 0;
         // the self test is needed only where a lane's own body lies in the range
-        // (the centre row, unless positions moved since the build)
-        if (__any((k >= ulo) & (k < uhi)))
-          walk_culled<true>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x, me.y, me.z,
-                            cutoff2, eps2, fx, fy, fz);
+        // (the centre row, unless positions moved since the build), and only for
+        // eps2 below kUncheckedMinEps2: above it the self pair adds an exact zero
+        if (self_test && __any((k >= ulo) & (k < uhi)))
+          walk_culled<true>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x, me.y,
+                            me.z, cut2v, eps2v, fx, fy, fz);
         else
-          walk_culled<false>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x, me.y,
-                             me.z, cutoff2, eps2, fx, fy, fz);
+          walk_culled<false>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x,
+                             me.y, me.z, cut2v, eps2v, fx, fy, fz);
       }
     }
   } else {
