@@ -566,7 +566,8 @@ def run_ensemble(nbx, args):
            "interactions_per_step": pairs, "batched_ms_per_step": batched,
            "sequential_ms_per_step": sequential, "speedup": sequential / batched,
            "batched_Ginteractions_per_s": pairs / batched / 1e6,
-           "graph_chunk": chunk, "tile_variant": names[v], "force_error": err,
+           "steps_timed": K, "graph_chunk": chunk, "graph_launches": -(-K // chunk),
+           "tile_variant": names[v], "force_error": err,
            "workload": "512 Plummer systems (a=1, M=1 each), N_s log-uniform in [256, 16384], "
                        "eps=0.01, dt=1e-3"}
     log(f"ensemble: batched {batched:.3f} ms/step, sequential {sequential:.3f} ms/step, "
@@ -727,10 +728,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--bh-n", type=int, default=1 << 20)
-    ap.add_argument("--bh-steps", type=int, default=5)
+    ap.add_argument("--bh-steps", type=int, default=10)
     ap.add_argument("--bh-cpu-targets", type=int, default=65536)
     ap.add_argument("--bh-cpu1-targets", type=int, default=8192)
-    ap.add_argument("--ens-steps", type=int, default=100)
+    ap.add_argument("--ens-steps", type=int, default=1000)
     ap.add_argument("--ens-chunk", type=int, default=100)
     ap.add_argument("--ens-seq-steps", type=int, default=5)
     ap.add_argument("--sh-n", type=int, default=1 << 20)

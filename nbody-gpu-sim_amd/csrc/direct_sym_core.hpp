@@ -65,6 +65,14 @@ __device__ __forceinline__ void sym_chunk(int src_addr, float& xj, float& yj, fl
       dz[q] = zj - zi[q];
     }
     __builtin_amdgcn_sched_barrier(0);
+    // The chunk's positions are dead from here on in this step: rotate them now,
+    // so the LDS round trip overlaps the rest of the step instead of stalling
+    // the next step's differences (the mass, used by the force factors below,
+    // rotates at the end with the reaction sums).
+    xj = ror1(xj, src_addr);
+    yj = ror1(yj, src_addr);
+    zj = ror1(zj, src_addr);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
       float r2 = __builtin_fmaf(dx[q], dx[q], eps2);
@@ -102,9 +110,6 @@ __device__ __forceinline__ void sym_chunk(int src_addr, float& xj, float& yj, fl
       }
     }
     __builtin_amdgcn_sched_barrier(0);
-    xj = ror1(xj, src_addr);
-    yj = ror1(yj, src_addr);
-    zj = ror1(zj, src_addr);
     if (!UNI) mj = ror1(mj, src_addr);
     if (JSIDE) {
       gx = ror1(gx, src_addr);
