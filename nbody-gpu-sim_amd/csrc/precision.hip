@@ -202,6 +202,11 @@ __device__ __forceinline__ void ds_sym_chunk(int src_addr, float (&b)[7],
 #pragma unroll
       for (int a = 0; a < 3; ++a) d[q][a] = (b[a] - ih[q][a]) + (b[3 + a] - il[q][a]);
     __builtin_amdgcn_sched_barrier(0);
+    // the chunk's positions are dead for the rest of the step: rotate them now,
+    // so the step's arithmetic hides the LDS round trip
+#pragma unroll
+    for (int k = 0; k < 6; ++k) b[k] = ror1(b[k], src_addr);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int q = 0; q < IPL; ++q) {
       float r2 = __builtin_fmaf(d[q][0], d[q][0], eps2);
@@ -229,8 +234,6 @@ __device__ __forceinline__ void ds_sym_chunk(int src_addr, float (&b)[7],
         if (JSIDE) kahan_add(g[a], cg[a], -(wj[q] * d[q][a]));
       }
     __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) b[k] = ror1(b[k], src_addr);
     if (!UNI) b[6] = ror1(b[6], src_addr);
     if (JSIDE)
 #pragma unroll
