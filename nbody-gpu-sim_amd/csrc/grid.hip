@@ -180,6 +180,7 @@ __global__ __launch_bounds__(kThreads) void grid_ranges(const uint32_t* __restri
 // 1e18, beyond any cutoff), and the bounding box of every aligned group of
 // kGroup bodies (box[2g] = min, box[2g+1] = max over the group's real bodies).
 // The grid covers the padded length; the groups' lanes sit in one wave.
+// ublk[block] = 1 when every body of the block has the G m of sorted body 0.
 __global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restrict__ idx,
                                                         const float* __restrict__ x,
                                                         const float* __restrict__ y,
@@ -187,7 +188,8 @@ __global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restric
                                                         const float* __restrict__ m, int n,
                                                         int n_pad, float G,
                                                         float4* __restrict__ body,
-                                                        float4* __restrict__ box) {
+                                                        float4* __restrict__ box,
+                                                        int32_t* __restrict__ ublk) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
   float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -205,6 +207,9 @@ __global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restric
       lo[a] = fminf(lo[a], __shfl_xor(lo[a], off, 64));
       hi[a] = fmaxf(hi[a], __shfl_xor(hi[a], off, 64));
     }
+  const float gm0 = mul_rn(G, m[idx[0]]);
+  const bool same = __syncthreads_and(k >= n || b.w == gm0);
+  if (threadIdx.x == 0) ublk[blockIdx.x] = same ? 1 : 0;
   if (k < kGroup) body[n_pad + k] = make_float4(1.0e18f, 1.0e18f, 1.0e18f, 0.0f);
   if (k >= n_pad) return;
   body[k] = b;
@@ -212,6 +217,15 @@ __global__ __launch_bounds__(kThreads) void grid_gather(const int32_t* __restric
     box[2 * (k / kGroup)] = make_float4(lo[0], lo[1], lo[2], 0.0f);
     box[2 * (k / kGroup) + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
   }
+}
+
+// uni = 1 when every block of the gather found one G m (all bodies share it).
+__global__ __launch_bounds__(kThreads) void grid_uniform(const int32_t* __restrict__ ublk, int nb,
+                                                         int32_t* __restrict__ uni) {
+  int ok = 1;
+  for (int i = threadIdx.x; i < nb; i += kThreads) ok &= ublk[i];
+  ok = __syncthreads_and(ok);
+  if (threadIdx.x == 0) *uni = ok;
 }
 
 // r^2 of one pair as fma(rz, rz, fma(rx, rx, ry*ry)): the fused form an LLVM-based
@@ -245,8 +259,9 @@ __device__ __forceinline__ void load_groups(const float4* __restrict__ body,
 // r^2 and the masks, the rsq back to back, the force factors, the
 // accumulations. Per pair (:133-149): keep = r^2 < cutoff2 [and j != self] [and
 // j in the lane's own range], f = (G m_j) ((inv inv) inv), inv = rsq(r^2 + eps2),
-// a += f r.
-template <bool SELF, bool MEMBER>
+// a += f r. UNI: every body has the same G m, which the caller applies to the
+// finished sum instead (f = (inv inv) inv).
+template <bool SELF, bool MEMBER, bool UNI>
 __device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], const int (&jg)[kPer],
                                            const float4* __restrict__ body,
                                            const int (&nj)[kPer],
@@ -282,7 +297,7 @@ __device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], const int (&j
 #pragma unroll
   for (int q = 0; q < kNB; ++q) {
     const float inv = w[q];
-    w[q] = mul_rn(b[q].w, mul_rn(mul_rn(inv, inv), inv));
+    w[q] = UNI ? mul_rn(mul_rn(inv, inv), inv) : mul_rn(b[q].w, mul_rn(mul_rn(inv, inv), inv));
   }
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -294,6 +309,7 @@ __device__ __forceinline__ void grid_batch(const float4 (&b)[kNB], const int (&j
 }
 
 // One pair with the same arithmetic (tails of a walk, per-lane walks).
+template <bool UNI>
 __device__ __forceinline__ void grid_pair(float4 b, bool keep, float xi, float yi, float zi,
                                           float cutoff2, float eps2, float& fx, float& fy,
                                           float& fz) {
@@ -301,7 +317,8 @@ __device__ __forceinline__ void grid_pair(float4 b, bool keep, float xi, float y
   const float d2 = grid_d2(rx, ry, rz);
   const uint64_t k = __builtin_amdgcn_ballot_w64(keep & (d2 < cutoff2));
   const float inv = __builtin_amdgcn_rsqf(add_rn(d2, eps2));
-  const float f = keep_or_zero(k, mul_rn(b.w, mul_rn(mul_rn(inv, inv), inv)));
+  const float c = mul_rn(mul_rn(inv, inv), inv);
+  const float f = keep_or_zero(k, UNI ? c : mul_rn(b.w, c));
   fx = __builtin_fmaf(f, rx, fx);
   fy = __builtin_fmaf(f, ry, fy);
   fz = __builtin_fmaf(f, rz, fz);
@@ -320,7 +337,7 @@ __device__ __forceinline__ void grid_pair(float4 b, bool keep, float xi, float y
 // SELF: the range may hold a lane's own body. multi: the wave's targets lie in
 // more than one cell, so each lane keeps only its own [own_lo, own_hi); edge
 // groups keep [ulo, uhi) the same way.
-template <bool SELF>
+template <bool SELF, bool UNI>
 __device__ __forceinline__ void walk_culled(const float4* __restrict__ body,
                                             const float4* __restrict__ box, int ulo, int uhi,
                                             int pad, bool multi, int own_lo, int own_hi,
@@ -361,30 +378,31 @@ __device__ __forceinline__ void walk_culled(const float4* __restrict__ body,
     for (;;) {  // two batches per turn, so the buffers alternate without copies
       next(jb);
       if (multi || ja[0] < ulo || ja[kPer - 1] + kGroup > uhi)
-        grid_batch<SELF, true>(ba, ja, body, jb, bb, mlo, mhi, self, xi, yi, zi, cutoff2, eps2,
-                               fx, fy, fz);
+        grid_batch<SELF, true, UNI>(ba, ja, body, jb, bb, mlo, mhi, self, xi, yi, zi, cutoff2,
+                                    eps2, fx, fy, fz);
       else
-        grid_batch<SELF, false>(ba, ja, body, jb, bb, 0, 0, self, xi, yi, zi, cutoff2, eps2, fx,
-                                fy, fz);
+        grid_batch<SELF, false, UNI>(ba, ja, body, jb, bb, 0, 0, self, xi, yi, zi, cutoff2, eps2,
+                                     fx, fy, fz);
       if (jb[0] == pad) break;
       next(ja);
       if (multi || jb[0] < ulo || jb[kPer - 1] + kGroup > uhi)
-        grid_batch<SELF, true>(bb, jb, body, ja, ba, mlo, mhi, self, xi, yi, zi, cutoff2, eps2,
-                               fx, fy, fz);
+        grid_batch<SELF, true, UNI>(bb, jb, body, ja, ba, mlo, mhi, self, xi, yi, zi, cutoff2,
+                                    eps2, fx, fy, fz);
       else
-        grid_batch<SELF, false>(bb, jb, body, ja, ba, 0, 0, self, xi, yi, zi, cutoff2, eps2, fx,
-                                fy, fz);
+        grid_batch<SELF, false, UNI>(bb, jb, body, ja, ba, 0, 0, self, xi, yi, zi, cutoff2, eps2,
+                                     fx, fy, fz);
       if (ja[0] == pad) break;
     }
   }
 }
 
 // Walk of a lane's own [lo, hi) with per-lane (vector) loads.
+template <bool UNI>
 __device__ __forceinline__ void walk_lane(const float4* __restrict__ body, int lo, int hi,
                                           int self, float xi, float yi, float zi, float cutoff2,
                                           float eps2, float& fx, float& fy, float& fz) {
   for (int j = lo; j < hi; ++j)
-    grid_pair(body[j], j != self, xi, yi, zi, cutoff2, eps2, fx, fy, fz);
+    grid_pair<UNI>(body[j], j != self, xi, yi, zi, cutoff2, eps2, fx, fy, fz);
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -414,16 +432,17 @@ __device__ __forceinline__ int xcd_block(int i, int nb) {
 }
 
 // mode 0: uniform walk where a wave's targets share a row; 1: per-lane walks only.
-__global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict__ body,
-                                                        const float4* __restrict__ box,
-                                                        const int32_t* __restrict__ idx,
-                                                        const int32_t* __restrict__ cstart,
-                                                        const int32_t* __restrict__ cend,
-                                                        GridGeom g, float cutoff2, float eps2,
-                                                        int n, int n_pad, int mode,
-                                                        float* __restrict__ ax,
-                                                        float* __restrict__ ay,
-                                                        float* __restrict__ az) {
+// UNI: every body has the same G m (that of sorted body 0), applied to each
+// target's finished sum.
+template <bool UNI>
+__device__ __forceinline__ void grid_forces_body(const float4* __restrict__ body,
+                                                 const float4* __restrict__ box,
+                                                 const int32_t* __restrict__ idx,
+                                                 const int32_t* __restrict__ cstart,
+                                                 const int32_t* __restrict__ cend, GridGeom g,
+                                                 float cutoff2, float eps2, int n, int n_pad,
+                                                 int mode, float* __restrict__ ax,
+                                                 float* __restrict__ ay, float* __restrict__ az) {
   const int b = xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int k0 = b * kThreads + (int)(threadIdx.x & ~63u);
   if (k0 >= n) return;  // the whole wave is past the end
@@ -465,11 +484,11 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
         // (the centre row, unless positions moved since the build), and only for
         // eps2 below kUncheckedMinEps2: above it the self pair adds an exact zero
         if (self_test && __any((k >= ulo) & (k < uhi)))
-          walk_culled<true>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x, me.y,
-                            me.z, cut2v, eps2v, fx, fy, fz);
+          walk_culled<true, UNI>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x,
+                                 me.y, me.z, cut2v, eps2v, fx, fy, fz);
         else
-          walk_culled<false>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi, me.x,
-                             me.y, me.z, cut2v, eps2v, fx, fy, fz);
+          walk_culled<false, UNI>(body, box, ulo, uhi, n_pad, multi, olo, ohi, k, tlo, thi,
+                                  me.x, me.y, me.z, cut2v, eps2v, fx, fy, fz);
       }
     }
   } else {
@@ -479,10 +498,16 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
         const int ny = cy + dy;
         if (nz < 0 || nz >= g.dims[2] || ny < 0 || ny >= g.dims[1]) continue;
         const int base = ny * g.dims[0] + nz * dxy;
-        walk_lane(body, cstart[base + lo_x], cend[base + hi_x], k, me.x, me.y, me.z, cutoff2,
-                  eps2, fx, fy, fz);
+        walk_lane<UNI>(body, cstart[base + lo_x], cend[base + hi_x], k, me.x, me.y, me.z,
+                       cutoff2, eps2, fx, fy, fz);
       }
     }
+  }
+  if (UNI) {
+    const float gm = body[0].w;
+    fx = mul_rn(gm, fx);
+    fy = mul_rn(gm, fy);
+    fz = mul_rn(gm, fz);
   }
   if (active) {
     const int i = idx[k];
@@ -490,6 +515,25 @@ __global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict
     ay[i] = fy;
     az[i] = fz;
   }
+}
+
+__global__ __launch_bounds__(kThreads) void grid_forces(const float4* __restrict__ body,
+                                                        const float4* __restrict__ box,
+                                                        const int32_t* __restrict__ idx,
+                                                        const int32_t* __restrict__ cstart,
+                                                        const int32_t* __restrict__ cend,
+                                                        GridGeom g, float cutoff2, float eps2,
+                                                        int n, int n_pad, int mode,
+                                                        const int32_t* __restrict__ uni,
+                                                        float* __restrict__ ax,
+                                                        float* __restrict__ ay,
+                                                        float* __restrict__ az) {
+  if (*uni)
+    grid_forces_body<true>(body, box, idx, cstart, cend, g, cutoff2, eps2, n, n_pad, mode, ax,
+                           ay, az);
+  else
+    grid_forces_body<false>(body, box, idx, cstart, cend, g, cutoff2, eps2, n, n_pad, mode, ax,
+                            ay, az);
 }
 
 int bits_for(int total) {  // key bits covering cell numbers 0 .. total-1
@@ -517,6 +561,8 @@ struct nbx_grid {
   int32_t* n_dev = nullptr;
   float4* body = nullptr;  // sorted bodies, padded to a multiple of kNB
   float4* group_box = nullptr;  // per group of kGroup sorted bodies: min, max
+  int32_t* ublk = nullptr;       // per gather block: its bodies share sorted body 0's G m
+  int32_t* uni = nullptr;        // every body shares it
   int32_t* cstart = nullptr;
   int32_t* cend = nullptr;
   size_t cell_cap = 0;
@@ -592,6 +638,8 @@ int nbx_grid_create(size_t max_particles, float cell_size, nbx_grid** out) {
   const size_t P8 = (P + kNB - 1) / kNB * kNB;
   A(&g->body, P8 + kGroup);  // + the pad group
   A(&g->group_box, 2 * (P8 / kGroup));
+  A(&g->ublk, (size_t)grid_blocks(P8, INT32_MAX));
+  A(&g->uni, 1);
   if (rc == NBX_OK && hipHostMalloc((void**)&g->box_host, 6 * sizeof(float)) != hipSuccess) {
     (void)hipGetLastError();
     g->box_host = nullptr;
@@ -702,12 +750,13 @@ int nbx_grid_forces(nbx_grid* g, nbx_particles* d, float cutoff, float G, float 
   const float cutoff2 = cutoff * cutoff;  // :320
   const float eps2 = eps * eps;
   const int n_pad = (n + kNB - 1) / kNB * kNB;
-  hipLaunchKernelGGL(grid_gather, dim3(grid_blocks(n_pad, INT32_MAX)), dim3(kThreads), 0, s,
-                     g->idx, d->pos_x, d->pos_y, d->pos_z, d->mass, n, n_pad, G, g->body,
-                     g->group_box);
+  const int gblocks = grid_blocks(n_pad, INT32_MAX);
+  hipLaunchKernelGGL(grid_gather, dim3(gblocks), dim3(kThreads), 0, s, g->idx, d->pos_x, d->pos_y,
+                     d->pos_z, d->mass, n, n_pad, G, g->body, g->group_box, g->ublk);
+  hipLaunchKernelGGL(grid_uniform, dim3(1), dim3(kThreads), 0, s, g->ublk, gblocks, g->uni);
   hipLaunchKernelGGL(grid_forces, dim3(grid_blocks(n, INT32_MAX)), dim3(kThreads), 0, s, g->body,
                      g->group_box, g->idx, g->cstart, g->cend, g->geom, cutoff2, eps2, n, n_pad,
-                     g->mode, d->acc_x, d->acc_y, d->acc_z);
+                     g->mode, g->uni, d->acc_x, d->acc_y, d->acc_z);
   NBX_LAUNCHED();
   return NBX_OK;
 }
