@@ -62,11 +62,16 @@ def case(nbx, name):
         return host_from([1.5], [-2.0], [0.25], [3.0]), 1.0, 1.0
     if name == "huge_cell":             # cell larger than the box: 2 x 2 x 2 cells
         return cube(500, 3.0, 5), 50.0, 50.0
+    if name == "sparse_69m":            # 411^3 cells, above 2^26: cell order alone, no sub-cells
+        rng = np.random.default_rng(8)
+        p = rng.uniform(-205.0, 205.0, (3, 1000)).astype(np.float32)
+        p[:, 900:] = rng.normal(0.0, 1.0, (3, 100)).astype(np.float32)  # a cluster with pairs
+        return host_from(p[0], p[1], p[2], rng.uniform(0.5, 2.0, 1000)), 1.0, 1.0
     raise KeyError(name)
 
 
 BUILD_CASES = ["uniform100", "cube2000", "cube2000_cell3", "sphere4096", "dense50000",
-               "plummer65536", "coincident", "collinear", "single", "huge_cell"]
+               "plummer65536", "coincident", "collinear", "single", "huge_cell", "sparse_69m"]
 
 
 def gpu_grid(nbx, h, cell, capacity=None):
