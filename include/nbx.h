@@ -232,7 +232,10 @@ int nbx_grid_create(size_t max_particles, float cell_size, nbx_grid** out);
 int nbx_grid_destroy(nbx_grid* g);
 /* SpatialHashGrid::build (:241-309): bbox padded by 0.001f, dims
  * (int)ceilf((max-min)/cell)+1, cell of every particle, particles grouped by
- * cell (ascending index within a cell), cell_start/cell_end of every cell.
+ * cell, cell_start/cell_end of every cell. Within a cell the order (which the
+ * reference's atomic scatter leaves open) is the Morton index of the particle's
+ * sub-cell (16 per axis up to 2^20 cells, 4 up to 2^26, none beyond), then the
+ * particle index.
  * Reads the bbox back (24 bytes) to size the grid, so it waits for the stream.
  * NBX_ERR_CAPACITY "Spatial hash grid too large: ..." above 1e8 cells (:257-260)
  * or when count exceeds max_particles. */
