@@ -44,6 +44,8 @@ def case(nbx, name):
         return cube(2000, 10.0, 2), 2.0, 2.0
     if name == "cube2000_cell3":
         return cube(2000, 10.0, 3), 3.0, 2.0
+    if name == "cube2000_cut3":         # cutoff beyond the cell: only the 27 cells are walked
+        return cube(2000, 10.0, 9), 2.0, 3.0
     if name == "sphere4096":            # the reference benchmark's distribution at its default N
         return sphere(nbx, 4096, 10.0), 2.0, 2.0
     if name == "dense50000":            # hundreds of bodies per cell: shared walks
@@ -70,7 +72,8 @@ def case(nbx, name):
     raise KeyError(name)
 
 
-BUILD_CASES = ["uniform100", "cube2000", "cube2000_cell3", "sphere4096", "dense50000",
+BUILD_CASES = ["uniform100", "cube2000", "cube2000_cell3", "cube2000_cut3", "sphere4096",
+               "dense50000",
                "plummer65536", "coincident", "collinear", "single", "huge_cell", "sparse_69m"]
 
 
@@ -128,8 +131,8 @@ def test_forces_match_oracle(nbx, orc, name):
     check_forces(orc, h, ref, acc, cutoff, 1.0, 0.05)
 
 
-@pytest.mark.parametrize("name", ["cube2000", "sphere4096", "dense50000", "plummer65536",
-                                  "coincident"])
+@pytest.mark.parametrize("name", ["cube2000", "cube2000_cut3", "sphere4096", "dense50000",
+                                  "plummer65536", "coincident"])
 def test_walks_are_bit_identical(nbx, name):
     h, cell, cutoff = case(nbx, name)
     print(f"grid walks {name}", flush=True)
