@@ -173,12 +173,24 @@ __global__ __launch_bounds__(kThreads) void bbox_final(const float* partial, int
 
 // Also packs each particle as (x, y, z, m), so that the paths pass gathers one
 // 16-byte record per particle through the sort permutation instead of four.
+// Bits 0..20 of v to bits 0, 3, ..., 60.
+__device__ __forceinline__ uint64_t spread3_64(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
 // Skilling's Hilbert index (AIP Conf. Proc. 707, 381 (2004)) of a cell of the
-// 1024^3 grid: the transpose form of the three 10-bit coordinates, then its
-// bits interleaved, the first axis most significant.
-__device__ __forceinline__ uint32_t hilbert_index(uint32_t x, uint32_t y, uint32_t z) {
+// 2^B x 2^B x 2^B grid: the transpose form of the three B-bit coordinates, then
+// its bits interleaved, the first axis most significant (3B bits).
+template <int B>
+__device__ __forceinline__ uint64_t hilbert_index(uint32_t x, uint32_t y, uint32_t z) {
   uint32_t X[3] = {x, y, z};
-  for (uint32_t q = 1u << 9; q > 1; q >>= 1) {
+  for (uint32_t q = 1u << (B - 1); q > 1; q >>= 1) {
     const uint32_t p = q - 1;
     for (int i = 0; i < 3; ++i) {
       if (X[i] & q) {
@@ -193,9 +205,21 @@ __device__ __forceinline__ uint32_t hilbert_index(uint32_t x, uint32_t y, uint32
   X[1] ^= X[0];
   X[2] ^= X[1];
   uint32_t t = 0;
-  for (uint32_t q = 1u << 9; q > 1; q >>= 1)
+  for (uint32_t q = 1u << (B - 1); q > 1; q >>= 1)
     if (X[2] & q) t ^= q - 1;
-  return (expand_bits(X[0] ^ t) << 2) | (expand_bits(X[1] ^ t) << 1) | expand_bits(X[2] ^ t);
+  return (spread3_64(X[0] ^ t) << 2) | (spread3_64(X[1] ^ t) << 1) | spread3_64(X[2] ^ t);
+}
+
+// The traversal's task key: the Hilbert index of the particle's cell of the
+// 2048^3 grid over the per-axis normalised box (33 bits), its top 32 bits.
+// (Task order only: forces do not depend on it.)
+__device__ __forceinline__ uint32_t task_hilbert_key(float nx, float ny, float nz) {
+  constexpr int B = 11;
+  const float top = (float)((1u << B) - 1);
+  const uint32_t hx = (uint32_t)(fminf(fmaxf(nx, 0.0f), 1.0f) * top);
+  const uint32_t hy = (uint32_t)(fminf(fmaxf(ny, 0.0f), 1.0f) * top);
+  const uint32_t hz = (uint32_t)(fminf(fmaxf(nz, 0.0f), 1.0f) * top);
+  return (uint32_t)(hilbert_index<B>(hx, hy, hz) >> (3 * B - 32));
 }
 
 __global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restrict__ x,
@@ -215,7 +239,7 @@ __global__ __launch_bounds__(kThreads) void morton_kernel(const float* __restric
   const uint32_t qx = quantise(nx), qy = quantise(ny), qz = quantise(nz);
   codes[i] = (expand_bits(qx) << 2) | (expand_bits(qy) << 1) | expand_bits(qz);
   idx[i] = i;
-  hkey[i] = hilbert_index(qx, qy, qz);  // the traversal's task order (task_keys)
+  hkey[i] = task_hilbert_key(nx, ny, nz);  // the traversal's task order (task_keys)
 }
 
 // ---------------------------------------------------------------- 4: paths
@@ -2294,7 +2318,7 @@ int nbx_bh_build(nbx_bh* t, const nbx_particles* d, nbx_stream stream) {
   hipLaunchKernelGGL(task_keys, dim3(div_up(n, kThreads)), dim3(kThreads), 0, s, t->st, t->hkey,
                      t->idx, t->O.ppart, t->task_key, t->task_order);
   NBX_LAUNCHED();
-  if (!t->task_tree_order) NBX_TRY(t->sort_particles.sort(t->task_key, t->task_order, &t->st->n, 30, s));
+  if (!t->task_tree_order) NBX_TRY(t->sort_particles.sort(t->task_key, t->task_order, &t->st->n, 32, s));
   mark(t, 7, s);
   t->built = true;
   t->built_n = d->count;
