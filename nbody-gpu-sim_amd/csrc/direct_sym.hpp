@@ -14,7 +14,7 @@ namespace nbx {
 // Shapes X(IPL, W) of the symmetric kernel: IPL targets per lane, W waves per
 // workgroup; super-blocks of 64 * IPL * W bodies (256 .. 2,048). New shapes go
 // at the end: the variant indices are part of the interface.
-#define NBX_SYM_VARIANTS(X) X(8, 4) X(4, 4) X(4, 8) X(8, 2) X(4, 2) X(2, 4) X(2, 2) X(4, 1) X(6, 4) X(6, 2)
+#define NBX_SYM_VARIANTS(X) X(8, 4) X(4, 4) X(4, 8) X(8, 2) X(4, 2) X(2, 4) X(2, 2) X(4, 1) X(6, 4) X(6, 2) X(6, 1) X(8, 1)
 
 // Device scratch of a symmetric evaluation: FP32 partial sums of one round of
 // tile distances, and the FP64 carry of every body's sum between rounds

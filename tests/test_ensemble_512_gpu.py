@@ -5,8 +5,8 @@ Plummer sphere (a = 1, M = 1) seeded 1000 + s; eps = 0.01, dt = 1e-3.
 
 * Batched forces are bit-identical to a standalone single-system call with the
   same tile shape for 20 systems spread over the size range, the largest
-  included, for the 512- and 256-body pair-symmetric shapes the autotuner
-  chooses between;
+  included, for the 384-body pair-symmetric shape the autotuner picks and the
+  512- and 256-body shapes next to it;
 * those systems' forces are within p99 <= 1e-6 of the oracle's FP64 sum;
 * 100 steps replayed from one captured hipGraph equal 100 eagerly launched
   steps bit for bit, across the whole ensemble.
@@ -50,7 +50,8 @@ def standalone(nbx, h, sl, variant):
     return d.acc()
 
 
-@pytest.mark.parametrize("variant", [12, 15], ids=["sym_ipl4_ws2", "sym_ipl4_ws1"])
+@pytest.mark.parametrize("variant", [12, 15, 18],
+                         ids=["sym_ipl4_ws2", "sym_ipl4_ws1", "sym_ipl6_ws1"])
 def test_full_ensemble_matches_standalone_and_fp64(nbx, orc, workload, variant):
     counts, h = workload
     assert counts.min() >= 256 and counts.max() <= 16384 and len(counts) == 512
@@ -81,7 +82,7 @@ def test_graph_chunk_of_100_steps_equals_eager(nbx, workload):
     a, b = nbx.Ensemble(counts), nbx.Ensemble(counts)
     for e in (a, b):
         e.upload(h)
-        e.set_variant(15)
+        e.set_variant(18)
         e.forces(G, e2())
     s = nbx.Stream()
     a.steps(G, e2(), DT, 100, graph_chunk=100, stream=s)

@@ -84,6 +84,7 @@ def test_variant_names_match_single_system(nbx):
     assert nbx.lib.nbx_ens_variant_count() == nbx.lib.nbx_direct_variant_count()
     assert nbx.variant_names()[9] == "sym_ipl4_ws4" and nbx.variant_names()[8] == "sym_ipl8_ws4"
     assert nbx.variant_names()[12] == "sym_ipl4_ws2" and nbx.variant_names()[15] == "sym_ipl4_ws1"
+    assert nbx.variant_names()[18] == "sym_ipl6_ws1" and nbx.variant_names()[19] == "sym_ipl8_ws1"
 
 
 @pytest.mark.parametrize("variant", [2, 9])
