@@ -82,6 +82,24 @@ __device__ __forceinline__ float div_rn(float a, float b) {
 
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// AND of p over a one-dimensional workgroup of W waves (64 * W threads): a
+// ballot per wave and, for W > 1, one LDS word per wave between two barriers.
+// __syncthreads_and forms a linear thread id from the y and z workitem ids,
+// which then occupy two VGPRs for the rest of a kernel that loops over tiles.
+template <int W>
+__device__ __forceinline__ bool block_all(bool p) {
+  const bool wave_all = __ballot(!p) == 0;
+  if (W == 1) return wave_all;
+  __shared__ int votes[W];
+  if ((threadIdx.x & 63) == 0) votes[threadIdx.x >> 6] = wave_all;
+  __syncthreads();
+  bool all = true;
+#pragma unroll
+  for (int w = 0; w < W; ++w) all = all && votes[w] != 0;
+  __syncthreads();  // the words are rewritten by the next call
+  return all;
+}
+
 // The self pair contributes exactly zero (dx = 0, finite f) when eps2 keeps
 // rsq(eps2)^3 finite for every mass in use; below this the j == i test runs.
 constexpr float kUncheckedMinEps2 = 1e-12f;

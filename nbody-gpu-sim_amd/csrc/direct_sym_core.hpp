@@ -141,12 +141,12 @@ __device__ __forceinline__ void sym_tile(const float* __restrict__ x,
                                          int C, int d, float mref, float eps2,
                                          const float (&xi)[IPL], const float (&yi)[IPL],
                                          const float (&zi)[IPL], const float (&mi)[IPL],
-                                         float (*red)[W][3][64]) {
+                                         float (*red)[W][3][64], int tid) {
   constexpr int B = 64 * IPL;
   constexpr int S = W * B;
   constexpr int kChunks = S / 64;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int jbase = C * S;
   const int src_addr = ((lane + 63) & 63) * 4;
   double sx[IPL], sy[IPL], sz[IPL];
@@ -192,7 +192,7 @@ __device__ __forceinline__ void sym_tile(const float* __restrict__ x,
       buf[wave][2][lane] = gz;
       __syncthreads();
       // 3 components x 64 bodies, whatever the workgroup size
-      for (int r = threadIdx.x; r < 192; r += 64 * W) {
+      for (int r = tid; r < 192; r += 64 * W) {
         const int k = r >> 6, l = r & 63;
         double t = 0.0;
 #pragma unroll
@@ -222,11 +222,13 @@ __device__ __forceinline__ void sym_dispatch(const float* __restrict__ x,
                                              int C, int d, float mref, float eps2,
                                              const float (&xi)[IPL], const float (&yi)[IPL],
                                              const float (&zi)[IPL], const float (&mi)[IPL],
-                                             float (*red)[W][3][64]) {
+                                             float (*red)[W][3][64], int tid) {
   if (d == 0)
-    sym_tile<IPL, W, UNI, false, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
+    sym_tile<IPL, W, UNI, false, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red,
+                                          tid);
   else
-    sym_tile<IPL, W, UNI, true, false>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
+    sym_tile<IPL, W, UNI, true, false>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red,
+                                       tid);
 }
 
 // Tile t of a round: its diagonal tiles first (t < K, round starting at d = 0),
@@ -256,8 +258,13 @@ __device__ __forceinline__ void sym_run_tile(const float* __restrict__ x,
                                              int d, float eps2, float (*red)[W][3][64]) {
   constexpr int B = 64 * IPL;
   constexpr int S = W * B;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // The thread index through an empty asm: a kernel that runs many tiles
+  // (the batched ensemble) then derives the lane values per tile instead of
+  // holding them in VGPRs across its whole item loop.
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int C = (A + d) % g.K;
   float xi[IPL], yi[IPL], zi[IPL], mi[IPL];
 #pragma unroll
@@ -266,15 +273,17 @@ __device__ __forceinline__ void sym_run_tile(const float* __restrict__ x,
   // Do all real bodies of A and C share one mass?
   const float mref = m[A * S];
   bool same = true;
-  for (int k = threadIdx.x; k < S; k += 64 * W) {
+  for (int k = tid; k < S; k += 64 * W) {
     const int ia = A * S + k, jc = C * S + k;
     if (ia < g.n) same = same && (m[ia] == mref);
     if (jc < g.n) same = same && (m[jc] == mref);
   }
-  if (__syncthreads_and(same))
-    sym_dispatch<IPL, W, true, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
+  if (block_all<W>(same))
+    sym_dispatch<IPL, W, true, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red,
+                                        tid);
   else
-    sym_dispatch<IPL, W, false, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red);
+    sym_dispatch<IPL, W, false, CHECKED>(x, y, z, m, g, A, C, d, mref, eps2, xi, yi, zi, mi, red,
+                                         tid);
 }
 
 // Body b's partials of one round added to t in the canonical order: for d

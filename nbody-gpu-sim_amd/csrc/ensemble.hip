@@ -71,7 +71,7 @@ __global__ __launch_bounds__(64 * WS) void ens_forces_kernel(
 // that system's slice (sym_run_tile), writing its partial sums. The queue
 // resets as in ens_forces_kernel.
 template <int IPL, int W, bool CHECKED>
-__global__ __launch_bounds__(64 * W) void ens_sym_kernel(
+__global__ __launch_bounds__(64 * W, IPL <= 6 ? 4 : 2) void ens_sym_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     const float* __restrict__ m, const int32_t* __restrict__ sys_off,
     const int32_t* __restrict__ sys_n, const int64_t* __restrict__ part_off, float* part,

@@ -356,7 +356,7 @@ __global__ __launch_bounds__(64 * W, 4) void ds_sym_kernel(
     if (ia < g.n) same = same && (m[ia] == mref);
     if (jc < g.n) same = same && (m[jc] == mref);
   }
-  const bool uni = __syncthreads_and(same);
+  const bool uni = block_all<W>(same);
 #define NBX_DS_TILE(U, J, CK) \
   ds_sym_tile<IPL, W, U, J, CK>(xh, yh, zh, xl, yl, zl, m, g, A, C, dd, mref, eps2, ih, il, mi, red)
   if (uni) {
