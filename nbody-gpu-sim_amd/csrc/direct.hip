@@ -99,7 +99,7 @@ static int size_class(size_t n) {
 
 // Defaults when nothing was tuned, from sweeps on MI355X (uniform-sphere
 // bodies, equal masses): the one-sided kernel for small systems and for calls
-// that bring no workspace, the symmetric one from 2^16 bodies.
+// that bring no workspace, the symmetric ones above 2^14 bodies.
 #define NBX_COUNT_V(I, B, W) +1
 constexpr int kNumOneSided = 0 NBX_DIRECT_VARIANTS(NBX_COUNT_V);
 #undef NBX_COUNT_V
@@ -111,10 +111,20 @@ static int one_sided_default(size_t n) {
 }
 
 static int default_variant(size_t n) {
-  // sym_ipl6_ws4: 1,536-body super-blocks, 4 waves per SIMD (6 targets per lane
-  // take 1.5x the pairs of 4 per rotation step; 8 need 163 VGPRs, 3 waves);
-  // 1.5-6 % faster than sym_ipl4_ws4 from 2^16 to 2^20 bodies
-  if (n >= (size_t(1) << 16)) return kNumOneSided + 8;
+  // From forces-only timings of every shape (Plummer spheres of 4,096 to
+  // 262,144 bodies, warm clocks):
+  //   n >= 3 * 2^16: sym_ipl6_ws4, 1,536-body super-blocks, 4 waves per SIMD (6
+  //     targets per lane take 1.5x the pairs of 4 per rotation step; 8 need
+  //     163 VGPRs, 3 waves); 1.5-6 % faster than sym_ipl4_ws4 up to 2^20;
+  //   2^17 <= n < 3 * 2^16: sym_ipl8_ws1 (512), 1-2.6 % faster than sym_ipl6_ws4;
+  //   2^16 <= n < 2^17: sym_ipl6_ws1 (384), 4-11 % faster than sym_ipl6_ws4;
+  //   2^14 < n < 2^16: sym_ipl4_ws1 (256), 1.7-1.9x the one-sided kernels;
+  //   n <= 2^14: the one-sided kernels (a symmetric tile's partial sums and
+  //     their reduction cost more than the halved pair work saves).
+  if (n >= 3 * (size_t(1) << 16)) return kNumOneSided + 8;
+  if (n >= (size_t(1) << 17)) return kNumOneSided + 11;
+  if (n >= (size_t(1) << 16)) return kNumOneSided + 10;
+  if (n > (size_t(1) << 14)) return kNumOneSided + 7;
   return one_sided_default(n);
 }
 
