@@ -477,7 +477,7 @@ __global__ void ds_drift_kernel(float* __restrict__ xh, float* __restrict__ yh,
 }
 
 constexpr int kIpl = 2, kWs = 8;           // one-sided shape
-constexpr int kSymIpl = 4, kSymW = 4;      // pair-symmetric shape, super-blocks of 1,024
+constexpr int kSymIpl = 4, kSymW = 2;      // pair-symmetric shape, super-blocks of 512
 constexpr int kSymS = 64 * kSymIpl * kSymW;
 
 // -1: symmetric from 2^15 bodies when the call brings a workspace, 0: one-sided,

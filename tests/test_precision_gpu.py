@@ -50,7 +50,7 @@ def test_ds_forces_match_fp64(nbx, orc, ws, ds_kernel):
 
 
 def test_ds_forces_ragged_and_zero_softening(nbx, orc, ws, ds_kernel):
-    # 1000 and 2500 bodies give 1 and 3 super-blocks of 1,024 in the symmetric kernel
+    # 1000 and 2500 bodies give 2 and 5 super-blocks of 512 in the symmetric kernel
     for n, eps in ((1, 0.01), (65, 0.01), (1000, 0.0), (2500, 0.01), (2500, 0.0)):
         h = nbx.gen_uniform_sphere(n, seed=n)
         d = nbx.DSDeviceParticles.from_host(h)
@@ -207,6 +207,9 @@ def test_ds_symmetric_rounds_bit_identical(nbx, orc):
            for k in ("pos_x", "pos_y", "pos_z")]
     e2 = eps2(orc, 0.01)
     res = []
+    # the smallest workspace accepted: three slots for the body count padded to the
+    # largest super-block (8,192); with the kernel's 512-body super-blocks (15 of
+    # them) that still holds only rounds of one distance
     for scratch in (0, 9 * 4 * 8 * 1024):
         w = nbx.DirectWorkspace(n, scratch)
         d = nbx.DSDeviceParticles.from_host(h, pos64=x64)
