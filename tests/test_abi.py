@@ -89,3 +89,15 @@ def test_ensemble_sizes(nbx):
     assert np.array_equal(s, nbx.gen_ensemble_sizes(512, seed=42))
     lg = np.log(s.astype(np.float64))
     assert abs(lg.mean() - 0.5 * (np.log(256) + np.log(16384))) < 0.2
+
+
+def test_default_direct_shape_by_size(nbx):
+    # the shape a call without tuning launches, at each size boundary (host logic only)
+    names = nbx.variant_names()
+    assert nbx.lib.nbx_direct_set_variant(-1) == 0
+    expect = [(4096, "ipl1_nb8_ws16"), (16384, "ipl1_nb8_ws16"), (16385, "sym_ipl4_ws1"),
+              (65535, "sym_ipl4_ws1"), (65536, "sym_ipl6_ws1"), (131071, "sym_ipl6_ws1"),
+              (131072, "sym_ipl8_ws1"), (196607, "sym_ipl8_ws1"), (196608, "sym_ipl6_ws4"),
+              (1 << 20, "sym_ipl6_ws4")]
+    got = [(n, names[nbx.lib.nbx_direct_variant_for(n)]) for n, _ in expect]
+    assert got == expect
