@@ -274,7 +274,7 @@ int nbx_ds_free(nbx_ds_particles* p);
 /* Forces; op is a direct workspace (nbx_direct_create) for the pair-symmetric
  * kernel, or NULL for the one-sided kernel (no scratch). */
 int nbx_ds_forces(nbx_direct* op, nbx_ds_particles* d, float G, float eps2, nbx_stream s);
-/* Kernel of nbx_ds_forces / nbx_ds_step: -1 pair-symmetric from 2^15 bodies when
+/* Kernel of nbx_ds_forces / nbx_ds_step: -1 pair-symmetric above 2^13 bodies when
  * a workspace is given (the default), 0 one-sided, 1 pair-symmetric (needs one). */
 int nbx_ds_set_kernel(int kind);
 /* store+drift in double-single: (hi,lo) += v*dt + a*(0.5*dt*dt) with renormalisation. */

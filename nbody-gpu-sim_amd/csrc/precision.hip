@@ -156,7 +156,7 @@ __global__ __launch_bounds__(64 * WS) void ds_forces_kernel(
   }
 }
 
-// ---- pair-symmetric form (the default from 2^15 bodies) --------------------
+// ---- pair-symmetric form (the default above 2^13 bodies) -------------------
 // The tiles, partial sums and reduction of the single-system symmetric kernel
 // (direct_sym_core.hpp) with double-single positions and Kahan-compensated
 // sums on both sides of every pair: the targets' sums per chunk, and the
@@ -480,7 +480,8 @@ constexpr int kIpl = 2, kWs = 8;           // one-sided shape
 constexpr int kSymIpl = 4, kSymW = 2;      // pair-symmetric shape, super-blocks of 512
 constexpr int kSymS = 64 * kSymIpl * kSymW;
 
-// -1: symmetric from 2^15 bodies when the call brings a workspace, 0: one-sided,
+// -1: symmetric above 2^13 bodies when the call brings a workspace (0.237 against
+// 0.293 ms at 12,288 bodies, 0.240 against 0.389 at 16,384; even at 8,192), 0: one-sided,
 // 1: symmetric (needs a workspace).
 int g_ds_kernel = -1;
 
@@ -536,7 +537,7 @@ int ds_forces_launch(nbx_direct* op, nbx_ds_particles* d, float G, float eps2,
   const int n = (int)p.count;
   if (g_ds_kernel == 1 && !op)
     return fail(NBX_ERR_INVALID, "the pair-symmetric double-single kernel needs a workspace");
-  const bool sym = op && (g_ds_kernel == 1 || (g_ds_kernel == -1 && n >= (1 << 15)));
+  const bool sym = op && (g_ds_kernel == 1 || (g_ds_kernel == -1 && n > (1 << 13)));
   if (sym) return ds_sym_launch(d, G, eps2, kick, op->work, s);
   dim3 grid(div_up(n, 64 * kIpl)), block(64 * kWs);
   KickArgs k{};
