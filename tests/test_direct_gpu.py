@@ -185,6 +185,19 @@ def test_uniform_sphere_262144_sample(nbx, orc):
     check_fp64(nbx, orc, h, 1.0, 0.005, targets=t, got=got)
 
 
+@pytest.mark.parametrize("n", [20000, 70000, 150000])
+def test_default_shape_by_size_fp64(nbx, orc, n):
+    # the default path at one size of each symmetric tier below the 262,144-body one
+    # (sym_ipl4_ws1, sym_ipl6_ws1, sym_ipl8_ws1): 1,024 seeded sample targets vs FP64
+    names = nbx.variant_names()
+    expect = {20000: "sym_ipl4_ws1", 70000: "sym_ipl6_ws1", 150000: "sym_ipl8_ws1"}[n]
+    assert names[nbx.lib.nbx_direct_variant_for(n)] == expect
+    h = mixed_masses(nbx.gen_plummer(n, seed=n), n)
+    t = np.sort(np.random.default_rng(n).choice(n, 1024, replace=False))
+    got = run_forces(nbx, h, 1.0, 0.01)
+    check_fp64(nbx, orc, h, 1.0, 0.01, targets=t, got=got)
+
+
 # ---------------------------------------------------------------- pair-symmetric kernel
 # Whole-system evaluations with each unordered pair computed once and applied to both
 # bodies (direct_sym.hip). Same FP64 bounds as the one-sided kernel.
